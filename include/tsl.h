@@ -1,0 +1,243 @@
+/*
+ * tsl.h -- C ABI of the MI355X (gfx950) MCL sensor update.
+ *
+ * This is the drop-in boundary for the hot path of uos/tsdf_localization:
+ * TSDF map upload, per-particle scan evaluation, normalisation + pose
+ * estimate, and systematic resampling.  Plain C types only: pointers, sizes,
+ * POD structs whose layouts equal the reference's (static-asserted in the
+ * implementation).  Every function returns TSL_OK or an error code and
+ * leaves a thread-local message in tsl_last_error().
+ *
+ * Threading: a handle may be used by one thread at a time.  Different
+ * handles (several maps, evaluators, filters) may be used concurrently; the
+ * library keeps no global device state, unlike the reference's file-scope
+ * __constant__/device globals (cuda_data.h:14-34).
+ *
+ * Streams: entry points taking `void* stream` enqueue asynchronously on that
+ * hipStream_t (NULL = the null stream) and do not synchronise.
+ */
+#ifndef TSL_H
+#define TSL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TSL_OK 0
+#define TSL_ERR_INVALID_ARGUMENT 1
+#define TSL_ERR_HIP 2               /* a HIP runtime call failed */
+#define TSL_ERR_NO_VALID_PARTICLE 3 /* weight sum is zero: "No particle is valid!" */
+#define TSL_ERR_OUT_OF_MEMORY 4
+#define TSL_ERR_CAPACITY 5          /* an output buffer is too small */
+#define TSL_ERR_UNSUPPORTED 6       /* input outside what this build handles */
+#define TSL_ERR_ADMISSION 7         /* serving: request refused by the HBM budget */
+
+/* ---- reference-compatible POD types ------------------------------------ */
+
+/* Particle (include/tsdf_localization/particle.h:26-55): pose
+ * (x, y, z, roll, pitch, yaw) and weight.  28 bytes. */
+typedef struct tsl_particle {
+    float first[6];
+    float second;
+} tsl_particle;
+
+/* CudaPoint (include/tsdf_localization/cuda/cuda_evaluator.h:22-39).  12 bytes. */
+typedef struct tsl_point {
+    float x, y, z;
+} tsl_point;
+
+/* CudaSubVoxelMap<float,float>::MapCoef
+ * (include/tsdf_localization/cuda/cuda_sub_voxel_map.h:22-50), same field
+ * order and types, 120 bytes; a reference map's coef() can be memcpy'd here. */
+typedef struct tsl_map_coef {
+    uint64_t dim_x, dim_y, dim_z;
+    float min_x, min_y, min_z;
+    float max_x, max_y, max_z;
+    float resolution;
+    float init_value;
+    uint64_t up_dim_x, up_dim_y, up_dim_z, up_dim_2;
+    uint64_t sub_dim, sub_dim_2;
+    uint64_t grid_occ_size, data_size;
+} tsl_map_coef;
+
+/* geometry_msgs::PoseWithCovariance as returned by Evaluator::evaluate
+ * (cuda_evaluator.h:102): position, orientation quaternion (x, y, z, w),
+ * row-major 6x6 covariance (the reference writes zeros on the diagonal). */
+typedef struct tsl_pose {
+    double position[3];
+    double orientation[4];
+    double covariance[36];
+} tsl_pose;
+
+typedef struct tsl_map tsl_map;
+typedef struct tsl_evaluator tsl_evaluator;
+typedef struct tsl_filter tsl_filter;
+
+/* ---- library ----------------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* tsl_last_error(void);
+
+/* Library ABI version (major*10000 + minor*100 + patch). */
+int tsl_version(void);
+
+/* Number of visible HIP devices.  No counterpart in the reference. */
+int tsl_device_count(int* count);
+
+/* Select the device that handles created afterwards on this thread use. */
+int tsl_set_device(int device);
+
+/* ---- map: CudaSubVoxelMap + the upload in CudaEvaluator's constructor --- */
+
+/* Constructor arithmetic of CudaSubVoxelMap
+ * (include/tsdf_localization/cuda/cuda_sub_voxel_map.tcc:5-45): dims,
+ * block dims and sub dims from the bounding box; data_size = 0. */
+int tsl_map_coef_init(float min_x, float min_y, float min_z, float max_x, float max_y,
+                      float max_z, float resolution, float init_value, tsl_map_coef* out);
+
+/* Upload a map given in the reference's representation: block table
+ * (grid_occ_size int32 offsets, -1 = empty block) and one float value per
+ * voxel (data_size floats), i.e. rawGridOcc()/rawData() of a
+ * CudaSubVoxelMap (cuda_sub_voxel_map.h:141,166).  Replaces the map upload of
+ * CudaEvaluator::CudaEvaluator (src/cuda/cuda_evaluator.cu:21-59).  Voxel
+ * values are stored on the device as 16-bit codes into a value palette
+ * (at most 4096 distinct values; TSL_ERR_UNSUPPORTED beyond). */
+int tsl_map_create(const tsl_map_coef* coef, const int32_t* grid_occ, const float* data,
+                   tsl_map** out);
+
+/* Upload a map given as TSDF distances in millimetres, one int16 per voxel
+ * (INT16_MIN = voxel never set, reads init_value), plus the likelihood
+ * parameters of createTSDFMap (include/tsdf_localization/map/map_util.h:17-154):
+ * a voxel with |tsdf| < truncation_mm gets (N(d; sigma))^3 computed with the
+ * reference's float/double arithmetic (map_util.h:68-73,124-126); others read
+ * init_value.  coef->init_value is used as given. */
+int tsl_map_create_tsdf(const tsl_map_coef* coef, const int32_t* grid_occ,
+                        const int16_t* tsdf_mm, int truncation_mm, float sigma, tsl_map** out);
+
+/* Build the block table and voxel array from a list of cells exactly as
+ * CudaSubVoxelMap::setData does (cuda_sub_voxel_map.tcc:171-231: cells are
+ * placed through the float lookup of their corner coordinates).  coef is
+ * updated (data_size).  grid_occ receives grid_occ_size entries; data
+ * receives data_size floats and must have room for data_capacity floats
+ * (TSL_ERR_CAPACITY, with coef->data_size set, if it does not).  Host only. */
+int tsl_map_set_data(tsl_map_coef* coef, const float* cell_xyz, const float* cell_value,
+                     size_t num_cells, int32_t* grid_occ, float* data, size_t data_capacity);
+
+/* Likelihood value of a valid TSDF cell and the map's init value
+ * (map_util.h:68-73,124-126).  Host only. */
+float tsl_likelihood_value(int16_t tsdf_mm, float sigma);
+float tsl_likelihood_init(float sigma);
+
+/* Device bytes held by the map. */
+int tsl_map_device_bytes(const tsl_map* map, uint64_t* bytes);
+
+int tsl_map_destroy(tsl_map* map);
+
+/* ---- evaluator: CudaEvaluator / TSDFEvaluator -------------------------- */
+
+/* CudaEvaluator(map, per_point, a_hit, a_range, a_max, max_range)
+ * (include/tsdf_localization/cuda/cuda_evaluator.h:112) together with the
+ * TSDFEvaluator constants (tsdf_evaluator.h:72-76).  per_point selects a
+ * point-split tile shape; the scoring is the reference CPU evaluator's in
+ * both cases (the reference's per-pair CUDA kernel tests the particle
+ * position instead of the point range, cuda_eval_particles.h:499). */
+int tsl_evaluator_create(tsl_map* map, int per_point, float a_hit, float a_range, float a_max,
+                         float max_range, tsl_evaluator** out);
+
+int tsl_evaluator_destroy(tsl_evaluator* ev);
+
+/* Evaluator::evaluate(particles, points, tf_matrix)
+ * (cuda_evaluator.h:102; src/cuda/cuda_evaluator.cu:118-428;
+ * CPU semantics src/evaluation/tsdf_evaluator.cpp:78-245).
+ * particles (host, in/out): on return particles[i].second holds the
+ * normalised weight.  points and tf_matrix (row-major 4x4, entries 0..11
+ * used) are read only.  No points: *out_pose is zeroed and particles are
+ * left untouched (cuda_evaluator.cu:122-125).  Zero weight sum:
+ * TSL_ERR_NO_VALID_PARTICLE.  Synchronous. */
+int tsl_evaluate(tsl_evaluator* ev, tsl_particle* particles, size_t num_particles,
+                 const tsl_point* points, size_t num_points, const float tf_matrix[16],
+                 tsl_pose* out_pose);
+
+/* Raw (unnormalised) per-particle weights of one evaluation, written to
+ * weights_out[num_particles]; particles are not modified.  Synchronous. */
+int tsl_evaluate_raw(tsl_evaluator* ev, const tsl_particle* particles, size_t num_particles,
+                     const tsl_point* points, size_t num_points, const float tf_matrix[16],
+                     float* weights_out);
+
+/* Voxel index that CudaSubVoxelMap::getIndex (cuda_sub_voxel_map.tcc:51-137)
+ * returns for every (particle, point) pair, row-major [particle][point]
+ * (coef.data_size = rejected).  Runs the production lookup on the device; for
+ * parity checking.  out has num_particles*num_points entries.  Synchronous. */
+int tsl_trace_indices(tsl_evaluator* ev, const tsl_particle* particles, size_t num_particles,
+                      const tsl_point* points, size_t num_points, const float tf_matrix[16],
+                      uint64_t* out);
+
+/* Pose matrices L[12] (tsdf_evaluator.cpp:102-145) computed on the device. */
+int tsl_pose_matrices(const tsl_particle* particles, size_t num_particles,
+                      const float tf_matrix[16], float* out);
+
+/* ---- resampling: SystematicResampler ----------------------------------- */
+
+/* SystematicResampler::resample
+ * (include/tsdf_localization/resampling/novel_resampling.h:38-73) on the
+ * device: in[M] are particles with normalised weights in .second, u0 the
+ * uniform draw from [0, 1/M).  Writes the emitted particles to out (room
+ * for out_capacity) and their source indices to out_indices (may be NULL),
+ * and the number emitted (which, as in the reference, can differ from M) to
+ * *out_count.  Indices are bit-exact with the reference's double running
+ * sum and float U recurrence.  Synchronous. */
+int tsl_resample_systematic(const tsl_particle* in, size_t M, float u0, tsl_particle* out,
+                            size_t out_capacity, int32_t* out_indices, size_t* out_count);
+
+/* Upper bound on the number of particles tsl_resample_systematic can emit
+ * for M particles and draw u0 (for sizing out). */
+int tsl_resample_capacity(size_t M, float u0, size_t* capacity);
+
+/* ---- device-resident filter (particle set and scan live in HBM) -------- */
+
+/* A filter owns a device-resident particle set (capacity particles) bound
+ * to an evaluator; the update path below never copies particles to the
+ * host.  No direct counterpart in the reference, whose particles live in
+ * ParticleCloud on the host (particle_cloud.h:46-259). */
+int tsl_filter_create(tsl_evaluator* ev, size_t capacity, tsl_filter** out);
+int tsl_filter_destroy(tsl_filter* f);
+
+/* Copy particles in / out (synchronous). */
+int tsl_filter_set_particles(tsl_filter* f, const tsl_particle* particles, size_t n);
+int tsl_filter_get_particles(tsl_filter* f, tsl_particle* particles, size_t capacity, size_t* n);
+
+/* Upload a scan and its scanner->base transform (synchronous). */
+int tsl_filter_set_scan(tsl_filter* f, const tsl_point* points, size_t n,
+                        const float tf_matrix[16]);
+
+/* Sensor update on `stream`: pose matrices, evaluation, normalisation and
+ * pose sums; particles' .second become the normalised weights.  Async. */
+int tsl_filter_update_async(tsl_filter* f, void* stream);
+
+/* Systematic resampling of the resident set with draw u0 on `stream`; the
+ * set is replaced by the emitted particles.  Async. */
+int tsl_filter_resample_async(tsl_filter* f, float u0, void* stream);
+
+/* Wait for `stream`, then read the pose estimate of the last update and the
+ * current particle count.  Returns TSL_ERR_NO_VALID_PARTICLE if the last
+ * update had a zero weight sum. */
+int tsl_filter_sync(tsl_filter* f, void* stream, tsl_pose* pose, size_t* count);
+
+/* ---- streams and timing (plumbing for benchmarks) ---------------------- */
+int tsl_stream_create(void** stream);
+int tsl_stream_destroy(void* stream);
+int tsl_stream_synchronize(void* stream);
+int tsl_event_create(void** event);
+int tsl_event_destroy(void* event);
+int tsl_event_record(void* event, void* stream);
+int tsl_event_elapsed_ms(void* start, void* stop, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSL_H */

@@ -1,0 +1,16 @@
+"""MI355X (gfx950) MCL sensor update for uos/tsdf_localization.
+
+The hot path (pose transform of every scan point by every particle, TSDF
+voxel lookup, score, per-particle sum, normalisation, systematic resampling)
+runs in hand-written HIP kernels in libtsl.so behind the C ABI of
+include/tsl.h.  This package is the Python mirror of the reference's
+evaluator/resampler interface over that ABI.
+"""
+from .evaluator import (CudaSubVoxelMap, DeviceFilter, Event, ParticleCloud, Stream, SystematicResampler,
+                        TSDFEvaluator, coef_init, likelihood_init, likelihood_value, pose_matrices,
+                        resample_systematic)
+from ._lib import TslError, device_count
+
+__all__ = ["CudaSubVoxelMap", "DeviceFilter", "Event", "ParticleCloud", "Stream", "SystematicResampler",
+           "TSDFEvaluator", "TslError", "coef_init", "device_count", "likelihood_init", "likelihood_value",
+           "pose_matrices", "resample_systematic"]

@@ -1,0 +1,127 @@
+"""ctypes binding of libtsl.so (the C ABI declared in include/tsl.h).
+
+The library is loaded from this package directory only.  There is no
+fallback: if libtsl.so is missing or fails to load, importing this module
+raises, so nothing can silently run a non-HIP path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtsl.so")
+
+TSL_OK = 0
+TSL_ERR_INVALID_ARGUMENT = 1
+TSL_ERR_HIP = 2
+TSL_ERR_NO_VALID_PARTICLE = 3
+TSL_ERR_OUT_OF_MEMORY = 4
+TSL_ERR_CAPACITY = 5
+TSL_ERR_UNSUPPORTED = 6
+TSL_ERR_ADMISSION = 7
+
+
+class MapCoef(ctypes.Structure):
+    """tsl_map_coef == CudaSubVoxelMap<float,float>::MapCoef (cuda_sub_voxel_map.h:22-50)."""
+    _fields_ = [
+        ("dim_x", ctypes.c_uint64), ("dim_y", ctypes.c_uint64), ("dim_z", ctypes.c_uint64),
+        ("min_x", ctypes.c_float), ("min_y", ctypes.c_float), ("min_z", ctypes.c_float),
+        ("max_x", ctypes.c_float), ("max_y", ctypes.c_float), ("max_z", ctypes.c_float),
+        ("resolution", ctypes.c_float), ("init_value", ctypes.c_float),
+        ("up_dim_x", ctypes.c_uint64), ("up_dim_y", ctypes.c_uint64), ("up_dim_z", ctypes.c_uint64),
+        ("up_dim_2", ctypes.c_uint64),
+        ("sub_dim", ctypes.c_uint64), ("sub_dim_2", ctypes.c_uint64),
+        ("grid_occ_size", ctypes.c_uint64), ("data_size", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Pose(ctypes.Structure):
+    """tsl_pose == geometry_msgs::PoseWithCovariance (position, quaternion xyzw, 6x6 covariance)."""
+    _fields_ = [("position", ctypes.c_double * 3), ("orientation", ctypes.c_double * 4),
+                ("covariance", ctypes.c_double * 36)]
+
+
+assert ctypes.sizeof(MapCoef) == 120
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                      "(or python -m tsdf_localization_amd.build)")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+_P = ctypes.c_void_p
+_S = ctypes.c_size_t
+_F = ctypes.c_float
+_I = ctypes.c_int
+
+_SIGS = {
+    "tsl_last_error": (ctypes.c_char_p, []),
+    "tsl_version": (_I, []),
+    "tsl_device_count": (_I, [ctypes.POINTER(_I)]),
+    "tsl_set_device": (_I, [_I]),
+    "tsl_map_coef_init": (_I, [_F, _F, _F, _F, _F, _F, _F, _F, ctypes.POINTER(MapCoef)]),
+    "tsl_map_create": (_I, [ctypes.POINTER(MapCoef), _P, _P, ctypes.POINTER(_P)]),
+    "tsl_map_create_tsdf": (_I, [ctypes.POINTER(MapCoef), _P, _P, _I, _F, ctypes.POINTER(_P)]),
+    "tsl_map_set_data": (_I, [ctypes.POINTER(MapCoef), _P, _P, _S, _P, _P, _S]),
+    "tsl_likelihood_value": (_F, [ctypes.c_int16, _F]),
+    "tsl_likelihood_init": (_F, [_F]),
+    "tsl_map_device_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "tsl_map_destroy": (_I, [_P]),
+    "tsl_evaluator_create": (_I, [_P, _I, _F, _F, _F, _F, ctypes.POINTER(_P)]),
+    "tsl_evaluator_destroy": (_I, [_P]),
+    "tsl_evaluate": (_I, [_P, _P, _S, _P, _S, _P, ctypes.POINTER(Pose)]),
+    "tsl_evaluate_raw": (_I, [_P, _P, _S, _P, _S, _P, _P]),
+    "tsl_trace_indices": (_I, [_P, _P, _S, _P, _S, _P, _P]),
+    "tsl_pose_matrices": (_I, [_P, _S, _P, _P]),
+    "tsl_resample_systematic": (_I, [_P, _S, _F, _P, _S, _P, ctypes.POINTER(_S)]),
+    "tsl_resample_capacity": (_I, [_S, _F, ctypes.POINTER(_S)]),
+    "tsl_filter_create": (_I, [_P, _S, ctypes.POINTER(_P)]),
+    "tsl_filter_destroy": (_I, [_P]),
+    "tsl_filter_set_particles": (_I, [_P, _P, _S]),
+    "tsl_filter_get_particles": (_I, [_P, _P, _S, ctypes.POINTER(_S)]),
+    "tsl_filter_set_scan": (_I, [_P, _P, _S, _P]),
+    "tsl_filter_update_async": (_I, [_P, _P]),
+    "tsl_filter_resample_async": (_I, [_P, _F, _P]),
+    "tsl_filter_sync": (_I, [_P, _P, ctypes.POINTER(Pose), ctypes.POINTER(_S)]),
+    "tsl_stream_create": (_I, [ctypes.POINTER(_P)]),
+    "tsl_stream_destroy": (_I, [_P]),
+    "tsl_stream_synchronize": (_I, [_P]),
+    "tsl_event_create": (_I, [ctypes.POINTER(_P)]),
+    "tsl_event_destroy": (_I, [_P]),
+    "tsl_event_record": (_I, [_P, _P]),
+    "tsl_event_elapsed_ms": (_I, [_P, _P, ctypes.POINTER(_F)]),
+}
+
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+class TslError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+def check(rc):
+    """Raise TslError for a non-zero status, with the library's message."""
+    if rc != TSL_OK:
+        msg = lib.tsl_last_error().decode(errors="replace")
+        raise TslError(rc, msg or f"tsl error {rc}")
+    return rc
+
+
+def ptr(a):
+    """Data pointer of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib.tsl_device_count(ctypes.byref(n)))
+    return n.value

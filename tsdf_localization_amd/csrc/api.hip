@@ -1,0 +1,696 @@
+// C ABI of the MI355X sensor update: handles, map upload, host-buffer
+// drop-in entry points (synchronous) and the device-resident filter API.
+#include "tsl_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+static_assert(sizeof(tsl_particle) == 28, "Particle is 7 floats (particle.h:26-55)");
+static_assert(sizeof(tsl_point) == 12, "CudaPoint is 3 floats (cuda_evaluator.h:22-39)");
+static_assert(sizeof(tsl_map_coef) == 120, "MapCoef layout (cuda_sub_voxel_map.h:22-50)");
+static_assert(offsetof(tsl_map_coef, min_x) == 24 && offsetof(tsl_map_coef, up_dim_x) == 56,
+              "MapCoef field offsets");
+
+namespace tsl {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+uint32_t grid_threshold(uint64_t dim, float res)
+{
+    // predicate is monotone in u: trunc(fl(fl(u)/res)) >= dim
+    auto pass = [&](uint32_t u) { return cvt_f32_u64(cvt_u64_f32(u) / res) >= dim; };
+    uint32_t lo = 0, hi = kFastLimit;  // answer in [lo, hi]; hi means "none below 2^23"
+    if (!pass(hi - 1)) return hi;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (pass(mid)) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+int launch_resample_buffers(const tsl_particle* d_in, size_t M, float u0, size_t cap, double* d_s,
+                            float* d_u, int32_t* d_idx, tsl_particle* d_out, uint32_t* d_count,
+                            uint32_t* d_status, hipStream_t stream);
+int launch_set_count(tsl_filter* f, uint32_t n, hipStream_t stream);
+
+int filter_reserve_particles(tsl_filter* f, size_t n)
+{
+    if (n <= f->capacity && f->particles.ptr) return TSL_OK;
+    // keep the resident set when growing
+    DevBuf grown;
+    TSL_TRY(grown.reserve(sizeof(tsl_particle) * n));
+    if (f->particles.ptr && f->count_bound)
+        TSL_HIP_CHECK(hipMemcpy(grown.ptr, f->particles.ptr, sizeof(tsl_particle) * f->count_bound,
+                                hipMemcpyDeviceToDevice));
+    std::swap(grown.ptr, f->particles.ptr);
+    std::swap(grown.bytes, f->particles.bytes);
+    TSL_TRY(f->particles_alt.reserve(sizeof(tsl_particle) * n));
+    TSL_TRY(f->L.reserve(sizeof(float) * 12 * n));
+    TSL_TRY(f->raw.reserve(sizeof(float) * n));
+    TSL_TRY(f->cumsum.reserve(sizeof(double) * n));
+    f->partial.release();
+    f->capacity = n;
+    return TSL_OK;
+}
+
+int filter_reserve_points(tsl_filter* f, size_t n)
+{
+    TSL_TRY(f->points.reserve(16 * (n ? n : 1)));
+    TSL_TRY(f->points_in.reserve(sizeof(tsl_point) * (n ? n : 1)));
+    return TSL_OK;
+}
+
+static FilterState* host_state(tsl_filter* f) { return static_cast<FilterState*>(f->host_state); }
+
+// pose from the ten sums: weighted mean position, circular mean of each
+// angle (tsdf_evaluator.cpp:219-221), quaternion as tf2::Quaternion::setRPY
+// (roll about x, pitch about y, yaw about z, q = qz(yaw) qy(pitch) qx(roll)).
+static void pose_from_stats(const double* st, tsl_pose* pose)
+{
+    std::memset(pose, 0, sizeof(*pose));
+    pose->position[0] = (float)st[1];
+    pose->position[1] = (float)st[2];
+    pose->position[2] = (float)st[3];
+    const double roll = (float)std::atan2(st[4], st[5]);
+    const double pitch = (float)std::atan2(st[6], st[7]);
+    const double yaw = (float)std::atan2(st[8], st[9]);
+    const double hr = roll * 0.5, hp = pitch * 0.5, hy = yaw * 0.5;
+    const double cr = std::cos(hr), sr = std::sin(hr);
+    const double cp = std::cos(hp), sp = std::sin(hp);
+    const double cy = std::cos(hy), sy = std::sin(hy);
+    pose->orientation[0] = sr * cp * cy - cr * sp * sy;
+    pose->orientation[1] = cr * sp * cy + sr * cp * sy;
+    pose->orientation[2] = cr * cp * sy - sr * sp * cy;
+    pose->orientation[3] = cr * cp * cy + sr * sp * sy;
+}
+
+}  // namespace tsl
+
+using namespace tsl;
+
+extern "C" {
+
+const char* tsl_last_error(void) { return g_last_error.c_str(); }
+
+int tsl_version(void) { return 10000; }
+
+int tsl_device_count(int* count)
+{
+    if (!count) return set_error(TSL_ERR_INVALID_ARGUMENT, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return TSL_OK;
+}
+
+int tsl_set_device(int device)
+{
+    TSL_HIP_CHECK(hipSetDevice(device));
+    return TSL_OK;
+}
+
+int tsl_map_coef_init(float min_x, float min_y, float min_z, float max_x, float max_y, float max_z,
+                      float resolution, float init_value, tsl_map_coef* c)
+{
+    if (!c) return set_error(TSL_ERR_INVALID_ARGUMENT, "coef is NULL");
+    // cuda_sub_voxel_map.tcc:16-34 (float arithmetic, std::ceil, size_t conversion)
+    const float sub_voxel_size = 1.0f;
+    c->dim_x = cvt_f32_u64(std::ceil(std::fabs(max_x - min_x) / resolution));
+    c->dim_y = cvt_f32_u64(std::ceil(std::fabs(max_y - min_y) / resolution));
+    c->dim_z = cvt_f32_u64(std::ceil(std::fabs(max_z - min_z) / resolution));
+    c->min_x = min_x; c->min_y = min_y; c->min_z = min_z;
+    c->max_x = max_x; c->max_y = max_y; c->max_z = max_z;
+    c->resolution = resolution;
+    c->init_value = init_value;
+    c->up_dim_x = cvt_f32_u64(std::ceil(std::fabs(max_x - min_x) / sub_voxel_size));
+    c->up_dim_y = cvt_f32_u64(std::ceil(std::fabs(max_y - min_y) / sub_voxel_size));
+    c->up_dim_z = cvt_f32_u64(std::ceil(std::fabs(max_z - min_z) / sub_voxel_size));
+    c->up_dim_2 = c->up_dim_x * c->up_dim_y;
+    c->sub_dim = cvt_f32_u64(std::ceil(sub_voxel_size / resolution));
+    c->sub_dim_2 = c->sub_dim * c->sub_dim;
+    c->grid_occ_size = c->up_dim_x * c->up_dim_y * c->up_dim_z;
+    c->data_size = 0;
+    return TSL_OK;
+}
+
+static int validate_coef(const tsl_map_coef* c)
+{
+    if (!(c->resolution > 0.0f) || !std::isfinite(c->resolution))
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "resolution must be positive and finite");
+    if (c->grid_occ_size != c->up_dim_x * c->up_dim_y * c->up_dim_z || c->grid_occ_size == 0)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "grid_occ_size does not match the block dims");
+    if (c->up_dim_2 != c->up_dim_x * c->up_dim_y || c->sub_dim_2 != c->sub_dim * c->sub_dim)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "inconsistent derived dims");
+    if (c->data_size > 0x7fffffffull || c->grid_occ_size > 0x7fffffffull)
+        return set_error(TSL_ERR_UNSUPPORTED, "map exceeds the int32 block table of the reference (%llu voxels)",
+                         (unsigned long long)c->data_size);
+    return TSL_OK;
+}
+
+static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uint16_t>& codes,
+                      const std::vector<float>& palette)
+{
+    const tsl_map_coef& c = m->coef;
+    TSL_TRY(m->grid_occ.reserve(sizeof(int32_t) * c.grid_occ_size));
+    TSL_TRY(m->codes.reserve(sizeof(uint16_t) * (c.data_size ? c.data_size : 1)));
+    TSL_TRY(m->palette.reserve(sizeof(float) * (palette.empty() ? 1 : palette.size())));
+    TSL_HIP_CHECK(hipMemcpy(m->grid_occ.ptr, grid_occ, sizeof(int32_t) * c.grid_occ_size,
+                            hipMemcpyHostToDevice));
+    if (c.data_size)
+        TSL_HIP_CHECK(hipMemcpy(m->codes.ptr, codes.data(), sizeof(uint16_t) * c.data_size,
+                                hipMemcpyHostToDevice));
+    if (!palette.empty())
+        TSL_HIP_CHECK(hipMemcpy(m->palette.ptr, palette.data(), sizeof(float) * palette.size(),
+                                hipMemcpyHostToDevice));
+    DevMap& d = m->dev;
+    d.grid_occ = m->grid_occ.as<int32_t>();
+    d.codes = m->codes.as<uint16_t>();
+    d.palette = m->palette.as<float>();
+    d.palette_size = (uint32_t)palette.size();
+    d.min_x = c.min_x; d.min_y = c.min_y; d.min_z = c.min_z;
+    d.res = c.resolution;
+    d.init = c.init_value;
+    d.gthr_x = grid_threshold(c.dim_x, c.resolution);
+    d.gthr_y = grid_threshold(c.dim_y, c.resolution);
+    d.gthr_z = grid_threshold(c.dim_z, c.resolution);
+    d.dim_x = c.dim_x; d.dim_y = c.dim_y; d.dim_z = c.dim_z;
+    d.up_dim_x = c.up_dim_x;
+    d.up_dim_2 = c.up_dim_2;
+    d.sub_dim = c.sub_dim;
+    d.sub_dim_2 = c.sub_dim_2;
+    d.grid_occ_size = c.grid_occ_size;
+    d.data_size = c.data_size;
+    return TSL_OK;
+}
+
+int tsl_map_create(const tsl_map_coef* coef, const int32_t* grid_occ, const float* data, tsl_map** out)
+{
+    if (!coef || !grid_occ || !out || (coef->data_size && !data))
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    TSL_TRY(validate_coef(coef));
+    // value palette: distinct bit patterns in first-seen order
+    std::vector<float> palette;
+    std::vector<uint16_t> codes(coef->data_size);
+    std::unordered_map<uint32_t, uint16_t> seen;
+    for (uint64_t i = 0; i < coef->data_size; ++i) {
+        uint32_t bits;
+        std::memcpy(&bits, &data[i], 4);
+        auto it = seen.find(bits);
+        if (it == seen.end()) {
+            if (palette.size() >= kMaxPalette)
+                return set_error(TSL_ERR_UNSUPPORTED, "map holds more than %u distinct voxel values", kMaxPalette);
+            it = seen.emplace(bits, (uint16_t)palette.size()).first;
+            palette.push_back(data[i]);
+        }
+        codes[i] = it->second;
+    }
+    tsl_map* m = new (std::nothrow) tsl_map();
+    if (!m) return set_error(TSL_ERR_OUT_OF_MEMORY, "out of host memory");
+    (void)hipGetDevice(&m->device);
+    m->coef = *coef;
+    int rc = map_upload(m, grid_occ, codes, palette);
+    if (rc != TSL_OK) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return TSL_OK;
+}
+
+float tsl_likelihood_init(float sigma)
+{
+    // map_util.h:68-71
+    const float sigma_quad = sigma * sigma;
+    const float e = std::exp((float)(-(10.0 * 10.0) / (double)sigma_quad / 2));
+    const float d = std::sqrt((float)((double)(2 * sigma_quad) * M_PI));
+    float init = e / d;
+    init = init * init * init;
+    return init;
+}
+
+float tsl_likelihood_value(int16_t tsdf_mm, float sigma)
+{
+    // map_util.h:108,124-126: value is a double, expf/sqrtf are float
+    const float sigma_quad = sigma * sigma;
+    const float tsdf_value = (float)tsdf_mm;
+    double value = (double)tsdf_value * 0.001;
+    const float e = expf((float)(-(value * value) / (double)sigma_quad / 2));
+    const float d = sqrtf((float)((double)(2 * sigma_quad) * M_PI));
+    value = (double)(e / d);
+    value = value * value * value;
+    return (float)value;
+}
+
+int tsl_map_create_tsdf(const tsl_map_coef* coef, const int32_t* grid_occ, const int16_t* tsdf_mm,
+                        int truncation_mm, float sigma, tsl_map** out)
+{
+    if (!coef || !grid_occ || !out || (coef->data_size && !tsdf_mm))
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    TSL_TRY(validate_coef(coef));
+    if (truncation_mm < 1 || 2 * (uint32_t)truncation_mm > kMaxPalette)
+        return set_error(TSL_ERR_UNSUPPORTED, "truncation_mm must be in [1, %u]", kMaxPalette / 2);
+    const int t = truncation_mm;
+    std::vector<float> palette(2 * t);
+    for (int mm = -(t - 1); mm <= t - 1; ++mm)
+        palette[mm + t - 1] = tsl_likelihood_value((int16_t)mm, sigma);
+    const uint16_t init_code = (uint16_t)(2 * t - 1);
+    palette[init_code] = coef->init_value;
+    std::vector<uint16_t> codes(coef->data_size);
+    for (uint64_t i = 0; i < coef->data_size; ++i) {
+        const int v = tsdf_mm[i];
+        codes[i] = (v != INT16_MIN && v > -t && v < t) ? (uint16_t)(v + t - 1) : init_code;
+    }
+    tsl_map* m = new (std::nothrow) tsl_map();
+    if (!m) return set_error(TSL_ERR_OUT_OF_MEMORY, "out of host memory");
+    (void)hipGetDevice(&m->device);
+    m->coef = *coef;
+    int rc = map_upload(m, grid_occ, codes, palette);
+    if (rc != TSL_OK) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return TSL_OK;
+}
+
+// CudaSubVoxelMap::getIndex on the host, used by the setData replay.
+static uint64_t host_get_index(const tsl_map_coef& c, const int32_t* grid_occ, float x, float y, float z)
+{
+    uint64_t gx = cvt_f32_u64(x - c.min_x), gy = cvt_f32_u64(y - c.min_y), gz = cvt_f32_u64(z - c.min_z);
+    gx = cvt_f32_u64(cvt_u64_f32(gx) / c.resolution);
+    gy = cvt_f32_u64(cvt_u64_f32(gy) / c.resolution);
+    gz = cvt_f32_u64(cvt_u64_f32(gz) / c.resolution);
+    if (gx >= c.dim_x || gy >= c.dim_y || gz >= c.dim_z) return c.data_size;
+    const float ox = x - c.min_x, oy = y - c.min_y, oz = z - c.min_z;
+    const uint64_t ux = cvt_f32_u64(ox / 1.0f), uy = cvt_f32_u64(oy / 1.0f), uz = cvt_f32_u64(oz / 1.0f);
+    const float spx = ox - cvt_u64_f32(ux) * 1.0f, spy = oy - cvt_u64_f32(uy) * 1.0f,
+                spz = oz - cvt_u64_f32(uz) * 1.0f;
+    const uint64_t up = ux + uy * c.up_dim_x + uz * c.up_dim_2;
+    if (up >= c.grid_occ_size) return c.data_size;
+    const int32_t s = grid_occ[up];
+    if (s < 0) return c.data_size;
+    return (uint64_t)(int64_t)s + cvt_f32_u64(spx / c.resolution) +
+           cvt_f32_u64(spy / c.resolution) * c.sub_dim + cvt_f32_u64(spz / c.resolution) * c.sub_dim_2;
+}
+
+int tsl_map_set_data(tsl_map_coef* c, const float* cell_xyz, const float* cell_value, size_t num_cells,
+                     int32_t* grid_occ, float* data, size_t data_capacity)
+{
+    if (!c || !grid_occ || (num_cells && (!cell_xyz || !cell_value)))
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    // cuda_sub_voxel_map.tcc:171-231
+    for (uint64_t i = 0; i < c->grid_occ_size; ++i) grid_occ[i] = -1;
+    if (num_cells == 0) return TSL_OK;  // :173-176 leaves data untouched
+    for (size_t i = 0; i < num_cells; ++i) {
+        const float ox = cell_xyz[3 * i] - c->min_x, oy = cell_xyz[3 * i + 1] - c->min_y,
+                    oz = cell_xyz[3 * i + 2] - c->min_z;
+        const uint64_t ux = cvt_f32_u64(ox / 1.0f), uy = cvt_f32_u64(oy / 1.0f), uz = cvt_f32_u64(oz / 1.0f);
+        const uint64_t up = ux + uy * c->up_dim_x + uz * c->up_dim_x * c->up_dim_y;
+        if (up >= c->grid_occ_size) return set_error(TSL_ERR_INVALID_ARGUMENT, "Upper voxel index overflow!");
+        grid_occ[up] = 0;
+    }
+    int32_t offset = 0;
+    const int32_t sub_size = (int32_t)(c->sub_dim * c->sub_dim * c->sub_dim);
+    for (uint64_t i = 0; i < c->grid_occ_size; ++i) {
+        if (grid_occ[i] >= 0) {
+            grid_occ[i] = offset;
+            offset += sub_size;
+        }
+    }
+    c->data_size = (uint64_t)(int64_t)offset;
+    if (!data || data_capacity < c->data_size)
+        return set_error(TSL_ERR_CAPACITY, "data needs %llu floats", (unsigned long long)c->data_size);
+    for (uint64_t i = 0; i < c->data_size; ++i) data[i] = c->init_value;
+    for (size_t i = 0; i < num_cells; ++i) {
+        const uint64_t idx = host_get_index(*c, grid_occ, cell_xyz[3 * i], cell_xyz[3 * i + 1], cell_xyz[3 * i + 2]);
+        if (idx < c->data_size) data[idx] = cell_value[i];
+    }
+    return TSL_OK;
+}
+
+int tsl_map_device_bytes(const tsl_map* m, uint64_t* bytes)
+{
+    if (!m || !bytes) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    *bytes = m->grid_occ.bytes + m->codes.bytes + m->palette.bytes;
+    return TSL_OK;
+}
+
+int tsl_map_destroy(tsl_map* m)
+{
+    if (m) {
+        (void)hipSetDevice(m->device);
+        delete m;
+    }
+    return TSL_OK;
+}
+
+int tsl_filter_create(tsl_evaluator* ev, size_t capacity, tsl_filter** out)
+{
+    if (!ev || !out) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    TSL_HIP_CHECK(hipSetDevice(ev->device));
+    tsl_filter* f = new (std::nothrow) tsl_filter();
+    if (!f) return set_error(TSL_ERR_OUT_OF_MEMORY, "out of host memory");
+    f->ev = ev;
+    f->device = ev->device;
+    int rc = f->state.reserve(sizeof(FilterState));
+    if (rc == TSL_OK && hipHostMalloc(&f->host_state, sizeof(FilterState), hipHostMallocDefault) != hipSuccess)
+        rc = set_error(TSL_ERR_OUT_OF_MEMORY, "hipHostMalloc failed");
+    if (rc == TSL_OK && hipMemset(f->state.ptr, 0, sizeof(FilterState)) != hipSuccess)
+        rc = set_error(TSL_ERR_HIP, "hipMemset failed");
+    if (rc == TSL_OK) rc = filter_reserve_particles(f, capacity ? capacity : 1);
+    if (rc == TSL_OK) rc = filter_reserve_points(f, 1);
+    if (rc != TSL_OK) {
+        tsl_filter_destroy(f);
+        return rc;
+    }
+    std::memset(f->host_state, 0, sizeof(FilterState));
+    *out = f;
+    return TSL_OK;
+}
+
+int tsl_filter_destroy(tsl_filter* f)
+{
+    if (!f) return TSL_OK;
+    (void)hipSetDevice(f->device);
+    (void)hipDeviceSynchronize();
+    if (f->host_state) (void)hipHostFree(f->host_state);
+    delete f;
+    return TSL_OK;
+}
+
+int tsl_filter_set_particles(tsl_filter* f, const tsl_particle* p, size_t n)
+{
+    if (!f || (n && !p)) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n > 0x7fffffffull) return set_error(TSL_ERR_UNSUPPORTED, "too many particles");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    f->count_bound = 0;
+    TSL_TRY(filter_reserve_particles(f, n ? n : 1));
+    if (n) TSL_HIP_CHECK(hipMemcpy(f->particles.ptr, p, sizeof(tsl_particle) * n, hipMemcpyHostToDevice));
+    f->count_bound = n;
+    TSL_TRY(launch_set_count(f, (uint32_t)n, 0));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    host_state(f)->count = (uint32_t)n;
+    return TSL_OK;
+}
+
+int tsl_filter_get_particles(tsl_filter* f, tsl_particle* p, size_t capacity, size_t* n)
+{
+    if (!f || !n) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    TSL_HIP_CHECK(hipMemcpy(f->host_state, f->state.ptr, sizeof(FilterState), hipMemcpyDeviceToHost));
+    const size_t cnt = host_state(f)->count;
+    *n = cnt;
+    if (cnt > capacity) return set_error(TSL_ERR_CAPACITY, "filter holds %zu particles", cnt);
+    if (cnt) TSL_HIP_CHECK(hipMemcpy(p, f->particles.ptr, sizeof(tsl_particle) * cnt, hipMemcpyDeviceToHost));
+    return TSL_OK;
+}
+
+int tsl_filter_set_scan(tsl_filter* f, const tsl_point* pts, size_t n, const float tf[16])
+{
+    if (!f || !tf || (n && !pts)) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n > 0x7fffffffull) return set_error(TSL_ERR_UNSUPPORTED, "too many points");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    TSL_TRY(filter_reserve_points(f, n));
+    if (n) TSL_HIP_CHECK(hipMemcpy(f->points_in.ptr, pts, sizeof(tsl_point) * n, hipMemcpyHostToDevice));
+    f->num_points = n;
+    std::memcpy(f->tf, tf, sizeof(float) * 16);
+    TSL_TRY(launch_prep_points(f, 0));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    return TSL_OK;
+}
+
+int tsl_filter_update_async(tsl_filter* f, void* stream)
+{
+    if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
+    if (f->num_points == 0) return TSL_OK;  // empty scan: nothing to do (cuda_evaluator.cu:122-125)
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    return launch_update(f, static_cast<hipStream_t>(stream));
+}
+
+int tsl_filter_resample_async(tsl_filter* f, float u0, void* stream)
+{
+    if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // M must be known exactly on the host to build the draw table
+    FilterState* hs = host_state(f);
+    TSL_HIP_CHECK(hipMemcpyAsync(hs, f->state.ptr, sizeof(FilterState), hipMemcpyDeviceToHost, s));
+    TSL_HIP_CHECK(hipStreamSynchronize(s));
+    const size_t M = hs->count;
+    if (M == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "filter holds no particles");
+    size_t cap = 0;
+    TSL_TRY(resample_capacity(M, u0, &cap));
+    if (cap > f->capacity) {
+        TSL_TRY(filter_reserve_particles(f, cap));
+    }
+    TSL_TRY(f->ubuf.reserve(sizeof(float) * cap));
+    FilterState* st = f->state.as<FilterState>();
+    TSL_TRY(launch_resample_buffers(f->particles.as<tsl_particle>(), M, u0, cap, f->cumsum.as<double>(),
+                                    f->ubuf.as<float>(), nullptr, f->particles_alt.as<tsl_particle>(),
+                                    &st->count, &st->status, s));
+    std::swap(f->particles.ptr, f->particles_alt.ptr);
+    std::swap(f->particles.bytes, f->particles_alt.bytes);
+    f->count_bound = cap;
+    return TSL_OK;
+}
+
+int tsl_filter_sync(tsl_filter* f, void* stream, tsl_pose* pose, size_t* count)
+{
+    if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    FilterState* hs = host_state(f);
+    TSL_HIP_CHECK(hipMemcpyAsync(hs, f->state.ptr, sizeof(FilterState), hipMemcpyDeviceToHost, s));
+    TSL_HIP_CHECK(hipStreamSynchronize(s));
+    if (count) *count = hs->count;
+    f->count_bound = hs->count;
+    if (pose) pose_from_stats(hs->stats, pose);
+    if (hs->status == 1u) return set_error(TSL_ERR_NO_VALID_PARTICLE, "No particle is valid!");
+    if (hs->status == 2u) return set_error(TSL_ERR_CAPACITY, "resampling emitted more particles than expected");
+    return TSL_OK;
+}
+
+int tsl_evaluator_create(tsl_map* map, int per_point, float a_hit, float a_range, float a_max,
+                         float max_range, tsl_evaluator** out)
+{
+    if (!map || !out) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    tsl_evaluator* ev = new (std::nothrow) tsl_evaluator();
+    if (!ev) return set_error(TSL_ERR_OUT_OF_MEMORY, "out of host memory");
+    ev->map = map;
+    ev->device = map->device;
+    ev->per_point = per_point;
+    ev->a_hit = a_hit;
+    ev->a_range = a_range;
+    ev->a_max = a_max;
+    ev->max_range = max_range;
+    // tsdf_evaluator.h:75: inv_max_range_(1.0 / max_range), max_range_squared_(max_range * max_range)
+    ev->inv_max_range = (float)(1.0 / (double)max_range);
+    ev->max_range_squared = max_range * max_range;
+    ev->range_term = a_range * ev->inv_max_range;
+    *out = ev;
+    return TSL_OK;
+}
+
+int tsl_evaluator_destroy(tsl_evaluator* ev)
+{
+    if (!ev) return TSL_OK;
+    if (ev->scratch) tsl_filter_destroy(ev->scratch);
+    delete ev;
+    return TSL_OK;
+}
+
+static int scratch_filter(tsl_evaluator* ev, size_t P, tsl_filter** f)
+{
+    if (!ev->scratch) TSL_TRY(tsl_filter_create(ev, P ? P : 1, &ev->scratch));
+    *f = ev->scratch;
+    return TSL_OK;
+}
+
+int tsl_evaluate(tsl_evaluator* ev, tsl_particle* particles, size_t P, const tsl_point* points, size_t N,
+                 const float tf[16], tsl_pose* out_pose)
+{
+    if (!ev || !out_pose || !tf || (P && !particles) || (N && !points))
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    clear_error();
+    if (N == 0) {  // cuda_evaluator.cu:122-125
+        std::memset(out_pose, 0, sizeof(*out_pose));
+        return TSL_OK;
+    }
+    if (P == 0) return set_error(TSL_ERR_NO_VALID_PARTICLE, "No particle is valid!");
+    tsl_filter* f;
+    TSL_TRY(scratch_filter(ev, P, &f));
+    TSL_TRY(tsl_filter_set_particles(f, particles, P));
+    TSL_TRY(tsl_filter_set_scan(f, points, N, tf));
+    TSL_TRY(tsl_filter_update_async(f, nullptr));
+    int rc = tsl_filter_sync(f, nullptr, out_pose, nullptr);
+    if (rc != TSL_OK) return rc;
+    std::vector<tsl_particle> back(P);
+    TSL_HIP_CHECK(hipMemcpy(back.data(), f->particles.ptr, sizeof(tsl_particle) * P, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < P; ++i) particles[i].second = back[i].second;  // cuda_evaluator.cu:405-408
+    return TSL_OK;
+}
+
+int tsl_evaluate_raw(tsl_evaluator* ev, const tsl_particle* particles, size_t P, const tsl_point* points,
+                     size_t N, const float tf[16], float* weights_out)
+{
+    if (!ev || !tf || (P && (!particles || !weights_out)) || (N && !points))
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (P == 0) return TSL_OK;
+    if (N == 0) {
+        std::memset(weights_out, 0, sizeof(float) * P);
+        return TSL_OK;
+    }
+    tsl_filter* f;
+    TSL_TRY(scratch_filter(ev, P, &f));
+    TSL_TRY(tsl_filter_set_particles(f, particles, P));
+    TSL_TRY(tsl_filter_set_scan(f, points, N, tf));
+    TSL_TRY(launch_raw(f, 0, f->raw.as<float>()));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    TSL_HIP_CHECK(hipMemcpy(weights_out, f->raw.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));
+    return TSL_OK;
+}
+
+int tsl_trace_indices(tsl_evaluator* ev, const tsl_particle* particles, size_t P, const tsl_point* points,
+                      size_t N, const float tf[16], uint64_t* out)
+{
+    if (!ev || !tf || (P && !particles) || (N && !points) || (P * N && !out))
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (P == 0 || N == 0) return TSL_OK;
+    tsl_filter* f;
+    TSL_TRY(scratch_filter(ev, P, &f));
+    TSL_TRY(tsl_filter_set_particles(f, particles, P));
+    TSL_TRY(tsl_filter_set_scan(f, points, N, tf));
+    DevBuf d_out;
+    TSL_TRY(d_out.reserve(sizeof(uint64_t) * P * N));
+    TSL_TRY(launch_trace(f, 0, d_out.as<uint64_t>()));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    TSL_HIP_CHECK(hipMemcpy(out, d_out.ptr, sizeof(uint64_t) * P * N, hipMemcpyDeviceToHost));
+    return TSL_OK;
+}
+
+int tsl_pose_matrices(const tsl_particle* particles, size_t P, const float tf[16], float* out)
+{
+    if (!tf || (P && (!particles || !out))) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (P == 0) return TSL_OK;
+    DevBuf d_p, d_l;
+    TSL_TRY(d_p.reserve(sizeof(tsl_particle) * P));
+    TSL_TRY(d_l.reserve(sizeof(float) * 12 * P));
+    TSL_HIP_CHECK(hipMemcpy(d_p.ptr, particles, sizeof(tsl_particle) * P, hipMemcpyHostToDevice));
+    TSL_TRY(launch_pose_matrices(d_p.as<tsl_particle>(), (uint32_t)P, tf, d_l.as<float>(), 0));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    TSL_HIP_CHECK(hipMemcpy(out, d_l.ptr, sizeof(float) * 12 * P, hipMemcpyDeviceToHost));
+    return TSL_OK;
+}
+
+int tsl_resample_capacity(size_t M, float u0, size_t* capacity)
+{
+    if (!capacity) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    return resample_capacity(M, u0, capacity);
+}
+
+int tsl_resample_systematic(const tsl_particle* in, size_t M, float u0, tsl_particle* out, size_t out_capacity,
+                            int32_t* out_indices, size_t* out_count)
+{
+    if (!in || !out_count || M == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument or M == 0");
+    size_t cap = 0;
+    TSL_TRY(resample_capacity(M, u0, &cap));
+    DevBuf d_in, d_out, d_s, d_u, d_idx, d_st;
+    TSL_TRY(d_in.reserve(sizeof(tsl_particle) * M));
+    TSL_TRY(d_out.reserve(sizeof(tsl_particle) * cap));
+    TSL_TRY(d_s.reserve(sizeof(double) * M));
+    TSL_TRY(d_u.reserve(sizeof(float) * cap));
+    TSL_TRY(d_idx.reserve(sizeof(int32_t) * cap));
+    TSL_TRY(d_st.reserve(2 * sizeof(uint32_t)));
+    TSL_HIP_CHECK(hipMemcpy(d_in.ptr, in, sizeof(tsl_particle) * M, hipMemcpyHostToDevice));
+    uint32_t* st = d_st.as<uint32_t>();
+    TSL_TRY(launch_resample_buffers(d_in.as<tsl_particle>(), M, u0, cap, d_s.as<double>(), d_u.as<float>(),
+                                    d_idx.as<int32_t>(), d_out.as<tsl_particle>(), st, st + 1, 0));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    uint32_t h[2];
+    TSL_HIP_CHECK(hipMemcpy(h, st, sizeof(h), hipMemcpyDeviceToHost));
+    if (h[1] != 0) return set_error(TSL_ERR_CAPACITY, "weights sum beyond 1 + 1e-5: normalise them first");
+    *out_count = h[0];
+    if (h[0] > out_capacity) return set_error(TSL_ERR_CAPACITY, "resampling emits %u particles", h[0]);
+    if (out && h[0]) TSL_HIP_CHECK(hipMemcpy(out, d_out.ptr, sizeof(tsl_particle) * h[0], hipMemcpyDeviceToHost));
+    if (out_indices && h[0])
+        TSL_HIP_CHECK(hipMemcpy(out_indices, d_idx.ptr, sizeof(int32_t) * h[0], hipMemcpyDeviceToHost));
+    return TSL_OK;
+}
+
+int tsl_stream_create(void** stream)
+{
+    if (!stream) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    hipStream_t s;
+    TSL_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return TSL_OK;
+}
+
+int tsl_stream_destroy(void* stream)
+{
+    if (stream) TSL_HIP_CHECK(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+    return TSL_OK;
+}
+
+int tsl_stream_synchronize(void* stream)
+{
+    TSL_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return TSL_OK;
+}
+
+int tsl_event_create(void** event)
+{
+    if (!event) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    hipEvent_t e;
+    TSL_HIP_CHECK(hipEventCreate(&e));
+    *event = e;
+    return TSL_OK;
+}
+
+int tsl_event_destroy(void* event)
+{
+    if (event) TSL_HIP_CHECK(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return TSL_OK;
+}
+
+int tsl_event_record(void* event, void* stream)
+{
+    TSL_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+    return TSL_OK;
+}
+
+int tsl_event_elapsed_ms(void* start, void* stop, float* ms)
+{
+    if (!ms) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    TSL_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+    TSL_HIP_CHECK(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    return TSL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+// Arithmetic shared by host and device code of the MI355X sensor update.
+//
+// Everything here must give bit-identical results on the host (x86-64, gcc
+// or clang) and on gfx950: IEEE float/double +,-,*,/ only, no FMA contraction
+// (all sources are compiled with -ffp-contract=off), no library math calls.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define TSL_HD __host__ __device__ __forceinline__
+#else
+#define TSL_HD inline
+#endif
+
+namespace tsl {
+
+// ---------------------------------------------------------------------------
+// float <-> size_t as compiled by gcc for x86-64 (the reference build,
+// CMakeLists.txt:5-14).  The reference assigns float values to size_t in its
+// lookup (cuda_sub_voxel_map.tcc:53-55,58-60,74-76,132-134); for values the
+// C++ standard leaves undefined (negative beyond -1, >= 2^64, NaN) the x86-64
+// code gcc emits is
+//     x >= 2^63 ? cvttss2si(x - 2^63) ^ 2^63 : cvttss2si(x)
+// with cvttss2si returning trunc(x) when it fits in int64 and
+// 0x8000000000000000 otherwise.  These helpers spell that out so that the GPU
+// reproduces the reference's indices for every input.
+// ---------------------------------------------------------------------------
+TSL_HD uint64_t cvtt_f32_i64(float x)
+{
+    if (x >= -9223372036854775808.0f && x < 9223372036854775808.0f)
+        return (uint64_t)(int64_t)x;
+    return 0x8000000000000000ull;
+}
+
+TSL_HD uint64_t cvt_f32_u64(float x)
+{
+    if (x >= 9223372036854775808.0f)
+        return cvtt_f32_i64(x - 9223372036854775808.0f) ^ 0x8000000000000000ull;
+    return cvtt_f32_i64(x);
+}
+
+// size_t -> float, round to nearest even.
+TSL_HD float cvt_u64_f32(uint64_t v) { return (float)v; }
+
+// ---------------------------------------------------------------------------
+// sin / cos of a float angle, evaluated in double and rounded once to float.
+// Cody-Waite reduction by pi/2 (pi/2 = P1 + P2 + P3 with P1, P2 of 33
+// significant bits, so k*P1 and k*P2 are exact for |k| < 2^20), then the
+// Taylor series of sin (to r^19) and cos (to r^20) on |r| <= pi/4 in Horner
+// form.  The same routine runs in the pose kernel and in the CPU oracle, so
+// pose matrices agree bit for bit.  Domain |x| < 1.6e6 rad; NaN outside.
+// ---------------------------------------------------------------------------
+TSL_HD double rint_d(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_rint(x);
+#else
+    return __builtin_rint(x);
+#endif
+}
+
+TSL_HD void sincosf_exact(float xf, float* s_out, float* c_out)
+{
+    const double x = (double)xf;
+    const double ax = x < 0.0 ? -x : x;
+    if (!(ax < 1.6e6)) {
+        *s_out = __builtin_nanf("");
+        *c_out = __builtin_nanf("");
+        return;
+    }
+    const double kd = rint_d(x * 0x1.45f306dc9c883p-1);
+    const double r = ((x - kd * 0x1.921fb54400000p+0) - kd * 0x1.0b4611a600000p-34) -
+                     kd * 0x1.3198a2e037073p-69;
+    const double r2 = r * r;
+    double ps = -0x1.2f49b46814157p-57;          // -1/19!
+    ps = r2 * ps + 0x1.952c77030ad4ap-49;        // +1/17!
+    ps = r2 * ps - 0x1.ae7f3e733b81fp-41;        // -1/15!
+    ps = r2 * ps + 0x1.6124613a86d09p-33;        // +1/13!
+    ps = r2 * ps - 0x1.ae64567f544e4p-26;        // -1/11!
+    ps = r2 * ps + 0x1.71de3a556c734p-19;        // +1/9!
+    ps = r2 * ps - 0x1.a01a01a01a01ap-13;        // -1/7!
+    ps = r2 * ps + 0x1.1111111111111p-7;         // +1/5!
+    ps = r2 * ps - 0x1.5555555555555p-3;         // -1/3!
+    const double s = r + (r * r2) * ps;
+    double pc = 0x1.e542ba4020225p-62;           // +1/20!
+    pc = r2 * pc - 0x1.6827863b97d97p-53;        // -1/18!
+    pc = r2 * pc + 0x1.ae7f3e733b81fp-45;        // +1/16!
+    pc = r2 * pc - 0x1.93974a8c07c9dp-37;        // -1/14!
+    pc = r2 * pc + 0x1.1eed8eff8d898p-29;        // +1/12!
+    pc = r2 * pc - 0x1.27e4fb7789f5cp-22;        // -1/10!
+    pc = r2 * pc + 0x1.a01a01a01a01ap-16;        // +1/8!
+    pc = r2 * pc - 0x1.6c16c16c16c17p-10;        // -1/6!
+    pc = r2 * pc + 0x1.5555555555555p-5;         // +1/4!
+    pc = r2 * pc - 0x1.0000000000000p-1;         // -1/2!
+    const double c = 1.0 + r2 * pc;
+    const int q = (int)((int64_t)kd & 3);
+    if (q == 0) { *s_out = (float)s;  *c_out = (float)c; }
+    else if (q == 1) { *s_out = (float)c;  *c_out = (float)-s; }
+    else if (q == 2) { *s_out = (float)-s; *c_out = (float)-c; }
+    else { *s_out = (float)-c; *c_out = (float)s; }
+}
+
+// ---------------------------------------------------------------------------
+// Pose matrix of a particle composed with the scanner->base transform
+// (tsdf_evaluator.cpp:102-145): L = [R | t] * T, R = Rz(yaw) Ry(pitch)
+// Rx(roll), left-to-right float evaluation exactly as the reference writes it.
+// pose = (x, y, z, roll, pitch, yaw); tf = row-major, entries 0..11 used.
+// ---------------------------------------------------------------------------
+TSL_HD void pose_matrix(const float* pose, const float* tf, float* L)
+{
+    float sa, ca, sb, cb, sg, cg;
+    sincosf_exact(pose[3], &sa, &ca);
+    sincosf_exact(pose[4], &sb, &cb);
+    sincosf_exact(pose[5], &sg, &cg);
+    float p[12];
+    p[0] = cb * cg;
+    p[4] = cb * sg;
+    p[8] = -sb;
+    p[3] = pose[0];
+    p[1] = sa * sb * cg - ca * sg;
+    p[5] = sa * sb * sg + ca * cg;
+    p[9] = sa * cb;
+    p[7] = pose[1];
+    p[2] = ca * sb * cg + sa * sg;
+    p[6] = ca * sb * sg - sa * cg;
+    p[10] = ca * cb;
+    p[11] = pose[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        L[4 * r + 0] = p[4 * r] * tf[0] + p[4 * r + 1] * tf[4] + p[4 * r + 2] * tf[8];
+        L[4 * r + 1] = p[4 * r] * tf[1] + p[4 * r + 1] * tf[5] + p[4 * r + 2] * tf[9];
+        L[4 * r + 2] = p[4 * r] * tf[2] + p[4 * r + 1] * tf[6] + p[4 * r + 2] * tf[10];
+        L[4 * r + 3] = p[4 * r] * tf[3] + p[4 * r + 1] * tf[7] + p[4 * r + 2] * tf[11] + p[4 * r + 3];
+    }
+}
+
+}  // namespace tsl

@@ -1,0 +1,224 @@
+"""Seeded synthetic workloads (scene -> TSDF map, ray-cast scan, particles).
+
+Wraps libsynth.so.  The named workloads follow the benchmark configurations
+of this project:
+  room()      dense 256x256x64 @ 0.05 m int16 TSDF (trunc 0.3 m) of a room with
+              walls, floor, ceiling and 20 boxes/cylinders; 16x1024 ray-cast
+              scan voxel-filtered to ~2k points; 1k particles ~ N(true; 0.2 m, 5 deg)
+  building()  sparse 1024x1024x128 @ 0.05 m int16 TSDF of a multi-room building;
+              128x2048 scan reduced to ~20k points; 100k particles uniform over
+              free space, yaw uniform, roll/pitch within +-3 deg
+"""
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import MapCoef
+from .evaluator import CudaSubVoxelMap, coef_init
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SYNTH_PATH = os.path.join(_HERE, "libsynth.so")
+if not os.path.exists(SYNTH_PATH):
+    raise ImportError(f"{SYNTH_PATH} is not built: run __graft_entry__.build()")
+_s = ctypes.CDLL(SYNTH_PATH)
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_sigs = {
+    "synth_scene_new": (_P, []),
+    "synth_scene_free": (None, [_P]),
+    "synth_free": (None, [_P]),
+    "synth_scene_add_box": (ctypes.c_int, [_P, _D, _D, _D, _D, _D, _D]),
+    "synth_scene_add_cylinder": (ctypes.c_int, [_P, _D, _D, _D, _D, _D]),
+    "synth_scene_count": (ctypes.c_int, [_P]),
+    "synth_scene_build_room": (None, [_P, ctypes.c_uint64, _D, _D, _D, ctypes.c_int, _D, _D, _D]),
+    "synth_scene_build_building": (None, [_P, ctypes.c_uint64, _D, _D, _D, _D, ctypes.c_int, _D]),
+    "synth_sdf": (_D, [_P, _D, _D, _D]),
+    "synth_sdf_batch": (None, [_P, _P, ctypes.c_size_t, _P]),
+    "synth_map_build": (ctypes.c_int, [_P, ctypes.POINTER(MapCoef), ctypes.c_int, ctypes.c_int, _P,
+                                       ctypes.POINTER(_P)]),
+    "synth_raycast": (ctypes.c_size_t, [_P, _P, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_uint64,
+                                        _P, _P]),
+    "synth_particles_normal": (None, [ctypes.c_uint64, ctypes.c_size_t, _P, _P, _P]),
+    "synth_particles_free": (ctypes.c_size_t, [_P, ctypes.c_uint64, ctypes.c_size_t, _P, _P, _D, _D, _P]),
+    "synth_voxel_filter": (ctypes.c_size_t, [_P, _P, ctypes.c_size_t, _D, _D, _P, _P]),
+}
+for _n, (_r, _a) in _sigs.items():
+    _f = getattr(_s, _n)
+    _f.restype = _r
+    _f.argtypes = _a
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Scene:
+    def __init__(self):
+        self.h = _s.synth_scene_new()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _s.synth_scene_free(self.h)
+            self.h = None
+
+    def add_box(self, lo, hi):
+        return _s.synth_scene_add_box(self.h, *map(float, lo), *map(float, hi))
+
+    def add_cylinder(self, cx, cy, z0, z1, r):
+        return _s.synth_scene_add_cylinder(self.h, float(cx), float(cy), float(z0), float(z1), float(r))
+
+    def count(self):
+        return _s.synth_scene_count(self.h)
+
+    def sdf(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        out = np.empty(xyz.shape[0], np.float64)
+        _s.synth_sdf_batch(self.h, _ptr(xyz), xyz.shape[0], _ptr(out))
+        return out
+
+    def tsdf_map(self, min_xyz, max_xyz, resolution, truncation_mm, sigma=0.1, dense=False, init_value=None):
+        """CudaSubVoxelMap of this scene with int16 TSDF voxels."""
+        if init_value is None:
+            from .evaluator import likelihood_init
+            init_value = likelihood_init(sigma)
+        c = coef_init(min_xyz, max_xyz, resolution, init_value)
+        grid = np.empty(c.grid_occ_size, np.int32)
+        out = ctypes.c_void_p()
+        rc = _s.synth_map_build(self.h, ctypes.byref(c), int(truncation_mm), int(bool(dense)), _ptr(grid),
+                                ctypes.byref(out))
+        if rc != 0:
+            raise RuntimeError(f"synth_map_build failed ({rc})")
+        n = int(c.data_size)
+        buf = (ctypes.c_int16 * max(n, 1)).from_address(out.value)
+        tsdf = np.frombuffer(buf, dtype=np.int16, count=n).copy()
+        _s.synth_free(out)
+        return CudaSubVoxelMap(c, grid, tsdf_mm=tsdf, truncation_mm=truncation_mm, sigma=sigma)
+
+    def raycast(self, sensor_pose, beams, azimuths, el_min_deg, el_max_deg, max_range, noise_sigma=0.0, seed=0):
+        total = beams * azimuths
+        pts = np.empty((total, 3), np.float32)
+        rings = np.empty(total, np.int32)
+        pose = np.ascontiguousarray(sensor_pose, np.float64)
+        n = _s.synth_raycast(self.h, _ptr(pose), beams, azimuths, el_min_deg, el_max_deg, max_range, noise_sigma,
+                             seed, _ptr(pts), _ptr(rings))
+        return pts[:n].copy(), rings[:n].copy()
+
+    def particles_free(self, seed, n, lo, hi, clearance, rp):
+        out = np.empty((n, 7), np.float32)
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        _s.synth_particles_free(self.h, seed, n, _ptr(lo), _ptr(hi), clearance, rp, _ptr(out))
+        return out
+
+
+def particles_normal(seed, n, mean, sigma):
+    out = np.empty((n, 7), np.float32)
+    m = np.ascontiguousarray(mean, np.float64)
+    s = np.ascontiguousarray(sigma, np.float64)
+    _s.synth_particles_normal(seed, n, _ptr(m), _ptr(s), _ptr(out))
+    return out
+
+
+def voxel_filter(pts, rings, cell, min_range=1.0):
+    pts = np.ascontiguousarray(pts, np.float32)
+    rings = np.ascontiguousarray(rings, np.int32)
+    out = np.empty_like(pts)
+    out_r = np.empty_like(rings)
+    n = _s.synth_voxel_filter(_ptr(pts), _ptr(rings), pts.shape[0], cell, min_range, _ptr(out), _ptr(out_r))
+    return out[:n].copy(), out_r[:n].copy()
+
+
+def rpy_matrix(roll, pitch, yaw):
+    sa, ca = math.sin(roll), math.cos(roll)
+    sb, cb = math.sin(pitch), math.cos(pitch)
+    sg, cg = math.sin(yaw), math.cos(yaw)
+    return np.array([[cb * cg, sa * sb * cg - ca * sg, ca * sb * cg + sa * sg],
+                     [cb * sg, sa * sb * sg + ca * cg, ca * sb * sg - sa * cg],
+                     [-sb, sa * cb, ca * cb]])
+
+
+def tf_from_xyzrpy(x, y, z, roll, pitch, yaw):
+    """Row-major 4x4 homogeneous transform (float32, 16 entries)."""
+    T = np.eye(4)
+    T[:3, :3] = rpy_matrix(roll, pitch, yaw)
+    T[:3, 3] = (x, y, z)
+    return T.astype(np.float32).reshape(-1)
+
+
+def compose_pose(base, tf16):
+    """Scanner pose in the map frame = base pose (x,y,z,r,p,y) composed with tf."""
+    B = np.eye(4)
+    B[:3, :3] = rpy_matrix(*base[3:6])
+    B[:3, 3] = base[:3]
+    W = B @ np.asarray(tf16, np.float64).reshape(4, 4)
+    R = W[:3, :3]
+    pitch = -math.asin(max(-1.0, min(1.0, R[2, 0])))
+    roll = math.atan2(R[2, 1], R[2, 2])
+    yaw = math.atan2(R[1, 0], R[0, 0])
+    return np.array([W[0, 3], W[1, 3], W[2, 3], roll, pitch, yaw])
+
+
+# scanner mounted 0.1 m ahead of and 0.4 m above the base, slightly rotated
+SCANNER_TF = dict(x=0.1, y=0.0, z=0.4, roll=0.0, pitch=math.radians(1.0), yaw=math.radians(2.0))
+
+
+@dataclass
+class Workload:
+    name: str
+    map: CudaSubVoxelMap
+    points: np.ndarray
+    rings: np.ndarray
+    particles: np.ndarray
+    tf: np.ndarray
+    true_pose: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+
+def room(seed=0x5EED0001, num_particles=1000, size=(12.8, 12.8, 3.2), resolution=0.05, truncation_mm=300,
+         beams=16, azimuths=1024, filter_cell=0.3, objects=20):
+    sx, sy, sz = size
+    true_pose = np.array([sx / 2, sy / 2, 1.0, 0.0, 0.0, 0.3])
+    sc = Scene()
+    _s.synth_scene_build_room(sc.h, seed, sx, sy, sz, objects, true_pose[0], true_pose[1], 1.5)
+    m = sc.tsdf_map((0, 0, 0), (sx, sy, sz), resolution, truncation_mm, dense=True)
+    tf = tf_from_xyzrpy(**SCANNER_TF)
+    sensor = compose_pose(true_pose, tf)
+    raw, rings = sc.raycast(sensor, beams, azimuths, -15.0, 15.0, 30.0, 0.01, seed)
+    pts, rings = voxel_filter(raw, rings, filter_cell, 1.0)
+    parts = particles_normal(seed, num_particles, true_pose,
+                             [0.2, 0.2, 0.2, math.radians(5), math.radians(5), math.radians(5)])
+    return Workload("room", m, pts, rings, parts, tf, true_pose,
+                    {"scene_objects": sc.count(), "raw_points": int(raw.shape[0])})
+
+
+def building(seed=0x5EED0002, num_particles=100_000, size=(51.2, 51.2, 6.4), resolution=0.05, truncation_mm=300,
+             beams=128, azimuths=2048, filter_cell=0.07, room_size=10.0, objects_per_room=4, ceiling_fraction=0.3,
+             wall_h=3.0, target_points=20000):
+    sx, sy, sz = size
+    sc = Scene()
+    _s.synth_scene_build_building(sc.h, seed, sx, sy, wall_h, room_size, objects_per_room, ceiling_fraction)
+    m = sc.tsdf_map((0, 0, 0), (sx, sy, sz), resolution, truncation_mm, dense=False)
+    rng = np.random.default_rng(seed)
+    # true pose: a free spot near the middle of a room
+    for _ in range(1000):
+        x, y = rng.uniform(2, sx - 2), rng.uniform(2, sy - 2)
+        if sc.sdf([[x, y, 1.0]])[0] > 1.0:
+            break
+    true_pose = np.array([x, y, 1.0, 0.0, 0.0, rng.uniform(-math.pi, math.pi)])
+    tf = tf_from_xyzrpy(**SCANNER_TF)
+    sensor = compose_pose(true_pose, tf)
+    raw, rings = sc.raycast(sensor, beams, azimuths, -22.5, 22.5, 60.0, 0.01, seed)
+    pts, rings = voxel_filter(raw, rings, filter_cell, 1.0)
+    if target_points is not None and pts.shape[0] > target_points:
+        keep = np.sort(rng.choice(pts.shape[0], target_points, replace=False))
+        pts, rings = pts[keep], rings[keep]
+    parts = sc.particles_free(seed, num_particles, (0.5, 0.5, 0.5), (sx - 0.5, sy - 0.5, 2.0), 0.3,
+                              math.radians(3.0))
+    blocks = int(np.count_nonzero(m.grid_occ >= 0))
+    return Workload("building", m, pts, rings, parts, tf, true_pose,
+                    {"scene_objects": sc.count(), "raw_points": int(raw.shape[0]), "allocated_blocks": blocks,
+                     "block_fraction": blocks / m.grid_occ.size})
