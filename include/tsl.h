@@ -176,6 +176,21 @@ int tsl_trace_indices(tsl_evaluator* ev, const tsl_particle* particles, size_t n
                       const tsl_point* points, size_t num_points, const float tf_matrix[16],
                       uint64_t* out);
 
+/* Voxel value CudaSubVoxelMap::getEntry (cuda_sub_voxel_map.tcc:140-158)
+ * returns for every (particle, point) pair, read through the device map's
+ * storage by the production kernel; for parity checking.  Synchronous. */
+int tsl_trace_values(tsl_evaluator* ev, const tsl_particle* particles, size_t num_particles,
+                     const tsl_point* points, size_t num_points, const float tf_matrix[16],
+                     float* out);
+
+/* Self-checks of the host-built lookup tables, host only (no GPU): the
+ * sub-voxel threshold table of a resolution against IEEE division over every
+ * float in [0, 1) whose bit pattern is a multiple of stride, and the
+ * constant-divisor multipliers of the brick layout over every x < 2^31 that is
+ * a multiple of stride.  No counterpart in the reference. */
+int tsl_check_sub_table(float resolution, uint32_t stride, uint64_t* mismatches, uint32_t* entries);
+int tsl_check_magic_divisor(uint32_t divisor, uint32_t stride, uint64_t* mismatches);
+
 /* Pose matrices L[12] (tsdf_evaluator.cpp:102-145) computed on the device. */
 int tsl_pose_matrices(const tsl_particle* particles, size_t num_particles,
                       const float tf_matrix[16], float* out);
@@ -226,6 +241,30 @@ int tsl_filter_resample_async(tsl_filter* f, float u0, void* stream);
  * current particle count.  Returns TSL_ERR_NO_VALID_PARTICLE if the last
  * update had a zero weight sum. */
 int tsl_filter_sync(tsl_filter* f, void* stream, tsl_pose* pose, size_t* count);
+
+/* Keep a device copy of the current particle set (synchronous), and restore
+ * the resident set from it on `stream` (async).  Lets a caller replay an
+ * update from the same particles, e.g. to time a fixed workload.  No
+ * counterpart in the reference. */
+int tsl_filter_snapshot(tsl_filter* f);
+int tsl_filter_restore_async(tsl_filter* f, void* stream);
+
+/* Tile shape of the evaluation: k particles per wavefront (1 or 2; 0 =
+ * automatic) and chunk_len scan points per wavefront (0 = automatic).
+ * Results do not depend on the tile beyond float summation order. */
+int tsl_filter_set_tile(tsl_filter* f, int k, uint32_t chunk_len);
+
+/* Time the candidate tile shapes on the filter's current particles and scan
+ * (reps launches each, on `stream`, synchronously) and keep the fastest;
+ * reports it through the non-NULL out arguments. */
+int tsl_filter_autotune(tsl_filter* f, void* stream, int reps, int* best_k, uint32_t* best_chunk,
+                        float* best_ms);
+
+/* Bracket every evaluation-kernel launch of this filter with HIP events on
+ * the launch stream (enable resets the totals); read the accumulated kernel
+ * time, number of launches and particle*point evaluations launched. */
+int tsl_filter_set_timing(tsl_filter* f, int enable);
+int tsl_filter_eval_timing(tsl_filter* f, double* ms, uint64_t* launches, uint64_t* pairs);
 
 /* ---- streams and timing (plumbing for benchmarks) ---------------------- */
 int tsl_stream_create(void** stream);

@@ -103,6 +103,11 @@ ORC_EXPORT void orc_sincosf(float xf, float* s_out, float* c_out)
         *c_out = NAN;
         return;
     }
+    if (x == 0.0) { /* sin(+-0) = +-0 */
+        *s_out = xf;
+        *c_out = 1.0f;
+        return;
+    }
     double kd = rint(x * ORC_TWO_OVER_PI);
     double r = ((x - kd * ORC_PIO2_1) - kd * ORC_PIO2_2) - kd * ORC_PIO2_3;
     double r2 = r * r;

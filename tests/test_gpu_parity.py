@@ -225,3 +225,103 @@ def test_device_filter_update_and_resample(tsl, room, room_oracle):
     ref_idx = ob.resample(got[:, 6], u0)
     assert n2 == ref_idx.size
     assert np.array_equal(after, got[ref_idx])
+
+
+def test_general_tier_huge_and_nan(tsl, room, room_oracle):
+    """Offsets beyond 2^23 m, infinities and NaN take the general tier, which
+    spells out every x86-64 conversion; indices must still match."""
+    rng = np.random.default_rng(11)
+    parts = room.particles[:64].copy()
+    parts[0, 0] = 1e8
+    parts[1, 1] = -1e8
+    parts[2, 2] = np.inf
+    parts[3, 0] = np.nan
+    parts[4, 5] = np.nan
+    parts[5, 0] = 3e38
+    parts[6, 0] = -9.3e18
+    parts[7, 0] = 1.9e19
+    parts[8:40, :3] += rng.uniform(-4e6, 4e6, (32, 3)).astype(np.float32)
+    pts = np.concatenate([room.points[:300], rng.uniform(-1e7, 1e7, (40, 3)).astype(np.float32)])
+    ev = tsl.TSDFEvaluator(room.map)
+    gpu = ev.trace_indices(parts, pts, room.tf)
+    ref = room_oracle.trace(parts, pts, room.tf)
+    assert np.array_equal(gpu, ref), f"{np.count_nonzero(gpu != ref)} mismatches"
+
+
+def oracle_values(om, idx):
+    """getEntry (cuda_sub_voxel_map.tcc:140-158) from the oracle's indices."""
+    ds = om.coef.data_size
+    vals = np.full(idx.shape, np.float32(om.coef.init_value), np.float32)
+    ok = idx < ds
+    vals[ok] = om.data[idx[ok].astype(np.int64)]
+    return vals
+
+
+def test_room_values_bit_exact(tsl, room, room_oracle):
+    """Bricked device layout (room map: 20^3 blocks, aligned offsets)."""
+    ev = tsl.TSDFEvaluator(room.map)
+    got = ev.trace_values(room.particles[:300], room.points, room.tf)
+    ref = oracle_values(room_oracle, room_oracle.trace(room.particles[:300], room.points, room.tf))
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_values_linear_layout_odd_sub_dim(tsl):
+    """resolution 0.03 -> 34^3 blocks: codes stay in the reference's linear layout."""
+    from tsdf_localization_amd import synth
+    sc = synth.Scene()
+    sc.add_box((-5, -5, -5), (20, 20, 0.3))
+    sc.add_box((-5, -5, -5), (0.35, 20, 20))
+    sc.add_cylinder(1.7, 1.5, 0.0, 2.0, 0.3)
+    m = sc.tsdf_map((0, 0, 0), (3.2, 3.2, 2.4), 0.03, 150, dense=False)
+    assert m.coef().sub_dim == 34
+    rng = np.random.default_rng(5)
+    P = 64
+    parts = np.zeros((P, 7), np.float32)
+    parts[:, :3] = rng.uniform([0.5, 0.5, 0.5], [2.5, 2.5, 1.5], (P, 3))
+    parts[:, 5] = rng.uniform(-math.pi, math.pi, P)
+    pts = rng.uniform(-2.5, 2.5, (700, 3)).astype(np.float32)
+    tf = synth.tf_from_xyzrpy(0, 0, 0, 0, 0, 0)
+    ev = tsl.TSDFEvaluator(m)
+    om = ob.OracleMap.from_host_map(m)
+    idx_ref = om.trace(parts, pts, tf)
+    assert np.array_equal(ev.trace_indices(parts, pts, tf), idx_ref)
+    got = ev.trace_values(parts, pts, tf)
+    assert np.array_equal(got.view(np.uint32), oracle_values(om, idx_ref).view(np.uint32))
+
+
+def test_values_unaligned_block_offsets(tsl, room):
+    """A block table whose offsets are not multiples of sub_dim^3 (legal in the
+    reference's representation) keeps the linear layout and exact values."""
+    c = room.map.coef()
+    grid = room.map.grid_occ.copy()
+    data = ob.float_data(room.map).copy()
+    sd3 = c.sub_dim ** 3
+    # shift every block by 7 voxels into a longer array
+    grid = np.where(grid >= 0, grid + 7, grid).astype(np.int32)
+    data = np.concatenate([np.arange(7, dtype=np.float32) + 100.0, data]).astype(np.float32)
+    c2 = type(c)()
+    for name, _ in type(c)._fields_:
+        setattr(c2, name, getattr(c, name))
+    c2.data_size = data.size
+    m = tsl.CudaSubVoxelMap(c2, grid, data=data)
+    assert data.size % sd3 != 0
+    om = ob.OracleMap(c2, grid, data)
+    ev = tsl.TSDFEvaluator(m)
+    parts = room.particles[:100]
+    idx_ref = om.trace(parts, room.points, room.tf)
+    assert np.array_equal(ev.trace_indices(parts, room.points, room.tf), idx_ref)
+    got = ev.trace_values(parts, room.points, room.tf)
+    assert np.array_equal(got.view(np.uint32), oracle_values(om, idx_ref).view(np.uint32))
+
+
+def test_values_wrap_and_general_tiers(tsl, room, room_oracle):
+    rng = np.random.default_rng(21)
+    parts = room.particles[:96].copy()
+    parts[:32, :3] -= rng.uniform(1.0, 14.0, (32, 3)).astype(np.float32)   # wrap band
+    parts[32:40, 0] = rng.uniform(-3e7, 3e7, 8).astype(np.float32)          # general tier
+    pts = room.points
+    ev = tsl.TSDFEvaluator(room.map)
+    idx_ref = room_oracle.trace(parts, pts, room.tf)
+    assert np.array_equal(ev.trace_indices(parts, pts, room.tf), idx_ref)
+    got = ev.trace_values(parts, pts, room.tf)
+    assert np.array_equal(got.view(np.uint32), oracle_values(room_oracle, idx_ref).view(np.uint32))

@@ -1,0 +1,162 @@
+"""CPU tests of the C ABI library: it loads, exports every function the
+header declares, and its host-only parts (map constructor arithmetic,
+setData, likelihood values, the division-free lookup tables) agree with the
+oracle or with exhaustive checks.  No GPU call is made here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+from tsdf_localization_amd import _lib
+from tsdf_localization_amd import evaluator as tev
+from tsdf_localization_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tsl.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(tsl_\w+)\s*\(", text, flags=re.M)))
+
+
+def _exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    return sorted({ln.split()[-1] for ln in out.splitlines() if ln.split() and ln.split()[-1].startswith("tsl_")})
+
+
+def test_header_declares_the_expected_surface():
+    names = _declared()
+    for n in ["tsl_evaluate", "tsl_map_create", "tsl_map_create_tsdf", "tsl_map_set_data", "tsl_evaluator_create",
+              "tsl_resample_systematic", "tsl_filter_update_async", "tsl_filter_resample_async"]:
+        assert n in names
+    assert len(names) >= 40
+
+
+def test_library_exports_exactly_the_header():
+    declared, exported = _declared(), _exported()
+    assert set(declared) - set(exported) == set(), "declared but not exported"
+    assert set(exported) - set(declared) == set(), "exported but not declared"
+    for n in declared:
+        assert hasattr(_lib.lib, n)
+
+
+def test_version_and_error_channel():
+    assert _lib.lib.tsl_version() >= 1
+    c = _lib.MapCoef()
+    rc = _lib.lib.tsl_map_set_data(ctypes.byref(c), None, None, 1, None, None, 0)
+    assert rc == _lib.TSL_ERR_INVALID_ARGUMENT
+    assert _lib.lib.tsl_last_error().decode()
+    with pytest.raises(_lib.TslError):
+        _lib.check(rc)
+
+
+@pytest.mark.parametrize("box,res", [
+    (((0, 0, 0), (12.8, 12.8, 3.2)), 0.05),
+    (((-25.6, -25.6, -1.0), (25.6, 25.6, 5.4)), 0.05),
+    (((-3.3, 1.1, -0.7), (4.05, 9.9, 2.2)), 0.064),
+    (((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)), 0.1),
+    (((-100.0, -50.0, -3.0), (100.0, 50.0, 9.0)), 0.2),
+])
+def test_coef_init_matches_oracle(box, res):
+    got = tev.coef_init(box[0], box[1], res, -2.5)
+    ref = ob.coef_init(box[0], box[1], res, -2.5)
+    for name, _ in ob.OrcCoef._fields_:
+        assert getattr(got, name) == getattr(ref, name), name
+
+
+@pytest.mark.parametrize("sigma", [0.05, 0.1, 0.3])
+def test_likelihood_values_match_oracle(sigma):
+    for mm in range(-1000, 1001):
+        a = np.float32(tev.likelihood_value(mm, sigma))
+        b = np.float32(ob.olib.orc_likelihood_value(mm, sigma))
+        assert a.view(np.uint32) == b.view(np.uint32), mm
+    assert tev.likelihood_init(sigma) == ob.olib.orc_likelihood_init(sigma)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_set_data_matches_oracle(seed):
+    rng = np.random.default_rng(seed)
+    lo, hi, res = (-2.0, -1.5, -0.5), (3.0, 2.5, 1.5), 0.05
+    n = 5000
+    # cells on the voxel lattice (as a TSDF map hands them over), some
+    # duplicated
+    ijk = np.stack([rng.integers(0, 100, n), rng.integers(0, 80, n), rng.integers(0, 40, n)], 1)
+    cells = (np.array(lo, np.float32) + ijk.astype(np.float32) * np.float32(res)).astype(np.float32)
+    cells = np.concatenate([cells, cells[:200]])
+    vals = rng.uniform(-1, 1, cells.shape[0]).astype(np.float32)
+    m = tev.CudaSubVoxelMap.from_cells(lo, hi, res, -9.0, cells, vals)
+    c, grid, data = ob.set_data(lo, hi, res, -9.0, cells, vals)
+    assert m.dataSize() == c.data_size
+    assert np.array_equal(m.rawGridOcc(), grid)
+    assert np.array_equal(m.rawData().view(np.uint32), data.view(np.uint32))
+
+
+def test_set_data_rejects_cells_past_the_block_table():
+    """setData throws "Upper voxel index overflow!" for a cell whose block
+    index is past the table (cuda_sub_voxel_map.tcc:188-191), including
+    cells below min, whose size_t conversion wraps."""
+    for cell in ([[0.5, 0.5, 2.5]], [[0.5, -1.5, 0.5]], [[0.5, 0.5, 0.5], [0.1, 0.1, 7.0]]):
+        cells = np.array(cell, np.float32)
+        vals = np.ones(len(cell), np.float32)
+        with pytest.raises(_lib.TslError, match="Upper voxel index overflow!"):
+            tev.CudaSubVoxelMap.from_cells((0, 0, 0), (2, 2, 2), 0.25, 0.0, cells, vals)
+        with pytest.raises(RuntimeError, match="Upper voxel index overflow!"):
+            ob.set_data((0, 0, 0), (2, 2, 2), 0.25, 0.0, cells, vals)
+
+
+def test_set_data_capacity_error_reports_size():
+    c = tev.coef_init((0, 0, 0), (2, 2, 2), 0.25, 0.0)
+    cells = np.array([[0.1, 0.1, 0.1], [1.9, 1.9, 1.9]], np.float32)
+    vals = np.ones(2, np.float32)
+    grid = np.empty(c.grid_occ_size, np.int32)
+    data = np.empty(10, np.float32)
+    rc = _lib.lib.tsl_map_set_data(ctypes.byref(c), _lib.ptr(cells), _lib.ptr(vals), 2, _lib.ptr(grid),
+                                   _lib.ptr(data), data.size)
+    assert rc == _lib.TSL_ERR_CAPACITY
+    assert c.data_size == 128
+
+
+@pytest.mark.parametrize("res", [0.05, 0.064, 0.1, 0.02, 0.25, 0.5, 1.0, 0.0333, 0.07, 0.125, 0.03])
+def test_sub_voxel_table_exhaustive(res):
+    """trunc(fl(a / res)) from the bucket table equals the IEEE division for
+    every float a in [0, 1)."""
+    bad, entries = ctypes.c_uint64(), ctypes.c_uint32()
+    _lib.check(_lib.lib.tsl_check_sub_table(res, 1, ctypes.byref(bad), ctypes.byref(entries)))
+    assert bad.value == 0
+    assert 0 < entries.value <= 4096
+
+
+@pytest.mark.parametrize("d", [2, 3, 7, 16, 20, 64, 400, 4096, 8000, 64000, 1000003, 2**31 - 1])
+def test_magic_divisor(d):
+    bad = ctypes.c_uint64()
+    _lib.check(_lib.lib.tsl_check_magic_divisor(d, 1 if d in (20, 400, 8000) else 97, ctypes.byref(bad)))
+    assert bad.value == 0
+
+
+def test_resample_capacity_bounds_oracle_count():
+    rng = np.random.default_rng(11)
+    for M in [1, 10, 1000, 100_000]:
+        for _ in range(3):
+            w = rng.uniform(0, 1, M)
+            w = (w / w.sum()).astype(np.float32)
+            u0 = np.float32(rng.uniform(0, 1.0 / M))
+            cap = ctypes.c_size_t()
+            _lib.check(_lib.lib.tsl_resample_capacity(M, float(u0), ctypes.byref(cap)))
+            n = ob.resample(w, u0).size
+            assert n <= cap.value
+
+
+def test_synthetic_workloads_are_deterministic():
+    a, b = synth.room(), synth.room()
+    assert np.array_equal(a.points, b.points)
+    assert np.array_equal(a.particles, b.particles)
+    assert np.array_equal(a.map.grid_occ, b.map.grid_occ)
+    assert np.array_equal(a.map.tsdf_mm, b.map.tsdf_mm)
+    assert a.points.shape[1] == 3 and a.particles.shape[1] == 7

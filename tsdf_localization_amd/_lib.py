@@ -74,6 +74,8 @@ _SIGS = {
     "tsl_evaluate": (_I, [_P, _P, _S, _P, _S, _P, ctypes.POINTER(Pose)]),
     "tsl_evaluate_raw": (_I, [_P, _P, _S, _P, _S, _P, _P]),
     "tsl_trace_indices": (_I, [_P, _P, _S, _P, _S, _P, _P]),
+    "tsl_trace_values": (_I, [_P, _P, _S, _P, _S, _P, _P]),
+    "tsl_check_magic_divisor": (_I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
     "tsl_pose_matrices": (_I, [_P, _S, _P, _P]),
     "tsl_resample_systematic": (_I, [_P, _S, _F, _P, _S, _P, ctypes.POINTER(_S)]),
     "tsl_resample_capacity": (_I, [_S, _F, ctypes.POINTER(_S)]),
@@ -85,6 +87,16 @@ _SIGS = {
     "tsl_filter_update_async": (_I, [_P, _P]),
     "tsl_filter_resample_async": (_I, [_P, _F, _P]),
     "tsl_filter_sync": (_I, [_P, _P, ctypes.POINTER(Pose), ctypes.POINTER(_S)]),
+    "tsl_filter_snapshot": (_I, [_P]),
+    "tsl_filter_restore_async": (_I, [_P, _P]),
+    "tsl_filter_set_tile": (_I, [_P, _I, ctypes.c_uint32]),
+    "tsl_filter_autotune": (_I, [_P, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_uint32),
+                                 ctypes.POINTER(_F)]),
+    "tsl_filter_set_timing": (_I, [_P, _I]),
+    "tsl_filter_eval_timing": (_I, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                                    ctypes.POINTER(ctypes.c_uint64)]),
+    "tsl_check_sub_table": (_I, [_F, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
+                                 ctypes.POINTER(ctypes.c_uint32)]),
     "tsl_stream_create": (_I, [ctypes.POINTER(_P)]),
     "tsl_stream_destroy": (_I, [_P]),
     "tsl_stream_synchronize": (_I, [_P]),

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Wno-unused-function",
 ]
 
-HIP_SOURCES = ["api.hip", "eval.hip", "resample.hip"]
+HIP_SOURCES = ["api.hip", "eval.hip", "resample.hip", "tables.hip", "scan.hip"]
 SYNTH_SOURCES = ["synth.cpp"]
 
 LIBTSL = os.path.join(HERE, "libtsl.so")
@@ -67,7 +67,7 @@ def build_tsl(verbose=False, jobs=4):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), procs))
     tmp = LIBTSL + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"], verbose)
     os.replace(tmp, LIBTSL)
     return LIBTSL
 

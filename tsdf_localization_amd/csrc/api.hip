@@ -2,6 +2,7 @@
 // drop-in entry points (synchronous) and the device-resident filter API.
 #include "tsl_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -48,7 +49,7 @@ uint32_t grid_threshold(uint64_t dim, float res)
 
 int launch_resample_buffers(const tsl_particle* d_in, size_t M, float u0, size_t cap, double* d_s,
                             float* d_u, int32_t* d_idx, tsl_particle* d_out, uint32_t* d_count,
-                            uint32_t* d_status, hipStream_t stream);
+                            uint32_t* d_status, uint32_t* d_first_bad, hipStream_t stream);
 int launch_set_count(tsl_filter* f, uint32_t n, hipStream_t stream);
 
 int filter_reserve_particles(tsl_filter* f, size_t n)
@@ -160,7 +161,7 @@ static int validate_coef(const tsl_map_coef* c)
         return set_error(TSL_ERR_INVALID_ARGUMENT, "grid_occ_size does not match the block dims");
     if (c->up_dim_2 != c->up_dim_x * c->up_dim_y || c->sub_dim_2 != c->sub_dim * c->sub_dim)
         return set_error(TSL_ERR_INVALID_ARGUMENT, "inconsistent derived dims");
-    if (c->data_size > 0x7fffffffull || c->grid_occ_size > 0x7fffffffull)
+    if (c->data_size > 0x7fffffffull || c->grid_occ_size > 0x7fffffffull || c->sub_dim > 1024)
         return set_error(TSL_ERR_UNSUPPORTED, "map exceeds the int32 block table of the reference (%llu voxels)",
                          (unsigned long long)c->data_size);
     return TSL_OK;
@@ -170,35 +171,105 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
                       const std::vector<float>& palette)
 {
     const tsl_map_coef& c = m->coef;
+    // brick layout when every block offset is an aligned multiple of sd^3
+    const uint64_t sd = c.sub_dim, sd3 = sd * sd * sd;
+    bool brick = sd >= 4 && sd % 4 == 0 && sd3 < (1ull << 30);
+    for (uint64_t b = 0; brick && b < c.grid_occ_size; ++b)
+        if (grid_occ[b] >= 0 && (uint64_t)grid_occ[b] % sd3 != 0) brick = false;
+    const std::vector<uint16_t>* upload = &codes;
+    std::vector<uint16_t> phys;
+    uint64_t phys_size = c.data_size;
+    if (brick && c.data_size) {
+        const uint64_t B = sd / 4;
+        phys_size = (c.data_size + sd3 - 1) / sd3 * sd3;
+        phys.assign(phys_size, 0);
+        for (uint64_t idx = 0; idx < c.data_size; ++idx) {
+            const uint64_t slot = idx / sd3, rem = idx % sd3;
+            const uint64_t lz = rem / (sd * sd), ly = (rem / sd) % sd, lx = rem % sd;
+            const uint64_t off = ((lx >> 2) + (ly >> 2) * B + (lz >> 2) * B * B) * 64 + (lx & 3) + (ly & 3) * 4 +
+                                 (lz & 3) * 16;
+            phys[slot * sd3 + off] = codes[idx];
+        }
+        upload = &phys;
+    }
     TSL_TRY(m->grid_occ.reserve(sizeof(int32_t) * c.grid_occ_size));
-    TSL_TRY(m->codes.reserve(sizeof(uint16_t) * (c.data_size ? c.data_size : 1)));
+    TSL_TRY(m->codes.reserve(sizeof(uint16_t) * (phys_size ? phys_size : 1)));
     TSL_TRY(m->palette.reserve(sizeof(float) * (palette.empty() ? 1 : palette.size())));
     TSL_HIP_CHECK(hipMemcpy(m->grid_occ.ptr, grid_occ, sizeof(int32_t) * c.grid_occ_size,
                             hipMemcpyHostToDevice));
-    if (c.data_size)
-        TSL_HIP_CHECK(hipMemcpy(m->codes.ptr, codes.data(), sizeof(uint16_t) * c.data_size,
+    if (phys_size)
+        TSL_HIP_CHECK(hipMemcpy(m->codes.ptr, upload->data(), sizeof(uint16_t) * phys_size,
                                 hipMemcpyHostToDevice));
     if (!palette.empty())
         TSL_HIP_CHECK(hipMemcpy(m->palette.ptr, palette.data(), sizeof(float) * palette.size(),
                                 hipMemcpyHostToDevice));
+    SubTable st;
+    TSL_TRY(build_sub_table(c.resolution, &st));
+    TSL_TRY(m->sub_tab.reserve(sizeof(SubEntry) * st.entries.size()));
+    TSL_HIP_CHECK(hipMemcpy(m->sub_tab.ptr, st.entries.data(), sizeof(SubEntry) * st.entries.size(),
+                            hipMemcpyHostToDevice));
+    DevMapFull fm{};
+    fm.grid_occ = m->grid_occ.as<int32_t>();
+    fm.res = c.resolution;
+    fm.dim_x = c.dim_x; fm.dim_y = c.dim_y; fm.dim_z = c.dim_z;
+    fm.up_dim_x = c.up_dim_x;
+    fm.up_dim_2 = c.up_dim_2;
+    fm.sub_dim = c.sub_dim;
+    fm.sub_dim_2 = c.sub_dim_2;
+    fm.grid_occ_size = c.grid_occ_size;
+    fm.data_size = c.data_size;
+    fm.gthr_x = grid_threshold(c.dim_x, c.resolution);
+    fm.gthr_y = grid_threshold(c.dim_y, c.resolution);
+    fm.gthr_z = grid_threshold(c.dim_z, c.resolution);
+    negative_band_constants(c, fm);
+    TSL_TRY(m->full.reserve(sizeof(DevMapFull)));
+    TSL_HIP_CHECK(hipMemcpy(m->full.ptr, &fm, sizeof(DevMapFull), hipMemcpyHostToDevice));
     DevMap& d = m->dev;
     d.grid_occ = m->grid_occ.as<int32_t>();
     d.codes = m->codes.as<uint16_t>();
     d.palette = m->palette.as<float>();
+    d.sub_tab = m->sub_tab.as<SubEntry>();
+    d.full = m->full.as<DevMapFull>();
     d.palette_size = (uint32_t)palette.size();
+    d.sub_entries = (uint32_t)st.entries.size();
+    d.sub_shift = st.shift;
+    d.sub_base = st.base_bits;
     d.min_x = c.min_x; d.min_y = c.min_y; d.min_z = c.min_z;
-    d.res = c.resolution;
     d.init = c.init_value;
-    d.gthr_x = grid_threshold(c.dim_x, c.resolution);
-    d.gthr_y = grid_threshold(c.dim_y, c.resolution);
-    d.gthr_z = grid_threshold(c.dim_z, c.resolution);
-    d.dim_x = c.dim_x; d.dim_y = c.dim_y; d.dim_z = c.dim_z;
-    d.up_dim_x = c.up_dim_x;
-    d.up_dim_2 = c.up_dim_2;
-    d.sub_dim = c.sub_dim;
-    d.sub_dim_2 = c.sub_dim_2;
-    d.grid_occ_size = c.grid_occ_size;
+    d.gthr_x = fm.gthr_x; d.gthr_y = fm.gthr_y; d.gthr_z = fm.gthr_z;
+    d.gneg = fm.gneg_x | (fm.gneg_y << 1) | (fm.gneg_z << 2);
+    {
+        // |offset| < R  =>  |trunc| * (1 + up_dim_x + up_dim_2) < 2^31
+        const double lim = 2147483647.0 / (1.0 + (double)c.up_dim_x + (double)c.up_dim_2);
+        float R = (float)std::min(8388608.0, std::floor(lim));
+        if (R < 1.0f) R = 1.0f;
+        std::memcpy(&d.range_bits, &R, 4);
+        uint64_t neg[8];
+        for (int mask = 0; mask < 8; ++mask)
+            neg[mask] = ((mask & 1) ? fm.neg_x : 0ull) + ((mask & 2) ? fm.neg_y : 0ull) + ((mask & 4) ? fm.neg_z : 0ull);
+        TSL_TRY(m->neg_tab.reserve(sizeof(neg)));
+        TSL_HIP_CHECK(hipMemcpy(m->neg_tab.ptr, neg, sizeof(neg), hipMemcpyHostToDevice));
+        d.neg_tab = m->neg_tab.as<uint64_t>();
+    }
+    d.up_dim_x = (uint32_t)c.up_dim_x;
+    d.up_dim_2 = (uint32_t)c.up_dim_2;
+    d.occ = (uint32_t)c.grid_occ_size;
+    d.ds = (uint32_t)c.data_size;
+    d.sub_dim = (int32_t)c.sub_dim;
+    d.sub_dim_2 = (int32_t)c.sub_dim_2;
     d.data_size = c.data_size;
+    d.brick = brick ? 1u : 0u;
+    d.sd3 = (uint32_t)sd3;
+    {
+        const uint32_t B = (uint32_t)(sd / 4);
+        d.brick_cy = 64u * B - 16u;
+        d.brick_cz = 64u * B * B - 64u;
+        if (brick) {
+            magic_divisor((uint32_t)sd, &d.mag_sd, &d.sh_sd);
+            magic_divisor((uint32_t)(sd * sd), &d.mag_sd2, &d.sh_sd2);
+            magic_divisor((uint32_t)sd3, &d.mag_sd3, &d.sh_sd3);
+        }
+    }
     return TSL_OK;
 }
 
@@ -352,7 +423,8 @@ int tsl_map_set_data(tsl_map_coef* c, const float* cell_xyz, const float* cell_v
 int tsl_map_device_bytes(const tsl_map* m, uint64_t* bytes)
 {
     if (!m || !bytes) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
-    *bytes = m->grid_occ.bytes + m->codes.bytes + m->palette.bytes;
+    *bytes = m->grid_occ.bytes + m->codes.bytes + m->palette.bytes + m->sub_tab.bytes + m->full.bytes +
+             m->neg_tab.bytes;
     return TSL_OK;
 }
 
@@ -377,8 +449,12 @@ int tsl_filter_create(tsl_evaluator* ev, size_t capacity, tsl_filter** out)
     int rc = f->state.reserve(sizeof(FilterState));
     if (rc == TSL_OK && hipHostMalloc(&f->host_state, sizeof(FilterState), hipHostMallocDefault) != hipSuccess)
         rc = set_error(TSL_ERR_OUT_OF_MEMORY, "hipHostMalloc failed");
-    if (rc == TSL_OK && hipMemset(f->state.ptr, 0, sizeof(FilterState)) != hipSuccess)
-        rc = set_error(TSL_ERR_HIP, "hipMemset failed");
+    if (rc == TSL_OK) {
+        FilterState init{};
+        init.first_bad = 0xffffffffu;
+        if (hipMemcpy(f->state.ptr, &init, sizeof(FilterState), hipMemcpyHostToDevice) != hipSuccess)
+            rc = set_error(TSL_ERR_HIP, "hipMemcpy failed");
+    }
     if (rc == TSL_OK) rc = filter_reserve_particles(f, capacity ? capacity : 1);
     if (rc == TSL_OK) rc = filter_reserve_points(f, 1);
     if (rc != TSL_OK) {
@@ -396,6 +472,8 @@ int tsl_filter_destroy(tsl_filter* f)
     (void)hipSetDevice(f->device);
     (void)hipDeviceSynchronize();
     if (f->host_state) (void)hipHostFree(f->host_state);
+    if (f->ev_start) (void)hipEventDestroy(static_cast<hipEvent_t>(f->ev_start));
+    if (f->ev_stop) (void)hipEventDestroy(static_cast<hipEvent_t>(f->ev_stop));
     delete f;
     return TSL_OK;
 }
@@ -410,6 +488,7 @@ int tsl_filter_set_particles(tsl_filter* f, const tsl_particle* p, size_t n)
     TSL_TRY(filter_reserve_particles(f, n ? n : 1));
     if (n) TSL_HIP_CHECK(hipMemcpy(f->particles.ptr, p, sizeof(tsl_particle) * n, hipMemcpyHostToDevice));
     f->count_bound = n;
+    f->count_known = true;
     TSL_TRY(launch_set_count(f, (uint32_t)n, 0));
     TSL_HIP_CHECK(hipDeviceSynchronize());
     host_state(f)->count = (uint32_t)n;
@@ -457,11 +536,16 @@ int tsl_filter_resample_async(tsl_filter* f, float u0, void* stream)
     if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
     TSL_HIP_CHECK(hipSetDevice(f->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    // M must be known exactly on the host to build the draw table
+    // M must be known exactly on the host to build the draw table; after a
+    // previous resampling it is only known on the device
     FilterState* hs = host_state(f);
-    TSL_HIP_CHECK(hipMemcpyAsync(hs, f->state.ptr, sizeof(FilterState), hipMemcpyDeviceToHost, s));
-    TSL_HIP_CHECK(hipStreamSynchronize(s));
-    const size_t M = hs->count;
+    if (!f->count_known) {
+        TSL_HIP_CHECK(hipMemcpyAsync(hs, f->state.ptr, sizeof(FilterState), hipMemcpyDeviceToHost, s));
+        TSL_HIP_CHECK(hipStreamSynchronize(s));
+        f->count_bound = hs->count;
+        f->count_known = true;
+    }
+    const size_t M = f->count_bound;
     if (M == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "filter holds no particles");
     size_t cap = 0;
     TSL_TRY(resample_capacity(M, u0, &cap));
@@ -472,10 +556,98 @@ int tsl_filter_resample_async(tsl_filter* f, float u0, void* stream)
     FilterState* st = f->state.as<FilterState>();
     TSL_TRY(launch_resample_buffers(f->particles.as<tsl_particle>(), M, u0, cap, f->cumsum.as<double>(),
                                     f->ubuf.as<float>(), nullptr, f->particles_alt.as<tsl_particle>(),
-                                    &st->count, &st->status, s));
+                                    &st->count, &st->status, &st->first_bad, s));
     std::swap(f->particles.ptr, f->particles_alt.ptr);
     std::swap(f->particles.bytes, f->particles_alt.bytes);
     f->count_bound = cap;
+    f->count_known = false;
+    return TSL_OK;
+}
+
+int tsl_filter_snapshot(tsl_filter* f)
+{
+    if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    TSL_HIP_CHECK(hipMemcpy(f->host_state, f->state.ptr, sizeof(FilterState), hipMemcpyDeviceToHost));
+    const size_t n = host_state(f)->count;
+    TSL_TRY(f->snapshot.reserve(sizeof(tsl_particle) * (n ? n : 1)));
+    if (n) TSL_HIP_CHECK(hipMemcpy(f->snapshot.ptr, f->particles.ptr, sizeof(tsl_particle) * n,
+                                   hipMemcpyDeviceToDevice));
+    f->snapshot_count = n;
+    f->count_bound = n;
+    f->count_known = true;
+    return TSL_OK;
+}
+
+int tsl_filter_restore_async(tsl_filter* f, void* stream)
+{
+    if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
+    if (!f->snapshot.ptr) return set_error(TSL_ERR_INVALID_ARGUMENT, "filter has no snapshot");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    if (f->snapshot_count > f->capacity) TSL_TRY(filter_reserve_particles(f, f->snapshot_count));
+    TSL_TRY(launch_restore(f, static_cast<hipStream_t>(stream)));
+    f->count_bound = f->snapshot_count;
+    f->count_known = true;
+    return TSL_OK;
+}
+
+int tsl_filter_set_tile(tsl_filter* f, int k, uint32_t chunk_len)
+{
+    if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
+    if (k != 0 && k != 1 && k != 2) return set_error(TSL_ERR_INVALID_ARGUMENT, "K must be 0, 1 or 2");
+    f->tile_k = k;
+    f->tile_chunk = chunk_len;
+    return TSL_OK;
+}
+
+int tsl_filter_autotune(tsl_filter* f, void* stream, int reps, int* best_k, uint32_t* best_chunk, float* best_ms)
+{
+    if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    if (!f->count_known) {
+        FilterState* hs = host_state(f);
+        TSL_HIP_CHECK(hipMemcpy(hs, f->state.ptr, sizeof(FilterState), hipMemcpyDeviceToHost));
+        f->count_bound = hs->count;
+        f->count_known = true;
+    }
+    return autotune_tile(f, static_cast<hipStream_t>(stream), reps > 0 ? reps : 3, best_k, best_chunk, best_ms);
+}
+
+int tsl_filter_set_timing(tsl_filter* f, int enable)
+{
+    if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    if (enable && !f->ev_start) {
+        hipEvent_t a, b;
+        TSL_HIP_CHECK(hipEventCreate(&a));
+        TSL_HIP_CHECK(hipEventCreate(&b));
+        f->ev_start = a;
+        f->ev_stop = b;
+    }
+    f->timing = enable != 0;
+    f->timing_pending = false;
+    f->eval_ms = 0.0;
+    f->eval_launches = 0;
+    f->eval_pairs = 0;
+    return TSL_OK;
+}
+
+int tsl_filter_eval_timing(tsl_filter* f, double* ms, uint64_t* launches, uint64_t* pairs)
+{
+    if (!f || !ms || !launches || !pairs) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    if (f->timing_pending) {
+        float t = 0.0f;
+        TSL_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(f->ev_stop)));
+        TSL_HIP_CHECK(hipEventElapsedTime(&t, static_cast<hipEvent_t>(f->ev_start),
+                                          static_cast<hipEvent_t>(f->ev_stop)));
+        f->eval_ms += t;
+        f->timing_pending = false;
+    }
+    *ms = f->eval_ms;
+    *launches = f->eval_launches;
+    *pairs = f->eval_pairs;
     return TSL_OK;
 }
 
@@ -489,6 +661,7 @@ int tsl_filter_sync(tsl_filter* f, void* stream, tsl_pose* pose, size_t* count)
     TSL_HIP_CHECK(hipStreamSynchronize(s));
     if (count) *count = hs->count;
     f->count_bound = hs->count;
+    f->count_known = true;
     if (pose) pose_from_stats(hs->stats, pose);
     if (hs->status == 1u) return set_error(TSL_ERR_NO_VALID_PARTICLE, "No particle is valid!");
     if (hs->status == 2u) return set_error(TSL_ERR_CAPACITY, "resampling emitted more particles than expected");
@@ -576,8 +749,8 @@ int tsl_evaluate_raw(tsl_evaluator* ev, const tsl_particle* particles, size_t P,
     return TSL_OK;
 }
 
-int tsl_trace_indices(tsl_evaluator* ev, const tsl_particle* particles, size_t P, const tsl_point* points,
-                      size_t N, const float tf[16], uint64_t* out)
+static int trace_common(tsl_evaluator* ev, const tsl_particle* particles, size_t P, const tsl_point* points,
+                        size_t N, const float tf[16], void* out, size_t elem, int values)
 {
     if (!ev || !tf || (P && !particles) || (N && !points) || (P * N && !out))
         return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -587,11 +760,23 @@ int tsl_trace_indices(tsl_evaluator* ev, const tsl_particle* particles, size_t P
     TSL_TRY(tsl_filter_set_particles(f, particles, P));
     TSL_TRY(tsl_filter_set_scan(f, points, N, tf));
     DevBuf d_out;
-    TSL_TRY(d_out.reserve(sizeof(uint64_t) * P * N));
-    TSL_TRY(launch_trace(f, 0, d_out.as<uint64_t>()));
+    TSL_TRY(d_out.reserve(elem * P * N));
+    TSL_TRY(launch_trace(f, 0, d_out.ptr, values));
     TSL_HIP_CHECK(hipDeviceSynchronize());
-    TSL_HIP_CHECK(hipMemcpy(out, d_out.ptr, sizeof(uint64_t) * P * N, hipMemcpyDeviceToHost));
+    TSL_HIP_CHECK(hipMemcpy(out, d_out.ptr, elem * P * N, hipMemcpyDeviceToHost));
     return TSL_OK;
+}
+
+int tsl_trace_indices(tsl_evaluator* ev, const tsl_particle* particles, size_t P, const tsl_point* points,
+                      size_t N, const float tf[16], uint64_t* out)
+{
+    return trace_common(ev, particles, P, points, N, tf, out, sizeof(uint64_t), 0);
+}
+
+int tsl_trace_values(tsl_evaluator* ev, const tsl_particle* particles, size_t P, const tsl_point* points,
+                     size_t N, const float tf[16], float* out)
+{
+    return trace_common(ev, particles, P, points, N, tf, out, sizeof(float), 1);
 }
 
 int tsl_pose_matrices(const tsl_particle* particles, size_t P, const float tf[16], float* out)
@@ -626,11 +811,13 @@ int tsl_resample_systematic(const tsl_particle* in, size_t M, float u0, tsl_part
     TSL_TRY(d_s.reserve(sizeof(double) * M));
     TSL_TRY(d_u.reserve(sizeof(float) * cap));
     TSL_TRY(d_idx.reserve(sizeof(int32_t) * cap));
-    TSL_TRY(d_st.reserve(2 * sizeof(uint32_t)));
+    TSL_TRY(d_st.reserve(3 * sizeof(uint32_t)));
     TSL_HIP_CHECK(hipMemcpy(d_in.ptr, in, sizeof(tsl_particle) * M, hipMemcpyHostToDevice));
     uint32_t* st = d_st.as<uint32_t>();
+    const uint32_t init_state[3] = {0u, 0u, 0xffffffffu};
+    TSL_HIP_CHECK(hipMemcpy(st, init_state, sizeof(init_state), hipMemcpyHostToDevice));
     TSL_TRY(launch_resample_buffers(d_in.as<tsl_particle>(), M, u0, cap, d_s.as<double>(), d_u.as<float>(),
-                                    d_idx.as<int32_t>(), d_out.as<tsl_particle>(), st, st + 1, 0));
+                                    d_idx.as<int32_t>(), d_out.as<tsl_particle>(), st, st + 1, st + 2, 0));
     TSL_HIP_CHECK(hipDeviceSynchronize());
     uint32_t h[2];
     TSL_HIP_CHECK(hipMemcpy(h, st, sizeof(h), hipMemcpyDeviceToHost));

@@ -71,26 +71,29 @@ __global__ void k_pose(const tsl_particle* __restrict__ particles, const uint32_
     for (int k = 0; k < 12; ++k) L[(size_t)i * 12 + k] = out[k];
 }
 
-// Scan points -> (x, y, z, c) with c the point's constant score term
-// (tsdf_evaluator.cpp:56-65): |p|^2 is evaluated as the reference does.
-__global__ void k_prep_points(const tsl_point* __restrict__ in, uint32_t n, float range_term,
-                              float a_max, float mr2, float4* __restrict__ out)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float x = in[i].x, y = in[i].y, z = in[i].z;
-    const float d2 = x * x + y * y + z * z;
-    out[i] = make_float4(x, y, z, d2 < mr2 ? range_term : a_max);
-}
-
-template <int K>
+// One wavefront: K particles (pose matrices wavefront-uniform) x the points
+// [j0, j1) of one chunk, 64 consecutive points per iteration.  Each
+// iteration picks, by wavefront-uniform ballot, the cheapest lookup tier
+// every lane qualifies for (fast, else wrap, else per-lane wrap/general).
+// kMode 1 writes the voxel index of every pair, kMode 2 the voxel value,
+// instead of accumulating the score, with the same tiling, tier selection
+// and addressing as the production kernel (kMode 0); trace_out is indexed
+// by the caller's point order (perm).
+template <int K, int kMode, bool kU24, bool kBrick>
 __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __restrict__ L,
                                                  const uint32_t* __restrict__ count,
                                                  const float4* __restrict__ pts, uint32_t N,
                                                  uint32_t chunk_len, float a_hit,
-                                                 float* __restrict__ partial, uint32_t stride)
+                                                 float* __restrict__ partial, uint32_t stride,
+                                                 void* __restrict__ trace_out, const uint32_t* __restrict__ perm)
 {
-    extern __shared__ float lut[];
+    constexpr bool kTrace = kMode != 0;
+    extern __shared__ uint4 smem_raw[];
+    uint64_t* neg = reinterpret_cast<uint64_t*>(smem_raw);
+    SubEntry* tab = reinterpret_cast<SubEntry*>(neg + 8);
+    float* lut = reinterpret_cast<float*>(tab + map.sub_entries);
+    if (threadIdx.x < 8) neg[threadIdx.x] = map.neg_tab[threadIdx.x];
+    for (uint32_t i = threadIdx.x; i < map.sub_entries; i += blockDim.x) tab[i] = map.sub_tab[i];
     for (uint32_t i = threadIdx.x; i < map.palette_size; i += blockDim.x) lut[i] = map.palette[i];
     __syncthreads();
 
@@ -113,18 +116,86 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0f;
 
-    for (uint32_t j = j0 + lane; j < j1; j += 64) {
-        const float4 q = pts[j];
+    for (uint32_t base = j0; base < j1; base += 64) {
+        const uint32_t j = base + lane;
+        const bool act = j < j1;
+        const float4 q = pts[act ? j : j0];
+        float ox[K], oy[K], oz[K];
+        bool fast = true;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
+            // tsdf_evaluator.cpp:44-46: ((L0*x + L1*y) + L2*z) + L3, no FMA
             const float tx = m[k][0] * q.x + m[k][1] * q.y + m[k][2] * q.z + m[k][3];
             const float ty = m[k][4] * q.x + m[k][5] * q.y + m[k][6] * q.z + m[k][7];
             const float tz = m[k][8] * q.x + m[k][9] * q.y + m[k][10] * q.z + m[k][11];
-            const uint64_t idx = lookup_index(map, tx, ty, tz);
-            const float v = idx < map.data_size ? lut[map.codes[idx]] : map.init;
-            acc[k] += a_hit * v + q.w;
+            ox[k] = tx - map.min_x;
+            oy[k] = ty - map.min_y;
+            oz[k] = tz - map.min_z;
+            fast = fast & fast_ok(ox[k], oy[k], oz[k], map.range_bits);
+        }
+        float v[K];
+        uint64_t full[K];
+        if (__builtin_amdgcn_ballot_w64(!fast) == 0) {
+            int32_t s[K];
+            uint32_t small[K], pbrick[K];
+            bool carry[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                fast_block<kU24, kBrick>(map, tab, ox[k], oy[k], oz[k], s[k], small[k], pbrick[k], carry[k]);
+            uint32_t code[K];
+            bool valid[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t sum = (uint32_t)s[k] + small[k];
+                valid[k] = (s[k] >= 0) & (sum < map.ds);
+                if (kTrace) full[k] = s[k] >= 0 ? (uint64_t)(uint32_t)s[k] + small[k] : map.data_size;
+                uint32_t addr = (uint32_t)s[k] + pbrick[k];
+                if (kBrick && __builtin_expect(carry[k], 0)) addr = phys_from_linear(map, sum);
+                code[k] = 0;
+                if (valid[k]) code[k] = map.codes[addr];
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) v[k] = valid[k] ? lut[code[k]] : map.init;
+        } else {
+            bool wrap = true;
+#pragma unroll
+            for (int k = 0; k < K; ++k) wrap = wrap & wrap_ok(ox[k], oy[k], oz[k], map.range_bits);
+            if (__builtin_amdgcn_ballot_w64(!wrap) == 0) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) full[k] = wrap_index<kU24>(map, tab, neg, ox[k], oy[k], oz[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if (wrap_ok(ox[k], oy[k], oz[k], map.range_bits))
+                        full[k] = wrap_index<kU24>(map, tab, neg, ox[k], oy[k], oz[k]);
+                    else
+                        full[k] = lookup_index_general(*map.full, ox[k], oy[k], oz[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                v[k] = map.init;
+                if (full[k] < map.data_size) {
+                    const uint32_t x = (uint32_t)full[k];
+                    v[k] = lut[map.codes[kBrick ? phys_from_linear(map, x) : x]];
+                }
+            }
+        }
+        if (kMode == 1) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (act && p0 + k < P) static_cast<uint64_t*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = full[k];
+        } else if (kMode == 2) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (act && p0 + k < P) static_cast<float*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (act) acc[k] += a_hit * v[k] + q.w;  // tsdf_evaluator.cpp:56-67
         }
     }
+    if (kTrace) return;
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = wave_sum(acc[k]);
     if (lane == 0) {
@@ -161,13 +232,17 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const float* __restrict__ par
         last = (done == gridDim.x - 1);
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
+    if (last) {
         __threadfence();
-        double total = 0.0;
-        for (uint32_t k = 0; k < gridDim.x; ++k) total += __builtin_nontemporal_load(&blocks[k]);
-        st->stats[0] = total;
-        st->status = ((float)total == 0.0f) ? 1u : 0u;
-        st->counter0 = 0;
+        // fixed order: thread t adds partials t, t+256, ...; then the block tree
+        double t = 0.0;
+        for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) t += *(volatile const double*)&blocks[k];
+        const double total = block_sum_d(t, sh);
+        if (threadIdx.x == 0) {
+            st->stats[0] = total;
+            st->status = ((float)total == 0.0f) ? 1u : 0u;
+            st->counter0 = 0;
+        }
     }
 }
 
@@ -214,29 +289,15 @@ __global__ __launch_bounds__(kBlock) void k_normalize(const float* __restrict__ 
     __syncthreads();
     if (last) {
         __threadfence();
-        if (threadIdx.x < 9) {
+        for (int q = 0; q < 9; ++q) {
             double t = 0.0;
-            for (uint32_t k = 0; k < gridDim.x; ++k)
-                t += __builtin_nontemporal_load(&blocks[(size_t)threadIdx.x * gridDim.x + k]);
-            st->stats[1 + threadIdx.x] = t;
+            for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x)
+                t += *(volatile const double*)&blocks[(size_t)q * gridDim.x + k];
+            const double total = block_sum_d(t, sh);
+            if (threadIdx.x == 0) st->stats[1 + q] = total;
         }
-        __syncthreads();
         if (threadIdx.x == 0) st->counter1 = 0;
     }
-}
-
-__global__ void k_trace(DevMap map, const float* __restrict__ L, uint32_t P,
-                        const float4* __restrict__ pts, uint32_t N, uint64_t* __restrict__ out)
-{
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)P * N) return;
-    const uint32_t p = (uint32_t)(t / N), j = (uint32_t)(t % N);
-    const float* m = L + (size_t)p * 12;
-    const float4 q = pts[j];
-    const float tx = m[0] * q.x + m[1] * q.y + m[2] * q.z + m[3];
-    const float ty = m[4] * q.x + m[5] * q.y + m[6] * q.z + m[7];
-    const float tz = m[8] * q.x + m[9] * q.y + m[10] * q.z + m[11];
-    out[t] = lookup_index(map, tx, ty, tz);
 }
 
 // Copy the device count into the filter state (count comes from the host).
@@ -245,34 +306,78 @@ __global__ void k_set_count(FilterState* st, uint32_t n)
     st->count = n;
 }
 
-struct Tile {
-    int K;
-    uint32_t chunk_len;
-    uint32_t chunks;
-};
+// Restore the resident set from the snapshot and reset the count.
+__global__ void k_restore(const tsl_particle* __restrict__ src, tsl_particle* __restrict__ dst, uint32_t n,
+                          FilterState* st)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+    if (i == 0) st->count = n;
+}
 
-// Tile shape: enough (particle-group, point-chunk) wavefronts to fill 256
-// CUs several times over, chunks no shorter than 512 points.
-Tile choose_tile(size_t P, size_t N, int per_point)
+// Tile shape: K particles per wavefront (template instantiations 1, 2)
+// and the scan chunk a wavefront walks.  Automatic choice: K = 1 (fewest
+// scalar registers, full occupancy on gfx950) and enough point chunks to
+// give the 256 CUs at least four full rounds of wavefronts.
+Tile choose_tile(const tsl_filter* f, size_t P, size_t N)
 {
     Tile t;
-    t.K = 4;
+    t.K = f->tile_k ? f->tile_k : 1;
     const size_t groups = (P + t.K - 1) / t.K;
-    const size_t target_waves = 256 * 4 * 8;
-    size_t chunks = 1;
-    if (per_point || groups < target_waves) {
-        chunks = (target_waves + groups - 1) / groups;
-        const size_t max_chunks = (N + 511) / 512;
-        if (chunks > max_chunks) chunks = max_chunks;
-        if (chunks < 1) chunks = 1;
+    size_t len;
+    if (f->tile_chunk) {
+        len = f->tile_chunk;
+    } else {
+        const size_t target_waves = 256 * 4 * 8 * 4;
+        size_t chunks = 1;
+        if (f->ev->per_point || groups < target_waves) {
+            chunks = (target_waves + groups - 1) / groups;
+            const size_t max_chunks = (N + 255) / 256;
+            if (chunks > max_chunks) chunks = max_chunks;
+            if (chunks < 1) chunks = 1;
+        }
+        len = (N + chunks - 1) / chunks;
     }
-    size_t len = (N + chunks - 1) / chunks;
     len = (len + 63) / 64 * 64;
     if (len == 0) len = 64;
     t.chunk_len = (uint32_t)len;
     t.chunks = (uint32_t)((N + len - 1) / len);
     if (t.chunks == 0) t.chunks = 1;
     return t;
+}
+
+template <int kMode>
+int launch_eval_kernel(const Tile& tile, const tsl_filter* f, size_t P, size_t N, hipStream_t stream,
+                       float* partial, void* trace_out)
+{
+    const tsl_map* map = f->ev->map;
+    FilterState* st = f->state.as<FilterState>();
+    const size_t groups = (P + tile.K - 1) / tile.K;
+    dim3 grid((unsigned)((groups + kWaves - 1) / kWaves), tile.chunks);
+    const size_t lds = 8 * sizeof(uint64_t) + sizeof(SubEntry) * map->dev.sub_entries +
+                       sizeof(float) * map->dev.palette_size;
+    const bool u24 = map->dev.up_dim_x < (1u << 23) && map->dev.up_dim_2 < (1u << 23) &&
+                     map->dev.sub_dim_2 < (1 << 23);
+    const bool brick = map->dev.brick != 0;
+#define TSL_LAUNCH_EVAL(KK, U24, BR)                                                                            \
+    hipLaunchKernelGGL((k_eval<KK, kMode, U24, BR>), grid, dim3(kBlock), lds, stream, map->dev, f->L.as<float>(), \
+                       &st->count, f->points.as<float4>(), (uint32_t)N, tile.chunk_len, f->ev->a_hit, partial,     \
+                       (uint32_t)f->capacity, trace_out, f->perm.as<uint32_t>())
+    const int sel = tile.K * 4 + (u24 ? 2 : 0) + (brick ? 1 : 0);
+    switch (sel) {
+    case 4: TSL_LAUNCH_EVAL(1, false, false); break;
+    case 5: TSL_LAUNCH_EVAL(1, false, true); break;
+    case 6: TSL_LAUNCH_EVAL(1, true, false); break;
+    case 7: TSL_LAUNCH_EVAL(1, true, true); break;
+    case 8: TSL_LAUNCH_EVAL(2, false, false); break;
+    case 9: TSL_LAUNCH_EVAL(2, false, true); break;
+    case 10: TSL_LAUNCH_EVAL(2, true, false); break;
+    case 11: TSL_LAUNCH_EVAL(2, true, true); break;
+    default: return set_error(TSL_ERR_INVALID_ARGUMENT, "tile K must be 1 or 2 (got %d)", tile.K);
+    }
+#undef TSL_LAUNCH_EVAL
+    TSL_HIP_CHECK(hipGetLastError());
+    return TSL_OK;
 }
 
 }  // namespace
@@ -289,18 +394,6 @@ int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const floa
     return TSL_OK;
 }
 
-int launch_prep_points(tsl_filter* f, hipStream_t stream)
-{
-    const uint32_t n = (uint32_t)f->num_points;
-    if (n == 0) return TSL_OK;
-    const tsl_evaluator* ev = f->ev;
-    hipLaunchKernelGGL(k_prep_points, dim3((n + 255) / 256), dim3(256), 0, stream,
-                       f->points_in.as<tsl_point>(), n, ev->range_term, ev->a_max,
-                       ev->max_range_squared, f->points.as<float4>());
-    TSL_HIP_CHECK(hipGetLastError());
-    return TSL_OK;
-}
-
 static int launch_eval_core(tsl_filter* f, hipStream_t stream, Tile& tile)
 {
     const size_t P = f->count_bound;
@@ -311,16 +404,86 @@ static int launch_eval_core(tsl_filter* f, hipStream_t stream, Tile& tile)
     hipLaunchKernelGGL(k_pose, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, stream,
                        f->particles.as<tsl_particle>(), &st->count, (uint32_t)P, t, f->L.as<float>());
     TSL_HIP_CHECK(hipGetLastError());
-    tile = choose_tile(P, N, f->ev->per_point);
+    tile = choose_tile(f, P, N);
     TSL_TRY(f->partial.reserve(sizeof(float) * (size_t)tile.chunks * f->capacity));
-    const tsl_map* map = f->ev->map;
-    const size_t groups = (P + tile.K - 1) / tile.K;
-    dim3 grid((unsigned)((groups + kWaves - 1) / kWaves), tile.chunks);
-    const size_t lds = sizeof(float) * map->dev.palette_size;
-    hipLaunchKernelGGL(k_eval<4>, grid, dim3(kBlock), lds, stream, map->dev, f->L.as<float>(),
-                       &st->count, f->points.as<float4>(), (uint32_t)N, tile.chunk_len, f->ev->a_hit,
-                       f->partial.as<float>(), (uint32_t)f->capacity);
-    TSL_HIP_CHECK(hipGetLastError());
+    if (f->timing) {
+        if (f->timing_pending) {
+            // fold the previous bracket in before reusing the events
+            float ms = 0.0f;
+            TSL_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(f->ev_stop)));
+            TSL_HIP_CHECK(hipEventElapsedTime(&ms, static_cast<hipEvent_t>(f->ev_start),
+                                              static_cast<hipEvent_t>(f->ev_stop)));
+            f->eval_ms += ms;
+            f->timing_pending = false;
+        }
+        TSL_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(f->ev_start), stream));
+    }
+    TSL_TRY(launch_eval_kernel<0>(tile, f, P, N, stream, f->partial.as<float>(), nullptr));
+    if (f->timing) {
+        TSL_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(f->ev_stop), stream));
+        f->timing_pending = true;
+        f->eval_launches += 1;
+        f->eval_pairs += (uint64_t)P * N;
+    }
+    return TSL_OK;
+}
+
+// Time every candidate tile shape on the current particles and scan; keep
+// the fastest.  Candidates: K in {1, 2} x {automatic chunks, 1/2/4/8
+// chunks of the scan}.
+int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_k, uint32_t* best_chunk, float* best_ms)
+{
+    const size_t P = f->count_bound;
+    const size_t N = f->num_points;
+    if (P == 0 || N == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "autotune needs particles and a scan");
+    const int saved_k = f->tile_k;
+    const uint32_t saved_chunk = f->tile_chunk;
+    TSL_TRY(launch_pose_matrices(f->particles.as<tsl_particle>(), (uint32_t)P, f->tf, f->L.as<float>(), stream));
+    hipEvent_t a, b;
+    TSL_HIP_CHECK(hipEventCreate(&a));
+    TSL_HIP_CHECK(hipEventCreate(&b));
+    float best = 1e30f;
+    int bk = 2;
+    uint32_t bc = 0;
+    const int ks[2] = {1, 2};
+    const uint32_t div[5] = {0, 1, 2, 4, 8};
+    int rc = TSL_OK;
+    for (int ki = 0; ki < 2 && rc == TSL_OK; ++ki) {
+        for (int ci = 0; ci < 5 && rc == TSL_OK; ++ci) {
+            f->tile_k = ks[ki];
+            f->tile_chunk = div[ci] ? (uint32_t)((N + div[ci] - 1) / div[ci]) : 0u;
+            const Tile tile = choose_tile(f, P, N);
+            rc = f->partial.reserve(sizeof(float) * (size_t)tile.chunks * f->capacity);
+            if (rc != TSL_OK) break;
+            rc = launch_eval_kernel<0>(tile, f, P, N, stream, f->partial.as<float>(), nullptr);  // warm-up
+            if (rc != TSL_OK) break;
+            (void)hipEventRecord(a, stream);
+            for (int r = 0; r < reps && rc == TSL_OK; ++r)
+                rc = launch_eval_kernel<0>(tile, f, P, N, stream, f->partial.as<float>(), nullptr);
+            (void)hipEventRecord(b, stream);
+            (void)hipEventSynchronize(b);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            ms /= (float)(reps > 0 ? reps : 1);
+            if (ms < best) {
+                best = ms;
+                bk = ks[ki];
+                bc = f->tile_chunk;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc != TSL_OK) {
+        f->tile_k = saved_k;
+        f->tile_chunk = saved_chunk;
+        return rc;
+    }
+    f->tile_k = bk;
+    f->tile_chunk = bc;
+    if (best_k) *best_k = bk;
+    if (best_chunk) *best_chunk = bc;
+    if (best_ms) *best_ms = best;
     return TSL_OK;
 }
 
@@ -357,17 +520,24 @@ int launch_raw(tsl_filter* f, hipStream_t stream, float* raw_out_device)
     return TSL_OK;
 }
 
-int launch_trace(tsl_filter* f, hipStream_t stream, uint64_t* out_device)
+int launch_trace(tsl_filter* f, hipStream_t stream, void* out_device, int values)
 {
     const size_t P = f->count_bound;
     const size_t N = f->num_points;
     if (P == 0 || N == 0) return TSL_OK;
     TSL_TRY(launch_pose_matrices(f->particles.as<tsl_particle>(), (uint32_t)P, f->tf, f->L.as<float>(),
                                  stream));
-    const uint64_t total = (uint64_t)P * N;
-    hipLaunchKernelGGL(k_trace, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
-                       f->ev->map->dev, f->L.as<float>(), (uint32_t)P, f->points.as<float4>(),
-                       (uint32_t)N, out_device);
+    const Tile tile = choose_tile(f, P, N);
+    if (values) return launch_eval_kernel<2>(tile, f, P, N, stream, nullptr, out_device);
+    return launch_eval_kernel<1>(tile, f, P, N, stream, nullptr, out_device);
+}
+
+int launch_restore(tsl_filter* f, hipStream_t stream)
+{
+    const uint32_t n = (uint32_t)f->snapshot_count;
+    hipLaunchKernelGGL(k_restore, dim3((n + 255) / 256 ? (n + 255) / 256 : 1), dim3(256), 0, stream,
+                       f->snapshot.as<tsl_particle>(), f->particles.as<tsl_particle>(), n,
+                       f->state.as<FilterState>());
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
 }

@@ -52,31 +52,24 @@ __device__ __forceinline__ double wave_incl_scan(double v, int lane)
     return v;
 }
 
-__global__ __launch_bounds__(kRsThreads) void k_resample(
-    const tsl_particle* __restrict__ in, uint32_t M, const USegTable segs, uint32_t nseg,
-    uint32_t cap, double inv_m, float u0, double* __restrict__ s, float* __restrict__ U,
-    int32_t* __restrict__ idx_out, tsl_particle* __restrict__ out, uint32_t* __restrict__ count_out,
-    uint32_t* __restrict__ status_out)
+// Phase A (one workgroup): s_m, the double running sum, candidate by block
+// scan, checked element by element, repaired serially from the first
+// mismatch.
+__global__ __launch_bounds__(kRsThreads) void k_rs_scan(const tsl_particle* __restrict__ in, uint32_t M,
+                                                        double* __restrict__ s)
 {
     __shared__ double wave_tot[kRsThreads / 64];
     __shared__ double carry_sh;
     __shared__ uint32_t first_bad;
-    __shared__ uint32_t jcount;
-    __shared__ USeg seg_sh[kMaxSeg];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     constexpr int kItems = 4;
     constexpr uint32_t kChunk = kRsThreads * kItems;
-
-    for (uint32_t k = tid; k < nseg; k += kRsThreads) seg_sh[k] = segs.seg[k];
     if (tid == 0) {
         carry_sh = 0.0;
         first_bad = 0xffffffffu;
-        jcount = 0;
     }
     __syncthreads();
-
-    // 1. candidate running sum
     for (uint32_t base = 0; base < M; base += kChunk) {
         double loc[kItems];
         double run = 0.0;
@@ -107,16 +100,12 @@ __global__ __launch_bounds__(kRsThreads) void k_resample(
         }
         __syncthreads();
     }
-    __threadfence_block();
-    __syncthreads();
-
-    // 2. check s_m == s_{m-1} + w_m
+    // check s_m == s_{m-1} + w_m
     for (uint32_t m = tid; m < M; m += kRsThreads) {
         const double prev = m == 0 ? 0.0 : s[m - 1];
         if (s[m] != prev + (double)in[m].second) atomicMin(&first_bad, m);
     }
     __syncthreads();
-    // 3. serial repair
     if (first_bad != 0xffffffffu && tid == 0) {
         double acc = first_bad == 0 ? 0.0 : s[first_bad - 1];
         for (uint32_t m = first_bad; m < M; ++m) {
@@ -124,63 +113,80 @@ __global__ __launch_bounds__(kRsThreads) void k_resample(
             s[m] = acc;
         }
     }
-    __syncthreads();
-    if (tid == 0) first_bad = 0xffffffffu;
-    __syncthreads();
+}
 
-    // 4. candidate U from the run table
-    for (uint32_t j = tid; j < cap; j += kRsThreads) {
-        uint32_t lo = 0, hi = nseg;  // last segment with j0 <= j
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (seg_sh[mid].j0 <= j) lo = mid; else hi = mid;
-        }
-        const USeg sg = seg_sh[lo];
-        U[j] = (float)((double)sg.u + (double)(j - sg.j0) * sg.step);
+__device__ __forceinline__ float u_candidate(const USegTable& segs, uint32_t nseg, uint32_t j)
+{
+    uint32_t lo = 0, hi = nseg;  // last run with j0 <= j
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (segs.seg[mid].j0 <= j) lo = mid; else hi = mid;
     }
-    __syncthreads();
-    // 5. check U_j == float(U_{j-1} + 1/M)
-    for (uint32_t j = tid; j < cap; j += kRsThreads) {
-        const float expect = j == 0 ? u0 : (float)((double)U[j - 1] + inv_m);
-        if (U[j] != expect) atomicMin(&first_bad, j);
-    }
-    __syncthreads();
-    // 6. serial repair
-    if (first_bad != 0xffffffffu && tid == 0) {
-        float u = first_bad == 0 ? u0 : (float)((double)U[first_bad - 1] + inv_m);
-        for (uint32_t j = first_bad; j < cap; ++j) {
+    const USeg sg = segs.seg[lo];
+    return (float)((double)sg.u + (double)(j - sg.j0) * sg.step);
+}
+
+// Phase B (grid): U_j candidates from the run table, each checked against one
+// step of the recurrence from its predecessor.
+__global__ void k_rs_u(const USegTable segs, uint32_t nseg, uint32_t cap, double inv_m, float u0,
+                       float* __restrict__ U, uint32_t* __restrict__ first_bad)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cap) return;
+    const float u = u_candidate(segs, nseg, j);
+    U[j] = u;
+    const float expect = j == 0 ? u0 : (float)((double)u_candidate(segs, nseg, j - 1) + inv_m);
+    if (u != expect) atomicMin(first_bad, j);
+}
+
+// Phase C (one thread): serial repair of U from the first mismatch, then the
+// number emitted J = #{j : U_j < s_{M-1}} (U is non-decreasing).
+__global__ void k_rs_count(uint32_t M, const double* __restrict__ s, float* __restrict__ U, uint32_t cap,
+                           double inv_m, float u0, uint32_t* __restrict__ first_bad, uint32_t* __restrict__ count_out,
+                           uint32_t* __restrict__ status_out)
+{
+    if (threadIdx.x != 0) return;
+    const uint32_t fb = *first_bad;
+    if (fb != 0xffffffffu) {
+        float u = fb == 0 ? u0 : (float)((double)U[fb - 1] + inv_m);
+        for (uint32_t j = fb; j < cap; ++j) {
             U[j] = u;
             u = (float)((double)u + inv_m);
         }
     }
-    __syncthreads();
-
-    // 7. J = #{j : U_j < s_final}
-    const double s_final = M ? s[M - 1] : 0.0;
-    uint32_t c = 0;
-    for (uint32_t j = tid; j < cap; j += kRsThreads) c += ((double)U[j] < s_final) ? 1u : 0u;
-    atomicAdd(&jcount, c);
-    __syncthreads();
-    const uint32_t J = jcount;
-    const bool overflow = cap > 0 && (double)U[cap - 1] < s_final;
-    if (tid == 0) {
-        *status_out = overflow ? 2u : 0u;
-        *count_out = overflow ? 0u : J;
+    *first_bad = 0xffffffffu;
+    const double s_final = s[M - 1];
+    if (cap == 0 || (double)U[cap - 1] < s_final) {
+        *status_out = 2u;
+        *count_out = 0u;
+        return;
     }
-    if (overflow) return;
-
-    // 8. m(j) = min{m : s_m > U_j}; gather
-    for (uint32_t j = tid; j < J; j += kRsThreads) {
-        const double u = (double)U[j];
-        uint32_t lo = 0, hi = M;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s[mid] > u) hi = mid; else lo = mid + 1;
-        }
-        const uint32_t m = lo < M ? lo : M - 1;
-        if (idx_out) idx_out[j] = (int32_t)m;
-        if (out) out[j] = in[m];
+    uint32_t lo = 0, hi = cap;  // first j with U_j >= s_final
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((double)U[mid] < s_final) lo = mid + 1; else hi = mid;
     }
+    *status_out = 0u;
+    *count_out = lo;
+}
+
+// Phase D (grid): output j is particle min{m : s_m > U_j}.
+__global__ void k_rs_assign(const tsl_particle* __restrict__ in, uint32_t M, const double* __restrict__ s,
+                            const float* __restrict__ U, const uint32_t* __restrict__ count,
+                            const uint32_t* __restrict__ status, int32_t* __restrict__ idx_out,
+                            tsl_particle* __restrict__ out)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*status != 0u || j >= *count) return;
+    const double u = (double)U[j];
+    uint32_t lo = 0, hi = M;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s[mid] > u) hi = mid; else lo = mid + 1;
+    }
+    const uint32_t m = lo < M ? lo : M - 1;
+    if (idx_out) idx_out[j] = (int32_t)m;
+    if (out) out[j] = in[m];
 }
 
 float rec_step(float u, double inv_m) { return (float)((double)u + inv_m); }
@@ -266,7 +272,7 @@ int resample_capacity(size_t M, float u0, size_t* cap_out)
 
 int launch_resample_buffers(const tsl_particle* d_in, size_t M, float u0, size_t cap, double* d_s,
                             float* d_u, int32_t* d_idx, tsl_particle* d_out, uint32_t* d_count,
-                            uint32_t* d_status, hipStream_t stream)
+                            uint32_t* d_status, uint32_t* d_first_bad, hipStream_t stream)
 {
     if (M == 0 || M > 0x7fffffffu || cap > 0xffffffffu)
         return set_error(TSL_ERR_INVALID_ARGUMENT, "resample: M=%zu out of range", M);
@@ -274,9 +280,18 @@ int launch_resample_buffers(const tsl_particle* d_in, size_t M, float u0, size_t
     TSL_TRY(build_segments(M, u0, (uint32_t)cap, segs));
     USegTable table{};
     for (size_t k = 0; k < segs.size(); ++k) table.seg[k] = segs[k];
-    hipLaunchKernelGGL(k_resample, dim3(1), dim3(kRsThreads), 0, stream, d_in, (uint32_t)M, table,
-                       (uint32_t)segs.size(), (uint32_t)cap, 1.0 / (double)M, u0, d_s, d_u, d_idx, d_out,
-                       d_count, d_status);
+    const double inv_m = 1.0 / (double)M;
+    const unsigned gc = (unsigned)((cap + 255) / 256);
+    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(kRsThreads), 0, stream, d_in, (uint32_t)M, d_s);
+    TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_rs_u, dim3(gc ? gc : 1), dim3(256), 0, stream, table, (uint32_t)segs.size(), (uint32_t)cap,
+                       inv_m, u0, d_u, d_first_bad);
+    TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_rs_count, dim3(1), dim3(64), 0, stream, (uint32_t)M, d_s, d_u, (uint32_t)cap, inv_m, u0,
+                       d_first_bad, d_count, d_status);
+    TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_rs_assign, dim3(gc ? gc : 1), dim3(256), 0, stream, d_in, (uint32_t)M, d_s, d_u, d_count,
+                       d_status, d_idx, d_out);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
 }

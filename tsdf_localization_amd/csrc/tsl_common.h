@@ -70,6 +70,11 @@ TSL_HD void sincosf_exact(float xf, float* s_out, float* c_out)
         *c_out = __builtin_nanf("");
         return;
     }
+    if (x == 0.0) {  // sin(+-0) = +-0
+        *s_out = xf;
+        *c_out = 1.0f;
+        return;
+    }
     const double kd = rint_d(x * 0x1.45f306dc9c883p-1);
     const double r = ((x - kd * 0x1.921fb54400000p+0) - kd * 0x1.0b4611a600000p-34) -
                      kd * 0x1.3198a2e037073p-69;

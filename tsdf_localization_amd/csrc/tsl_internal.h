@@ -14,6 +14,7 @@ namespace tsl {
 
 constexpr uint32_t kMaxPalette = 4096;  // 16-bit voxel codes index at most this many values
 constexpr uint32_t kFastLimit = 8388608u;  // 2^23: offsets below this take the 32-bit lookup
+constexpr uint32_t kMaxSubTable = 1024;    // buckets of the sub-voxel threshold table
 
 // ---------------------------------------------------------------------------
 // Error state
@@ -74,23 +75,68 @@ struct DevBuf {
 // Device view of a map: the reference's MapCoef fields plus the palette-coded
 // voxel array and the exact fast-path thresholds derived on the host.
 // ---------------------------------------------------------------------------
-struct DevMap {
-    const int32_t* grid_occ;   // block table: voxel offset of the block or -1
-    const uint16_t* codes;     // data_size palette codes, reference voxel order
-    const float* palette;      // palette_size values
-    uint32_t palette_size;
-    float min_x, min_y, min_z;
+struct SubEntry {
+    uint32_t thr;    // bit pattern of the threshold inside the bucket (+inf if none)
+    uint32_t count;  // thresholds below the bucket
+};
+
+struct SubTable {
+    uint32_t shift = 23;
+    uint32_t base_bits = 0;
+    std::vector<SubEntry> entries;
+};
+
+// Every field the exact tiers of the lookup need (64-bit reference sizes,
+// the wrap-band constants).  Lives in device memory; the kernels read it
+// only when a wavefront leaves the fast tier.
+struct DevMapFull {
+    const int32_t* grid_occ;
     float res;
-    float init;
-    uint32_t gthr_x, gthr_y, gthr_z;  // u < gthr  <=>  trunc(fl(u/res)) < dim, for u < 2^23
+    uint32_t gneg_x, gneg_y, gneg_z;  // grid check result of an axis offset in (-2^23, -1]
     uint64_t dim_x, dim_y, dim_z;
     uint64_t up_dim_x, up_dim_2;
     uint64_t sub_dim, sub_dim_2;
     uint64_t grid_occ_size, data_size;
+    uint64_t neg_x, neg_y, neg_z;     // index contribution of an axis offset in (-2^23, -1]
+    uint32_t gthr_x, gthr_y, gthr_z;
+};
+
+// Kernel-argument view of a map: what the fast tier reads every iteration.
+struct DevMap {
+    const int32_t* grid_occ;   // block table: voxel offset of the block or -1
+    const uint16_t* codes;     // data_size palette codes, reference voxel order
+    const float* palette;      // palette_size values
+    const SubEntry* sub_tab;   // sub-voxel threshold buckets
+    const uint64_t* neg_tab;   // index contribution of the wrap-band axes, by axis bit mask (8)
+    const DevMapFull* full;    // device copy of the exact-tier fields
+    uint32_t palette_size, sub_entries, sub_shift, sub_base;
+    float min_x, min_y, min_z;
+    float init;
+    uint32_t gthr_x, gthr_y, gthr_z;  // u < gthr  <=>  trunc(fl(u/res)) < dim, for u < 2^23
+    uint32_t gneg;                    // bit a: grid check passes for an axis-a offset in (-2^23, -1]
+    uint32_t range_bits;              // bit pattern of R: |offset| < R keeps every block index in int32
+    uint32_t up_dim_x, up_dim_2;      // 32-bit copies (block table < 2^31 entries)
+    uint32_t occ, ds;                 // grid_occ_size, data_size (< 2^31)
+    int32_t sub_dim, sub_dim_2;
+    uint64_t data_size;               // == ds, for the index comparisons of the exact tiers
+    // 4x4x4 brick layout of the codes (set when block offsets are aligned
+    // multiples of sub_dim^3 and sub_dim % 4 == 0): physical address of a
+    // voxel with reference index idx = slot*sd3 + lx + ly*sd + lz*sd^2 is
+    // slot*sd3 + ((lx>>2) + (ly>>2)*B + (lz>>2)*B^2)*64 + (lx&3) + (ly&3)*4 + (lz&3)*16.
+    uint32_t brick;                   // 1: bricked codes
+    uint32_t sd3;                     // sub_dim^3
+    uint32_t brick_cy, brick_cz;      // 64B - 16, 64B^2 - 64
+    uint32_t mag_sd, sh_sd, mag_sd2, sh_sd2, mag_sd3, sh_sd3;  // x / d = umulhi(x, mag) >> sh, x < 2^31
 };
 
 // Smallest u in [0, 2^23] with trunc(fl(fl(u)/res)) >= dim; host IEEE division.
 uint32_t grid_threshold(uint64_t dim, float res);
+
+// tables.hip
+int build_sub_table(float res, SubTable* out);
+void magic_divisor(uint32_t d, uint32_t* mag, uint32_t* sh);
+uint32_t sub_lookup_host(const SubTable& t, float a);
+void negative_band_constants(const tsl_map_coef& c, DevMapFull& d);
 
 }  // namespace tsl
 
@@ -100,7 +146,7 @@ uint32_t grid_threshold(uint64_t dim, float res);
 struct tsl_map {
     int device = 0;
     tsl_map_coef coef{};
-    tsl::DevBuf grid_occ, codes, palette;
+    tsl::DevBuf grid_occ, codes, palette, sub_tab, neg_tab, full;
     tsl::DevMap dev{};
 };
 
@@ -118,6 +164,18 @@ struct tsl_filter {
     int device = 0;
     size_t capacity = 0;     // particles the buffers hold
     size_t count_bound = 0;  // host-side upper bound of the device count
+    bool count_known = true;  // count_bound is the exact device count
+    size_t snapshot_count = 0;
+    tsl::DevBuf snapshot;    // tsl_particle[snapshot_count]
+    bool timing = false;     // bracket the evaluation kernel with events
+    void* ev_start = nullptr;
+    void* ev_stop = nullptr;
+    double eval_ms = 0.0;    // accumulated evaluation-kernel time
+    uint64_t eval_launches = 0;
+    uint64_t eval_pairs = 0;  // particle*point evaluations launched
+    bool timing_pending = false;
+    int tile_k = 0;            // particles per wavefront (0 = automatic)
+    uint32_t tile_chunk = 0;   // scan points per wavefront (0 = automatic)
     size_t num_points = 0;
     float tf[16] = {};
     tsl::DevBuf particles;   // tsl_particle[capacity]
@@ -126,7 +184,9 @@ struct tsl_filter {
     tsl::DevBuf raw;         // float[capacity]
     tsl::DevBuf partial;     // float[chunks * capacity]
     tsl::DevBuf points;      // float4[num_points]: x, y, z, range term
-    tsl::DevBuf points_in;   // tsl_point staging
+    tsl::DevBuf points_in;   // tsl_point staging (caller order)
+    tsl::DevBuf perm;        // uint32[num_points]: caller index of sorted point j
+    tsl::DevBuf sort_keys, sort_vals, sort_temp;
     tsl::DevBuf blocks;      // double block partials
     tsl::DevBuf state;       // FilterState
     tsl::DevBuf cumsum;      // double[capacity]
@@ -143,14 +203,23 @@ struct FilterState {
     uint32_t counter0;     // last-block-done counters, reset by the last block
     uint32_t counter1;
     uint32_t resample_count;
-    uint32_t pad[3];
+    uint32_t first_bad;    // resampling: first U candidate that failed its check
+    uint32_t pad[2];
     double stats[10];      // sum, Σwx, Σwy, Σwz, Σw sin/cos roll, pitch, yaw
 };
 
 // Kernel launchers (eval.hip)
 int launch_update(tsl_filter* f, hipStream_t stream);
 int launch_raw(tsl_filter* f, hipStream_t stream, float* raw_out_device);
-int launch_trace(tsl_filter* f, hipStream_t stream, uint64_t* out_device);
+int launch_trace(tsl_filter* f, hipStream_t stream, void* out_device, int values);
+int launch_restore(tsl_filter* f, hipStream_t stream);
+int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_k, uint32_t* best_chunk, float* best_ms);
+
+struct Tile {
+    int K;
+    uint32_t chunk_len;
+    uint32_t chunks;
+};
 int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const float tf[16],
                          float* d_out, hipStream_t stream);
 int launch_prep_points(tsl_filter* f, hipStream_t stream);

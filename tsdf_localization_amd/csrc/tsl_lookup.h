@@ -1,70 +1,178 @@
 // Device-side voxel lookup with the reference's exact index semantics.
 //
 // Reference: CudaSubVoxelMap::getIndex / getEntry,
-// include/tsdf_localization/cuda/cuda_sub_voxel_map.tcc:51-158 (the CPU
-// evaluator's lookup).  The returned index is the reference's linear voxel
-// index (block offset + sx + sy*sub_dim + sz*sub_dim^2); data_size means
-// "rejected, read init_value".
+// include/tsdf_localization/cuda/cuda_sub_voxel_map.tcc:51-158 (the lookup the
+// CPU evaluator uses).  The index produced is the reference's linear voxel
+// index (block offset + sx + sy*sub_dim + sz*sub_dim^2, modulo 2^64);
+// data_size means "rejected, read init_value".
+//
+// Per axis the reference computes, with o = t - min (float):
+//   g = (size_t)o;  grid check (size_t)((float)g / res) < dim;
+//   u = (size_t)(o / 1.0f) (== g);  sub = o - (float)u;  s = (size_t)(sub / res)
+// and then up = ux + uy*up_dim_x + uz*up_dim_2, offset = grid_occ[up],
+// idx = offset + sx + sy*sub_dim + sz*sub_dim^2.  The kernels evaluate this
+// in three exact tiers:
+//   fast:    every offset in (-1, 2^23): 32-bit integers; the grid check is
+//            an integer compare with a host-derived threshold; s comes from
+//            the sub-voxel threshold table instead of a division.
+//   wrap:    offsets in (-2^23, -1] on some axis: gcc's size_t conversion
+//            wraps (u = 2^64 + trunc(o)); that axis's grid result and index
+//            contribution are per-map constants; 64-bit index arithmetic.
+//   general: |offset| >= 2^23 or NaN: every conversion spelled out with x86-64
+//            semantics and IEEE division.
 #pragma once
 
 #include "tsl_internal.h"
 
 namespace tsl {
 
-// General path: every float->size_t conversion with x86-64 semantics, size_t
-// arithmetic modulo 2^64, IEEE float division.  Taken for offsets that are
-// negative, >= 2^23 or NaN (points outside the map on the low side, or
-// garbage); bit-exact with the reference for all inputs.
-__device__ __attribute__((noinline)) uint64_t lookup_index_general(const DevMap& m, float ox,
-                                                                     float oy, float oz)
+// General tier: the reference's arithmetic step by step.
+__device__ __forceinline__ uint64_t lookup_index_general(const DevMapFull& m, float ox, float oy, float oz)
 {
     // :53-62 integer-metre bounds check
-    uint64_t gx = cvt_f32_u64(ox), gy = cvt_f32_u64(oy), gz = cvt_f32_u64(oz);
-    uint64_t qx = cvt_f32_u64(cvt_u64_f32(gx) / m.res);
-    uint64_t qy = cvt_f32_u64(cvt_u64_f32(gy) / m.res);
-    uint64_t qz = cvt_f32_u64(cvt_u64_f32(gz) / m.res);
+    const uint64_t gx = cvt_f32_u64(ox), gy = cvt_f32_u64(oy), gz = cvt_f32_u64(oz);
+    const uint64_t qx = cvt_f32_u64(cvt_u64_f32(gx) / m.res);
+    const uint64_t qy = cvt_f32_u64(cvt_u64_f32(gy) / m.res);
+    const uint64_t qz = cvt_f32_u64(cvt_u64_f32(gz) / m.res);
     if (qx >= m.dim_x || qy >= m.dim_y || qz >= m.dim_z) return m.data_size;
     // :67-84: x_offset / 1.0f == x_offset, so the block coordinate equals gx
-    float spx = ox - cvt_u64_f32(gx) * 1.0f;
-    float spy = oy - cvt_u64_f32(gy) * 1.0f;
-    float spz = oz - cvt_u64_f32(gz) * 1.0f;
-    uint64_t up = gx + gy * m.up_dim_x + gz * m.up_dim_2;
+    const float spx = ox - cvt_u64_f32(gx) * 1.0f;
+    const float spy = oy - cvt_u64_f32(gy) * 1.0f;
+    const float spz = oz - cvt_u64_f32(gz) * 1.0f;
+    const uint64_t up = gx + gy * m.up_dim_x + gz * m.up_dim_2;
     if (up >= m.grid_occ_size) return m.data_size;
-    int32_t s = m.grid_occ[up];
+    const int32_t s = m.grid_occ[up];
     if (s < 0) return m.data_size;
-    uint64_t sx = cvt_f32_u64(spx / m.res);
-    uint64_t sy = cvt_f32_u64(spy / m.res);
-    uint64_t sz = cvt_f32_u64(spz / m.res);
+    const uint64_t sx = cvt_f32_u64(spx / m.res);
+    const uint64_t sy = cvt_f32_u64(spy / m.res);
+    const uint64_t sz = cvt_f32_u64(spz / m.res);
     return (uint64_t)(int64_t)s + sx + sy * m.sub_dim + sz * m.sub_dim_2;
 }
 
-// Voxel index of a map-frame point.  Offsets in [0, 2^23) use 32-bit
-// integer conversions (exact there) and an integer form of the grid check
-// (u < gthr, derived on the host from the same float division); the rest
-// goes through lookup_index_general.
-__device__ __forceinline__ uint64_t lookup_index(const DevMap& m, float tx, float ty, float tz)
+// trunc(fl(a / res)) for a in [0, 1): bucket of a's bit pattern in the
+// LDS-resident threshold table, one compare.
+__device__ __forceinline__ uint32_t sub_index(uint32_t sub_base, uint32_t sub_shift, const SubEntry* __restrict__ tab,
+                                              float a)
 {
-    const float ox = tx - m.min_x;
-    const float oy = ty - m.min_y;
-    const float oz = tz - m.min_z;
-    const bool fast = ox >= 0.0f && oy >= 0.0f && oz >= 0.0f && ox < 8388608.0f &&
-                      oy < 8388608.0f && oz < 8388608.0f;
-    if (__builtin_expect(fast, 1)) {
-        const uint32_t ux = (uint32_t)ox, uy = (uint32_t)oy, uz = (uint32_t)oz;
-        if (ux >= m.gthr_x || uy >= m.gthr_y || uz >= m.gthr_z) return m.data_size;
-        const uint64_t up = (uint64_t)ux + (uint64_t)uy * m.up_dim_x + (uint64_t)uz * m.up_dim_2;
-        if (up >= m.grid_occ_size) return m.data_size;
-        const int32_t s = m.grid_occ[up];
-        if (s < 0) return m.data_size;
-        const float spx = ox - (float)ux;
-        const float spy = oy - (float)uy;
-        const float spz = oz - (float)uz;
-        const uint32_t sx = (uint32_t)(spx / m.res);
-        const uint32_t sy = (uint32_t)(spy / m.res);
-        const uint32_t sz = (uint32_t)(spz / m.res);
-        return (uint64_t)(int64_t)s + sx + (uint64_t)sy * m.sub_dim + (uint64_t)sz * m.sub_dim_2;
+    const uint32_t key = __float_as_uint(a);
+    const uint32_t b = (key > sub_base ? key - sub_base : 0u) >> sub_shift;
+    const SubEntry e = tab[b];
+    return e.count + (key >= e.thr ? 1u : 0u);
+}
+
+// 32-bit products of operands known to fit in 24 bits (full-rate
+// v_mul_u32_u24 / v_mul_i32_i24); maps whose block-table strides exceed 2^23
+// use the general 32-bit multiply.
+template <bool kU24>
+__device__ __forceinline__ uint32_t mul_u(uint32_t a, uint32_t b)
+{
+    if (kU24) return __umul24(a, b);
+    return a * b;
+}
+
+template <bool kU24>
+__device__ __forceinline__ int32_t mul_i(int32_t a, int32_t b)
+{
+    if (kU24) return __mul24(a, b);
+    return a * b;
+}
+
+// Fast tier: 0 <= o < R on every axis (one unsigned compare of the bit
+// pattern per axis, since floats >= +0 order like their bits).
+__device__ __forceinline__ bool fast_ok(float ox, float oy, float oz, uint32_t range_bits)
+{
+    return (__float_as_uint(ox) < range_bits) & (__float_as_uint(oy) < range_bits) &
+           (__float_as_uint(oz) < range_bits);
+}
+
+// Wrap tier: |o| < R on every axis.
+__device__ __forceinline__ bool wrap_ok(float ox, float oy, float oz, uint32_t range_bits)
+{
+    return ((__float_as_uint(ox) & 0x7fffffffu) < range_bits) & ((__float_as_uint(oy) & 0x7fffffffu) < range_bits) &
+           ((__float_as_uint(oz) & 0x7fffffffu) < range_bits);
+}
+
+struct FastAxis {
+    uint32_t i;  // trunc(o)
+    uint32_t k;  // in-block coordinate
+};
+
+__device__ __forceinline__ FastAxis fast_axis(const DevMap& m, const SubEntry* __restrict__ tab, float o)
+{
+    FastAxis r;
+    r.i = (uint32_t)(int32_t)o;
+    const float f = o - (float)(int32_t)r.i;  // exact, in [0, 1)
+    r.k = sub_index(m.sub_base, m.sub_shift, tab, f);
+    return r;
+}
+
+// Offset of local voxel (lx, ly, lz), each < sub_dim, inside a bricked block:
+// ((lx>>2) + (ly>>2)*B + (lz>>2)*B^2)*64 + (lx&3) + (ly&3)*4 + (lz&3)*16.
+__device__ __forceinline__ uint32_t brick_offset(const DevMap& m, uint32_t lx, uint32_t ly, uint32_t lz)
+{
+    return lx + 60u * (lx >> 2) + 4u * ly + m.brick_cy * (ly >> 2) + 16u * lz + m.brick_cz * (lz >> 2);
+}
+
+// Physical position of reference index x (x < 2^31) in the bricked array.
+__device__ __forceinline__ uint32_t phys_from_linear(const DevMap& m, uint32_t x)
+{
+    const uint32_t slot = __umulhi(x, m.mag_sd3) >> m.sh_sd3;
+    const uint32_t rem = x - slot * m.sd3;
+    const uint32_t lz = __umulhi(rem, m.mag_sd2) >> m.sh_sd2;
+    const uint32_t r2 = rem - lz * (uint32_t)m.sub_dim_2;
+    const uint32_t ly = __umulhi(r2, m.mag_sd) >> m.sh_sd;
+    const uint32_t lx = r2 - ly * (uint32_t)m.sub_dim;
+    return slot * m.sd3 + brick_offset(m, lx, ly, lz);
+}
+
+// Fast tier for one lane: block offset s (or -1), in-block part of the
+// reference index, and (bricked layout) the in-block physical offset, with
+// carry set when an in-block coordinate equals sub_dim (the division rounded
+// up to the block edge; the reference index then lands in the next row).
+template <bool kU24, bool kBrick>
+__device__ __forceinline__ void fast_block(const DevMap& m, const SubEntry* __restrict__ tab, float ox, float oy,
+                                           float oz, int32_t& s, uint32_t& small, uint32_t& pbrick, bool& carry)
+{
+    const FastAxis ax = fast_axis(m, tab, ox), ay = fast_axis(m, tab, oy), az = fast_axis(m, tab, oz);
+    const bool grid = (ax.i < m.gthr_x) & (ay.i < m.gthr_y) & (az.i < m.gthr_z);
+    const uint32_t up = ax.i + mul_u<kU24>(ay.i, m.up_dim_x) + mul_u<kU24>(az.i, m.up_dim_2);
+    s = -1;
+    if (grid & (up < m.occ)) s = m.grid_occ[up];
+    small = ax.k + mul_u<kU24>(ay.k, (uint32_t)m.sub_dim) + mul_u<kU24>(az.k, (uint32_t)m.sub_dim_2);
+    if (kBrick) {
+        pbrick = brick_offset(m, ax.k, ay.k, az.k);
+        carry = max(max(ax.k, ay.k), az.k) >= (uint32_t)m.sub_dim;
+    } else {
+        pbrick = small;
+        carry = false;
     }
-    return lookup_index_general(m, ox, oy, oz);
+}
+
+// Wrap tier for one lane (|o| < R on every axis): the reference's index,
+// data_size when rejected.  neg is the LDS copy of the wrap-band table.
+template <bool kU24>
+__device__ __forceinline__ uint64_t wrap_index(const DevMap& m, const SubEntry* __restrict__ tab,
+                                               const uint64_t* __restrict__ neg, float ox, float oy, float oz)
+{
+    const int32_t ix = (int32_t)ox, iy = (int32_t)oy, iz = (int32_t)oz;
+    const bool nx = ox <= -1.0f, ny = oy <= -1.0f, nz = oz <= -1.0f;
+    const bool grid = (((uint32_t)ix < m.gthr_x) | (nx & ((m.gneg & 1u) != 0u))) &
+                      (((uint32_t)iy < m.gthr_y) | (ny & ((m.gneg & 2u) != 0u))) &
+                      (((uint32_t)iz < m.gthr_z) | (nz & ((m.gneg & 4u) != 0u)));
+    const int32_t up = ix + mul_i<kU24>(iy, (int32_t)m.up_dim_x) + mul_i<kU24>(iz, (int32_t)m.up_dim_2);
+    int32_t s = -1;
+    if (grid & ((uint32_t)up < m.occ)) s = m.grid_occ[up];
+    const float fx = ox - (float)ix, fy = oy - (float)iy, fz = oz - (float)iz;
+    int32_t ex = (int32_t)sub_index(m.sub_base, m.sub_shift, tab, fabsf(fx));
+    int32_t ey = (int32_t)sub_index(m.sub_base, m.sub_shift, tab, fabsf(fy));
+    int32_t ez = (int32_t)sub_index(m.sub_base, m.sub_shift, tab, fabsf(fz));
+    ex = nx ? 0 : (fx < 0.0f ? -ex : ex);
+    ey = ny ? 0 : (fy < 0.0f ? -ey : ey);
+    ez = nz ? 0 : (fz < 0.0f ? -ez : ez);
+    const int32_t small = ex + mul_i<kU24>(ey, m.sub_dim) + mul_i<kU24>(ez, m.sub_dim_2);
+    const uint32_t mask = (nx ? 1u : 0u) | (ny ? 2u : 0u) | (nz ? 4u : 0u);
+    const uint64_t idx = (uint64_t)((int64_t)s + (int64_t)small) + neg[mask];
+    return s >= 0 ? idx : m.data_size;
 }
 
 }  // namespace tsl

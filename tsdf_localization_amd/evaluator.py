@@ -192,6 +192,15 @@ class TSDFEvaluator:
         check(lib.tsl_trace_indices(self.handle, ptr(p), p.shape[0], ptr(pts), pts.shape[0], ptr(tf), ptr(out)))
         return out
 
+    def trace_values(self, particles, points, tf_matrix):
+        """getEntry result of every (particle, point) pair, shape (P, N), float32."""
+        p = _particles(particles)
+        pts = _points(points)
+        tf = _tf(tf_matrix)
+        out = np.empty((p.shape[0], pts.shape[0]), np.float32)
+        check(lib.tsl_trace_values(self.handle, ptr(p), p.shape[0], ptr(pts), pts.shape[0], ptr(tf), ptr(out)))
+        return out
+
     def __del__(self):
         if getattr(self, "handle", None):
             lib.tsl_evaluator_destroy(self.handle)
@@ -326,6 +335,36 @@ class DeviceFilter:
 
     def resample_async(self, u0, stream=None):
         check(lib.tsl_filter_resample_async(self.handle, float(u0), None if stream is None else stream.handle))
+
+    def snapshot(self):
+        check(lib.tsl_filter_snapshot(self.handle))
+
+    def restore_async(self, stream=None):
+        check(lib.tsl_filter_restore_async(self.handle, None if stream is None else stream.handle))
+
+    def set_tile(self, k=0, chunk_len=0):
+        check(lib.tsl_filter_set_tile(self.handle, int(k), int(chunk_len)))
+
+    def autotune(self, stream=None, reps=3):
+        """Pick the fastest tile shape on the current particles and scan;
+        returns (K, chunk_len (0 = automatic), ms per evaluation)."""
+        k = ctypes.c_int()
+        c = ctypes.c_uint32()
+        ms = ctypes.c_float()
+        check(lib.tsl_filter_autotune(self.handle, None if stream is None else stream.handle, int(reps),
+                                      ctypes.byref(k), ctypes.byref(c), ctypes.byref(ms)))
+        return k.value, c.value, ms.value
+
+    def set_timing(self, enable=True):
+        check(lib.tsl_filter_set_timing(self.handle, int(bool(enable))))
+
+    def eval_timing(self):
+        """(kernel ms, launches, particle*point evaluations) since set_timing."""
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        pairs = ctypes.c_uint64()
+        check(lib.tsl_filter_eval_timing(self.handle, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(pairs)))
+        return ms.value, n.value, pairs.value
 
     def sync(self, stream=None):
         pose = Pose()
