@@ -71,6 +71,36 @@ def _cpu_model():
     return "unknown"
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_eval_global_localisation_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per evaluation-kernel launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by the plain-copy calibration),
+    or None if that summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(mib, reps):
+    """Plain device-to-device copies of known size, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters: each copy reads and writes mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,12 +113,18 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--tile-k", type=int, default=None, help="particles per wavefront (1, 2, 4); default autotune")
     ap.add_argument("--chunk", type=int, default=0, help="scan points per wavefront (0 = automatic)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run plain copies of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "filters are independent, so N GPUs would each run their own share.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib, max(args.steps, 1))
+        return
 
     import tsdf_localization_amd as tsl
     from tsdf_localization_amd import synth
@@ -176,7 +212,7 @@ def main():
                    "step": "restore + pose + evaluate + normalise + resample",
                    "tile": {"particles_per_wavefront": k, "chunk_points": chunk or "auto"}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_eval", "kernel_avg_ms": kern_avg_ms,
                      "bytes_per_eval": BYTES_PER_EVAL, "evals_per_launch": pairs // max(launches, 1)},
         "cpu_baseline": cpu,

@@ -81,11 +81,28 @@ def build_synth(verbose=False):
     return LIBSYNTH
 
 
+DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_check.cpp")
+DROPIN_BIN = os.path.join(HERE, "_build", "dropin_check")
+
+
+def build_dropin_check(verbose=False):
+    """The C++ drop-in test program (include/tsdf_localization/hip_evaluator.h
+    over libtsl.so), a plain host build: g++ links the HIP library like any
+    C++ application of the reference would."""
+    os.makedirs(os.path.dirname(DROPIN_BIN), exist_ok=True)
+    tmp = DROPIN_BIN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), "-o", tmp,
+          DROPIN_SRC, "-L", HERE, "-ltsl", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."], verbose)
+    os.replace(tmp, DROPIN_BIN)
+    return DROPIN_BIN
+
+
 def build_all(verbose=False):
     build_synth(verbose)
     build_tsl(verbose)
+    build_dropin_check(verbose)
 
 
 if __name__ == "__main__":
     build_all(verbose="-v" in sys.argv)
-    print("built", LIBTSL, LIBSYNTH)
+    print("built", LIBTSL, LIBSYNTH, DROPIN_BIN)
