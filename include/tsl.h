@@ -250,21 +250,114 @@ int tsl_filter_snapshot(tsl_filter* f);
 int tsl_filter_restore_async(tsl_filter* f, void* stream);
 
 /* Tile shape of the evaluation: k particles per wavefront (1 or 2; 0 =
- * automatic) and chunk_len scan points per wavefront (0 = automatic).
- * Results do not depend on the tile beyond float summation order. */
-int tsl_filter_set_tile(tsl_filter* f, int k, uint32_t chunk_len);
+ * automatic), points_per_lane scan points each lane handles per iteration
+ * (1, 2 or 4; 0 = automatic; independent gathers in flight) and chunk_len
+ * scan points per wavefront (0 = automatic).  Results depend on the tile
+ * only through chunk_len (float summation order); k and points_per_lane
+ * leave them unchanged.  No counterpart in the reference. */
+int tsl_filter_set_tile(tsl_filter* f, int k, int points_per_lane, uint32_t chunk_len);
 
-/* Time the candidate tile shapes on the filter's current particles and scan
- * (reps launches each, on `stream`, synchronously) and keep the fastest;
- * reports it through the non-NULL out arguments. */
-int tsl_filter_autotune(tsl_filter* f, void* stream, int reps, int* best_k, uint32_t* best_chunk,
-                        float* best_ms);
+/* Order in which the particles are evaluated: 0 = caller order, 1 =
+ * spatial order (sorted by position and heading, so particles evaluated
+ * together share cached voxels), -1 = automatic (spatial from 8192
+ * particles).  Results are identical in every order.  No counterpart in the
+ * reference. */
+int tsl_filter_set_order(tsl_filter* f, int mode);
+
+/* Time the candidate evaluation shapes (particle order x tile) on the
+ * filter's current particles and scan (reps launches each, on `stream`,
+ * synchronously) and keep the fastest; reports it through the non-NULL out
+ * arguments.  No counterpart in the reference (its launch shape is fixed,
+ * src/cuda/cuda_evaluator.cu:301). */
+int tsl_filter_autotune(tsl_filter* f, void* stream, int reps, int* best_order, int* best_k,
+                        int* best_points_per_lane, uint32_t* best_chunk, float* best_ms);
 
 /* Bracket every evaluation-kernel launch of this filter with HIP events on
  * the launch stream (enable resets the totals); read the accumulated kernel
  * time, number of launches and particle*point evaluations launched. */
 int tsl_filter_set_timing(tsl_filter* f, int enable);
 int tsl_filter_eval_timing(tsl_filter* f, double* ms, uint64_t* launches, uint64_t* pairs);
+
+/* ---- motion update ------------------------------------------------------ */
+
+/* The six normal distributions of a motion step (dx, dy, dz, roll, pitch,
+ * yaw): means and standard deviations as the reference builds them, and the
+ * seed of the noise stream.  Particle i's draw k is a pure function of
+ * (seed, i, k) (SplitMix64 + Box-Muller), so a filter's particles move the
+ * same way alone or in a batch; the reference draws from a sequential
+ * std::mt19937 instead (resampler.h:23), which no device can reproduce. */
+typedef struct tsl_motion {
+    double mean[6];
+    double stddev[6];
+    uint64_t seed;
+} tsl_motion;
+
+/* Distributions of ParticleCloud::motion_update(dx, dy, dz, roll, pitch,
+ * yaw, lin_base_noise, ang_base_noise) (src/particle_cloud.cpp:456-497) with
+ * the motion parameters a_1..a_12 (particle_cloud.h:60-71). */
+int tsl_motion_from_delta(float dx, float dy, float dz, float roll, float pitch, float yaw, float lin_base_noise,
+                          float ang_base_noise, const float a[12], uint64_t seed, tsl_motion* out);
+
+/* Distributions of ParticleCloud::motionUpdate(const nav_msgs::Odometry&)
+ * (src/particle_cloud.cpp:153-207): forward velocity and yaw rate over
+ * time_diff seconds. */
+int tsl_motion_from_odom(float linear_velocity, float angular_velocity, float time_diff, const float a[12],
+                         uint64_t seed, tsl_motion* out);
+
+/* ParticleCloud::apply_model (src/particle_cloud.cpp:499-617) on the
+ * filter's device-resident particles. */
+int tsl_filter_motion_async(tsl_filter* f, const tsl_motion* motion, void* stream);
+
+/* ---- batched update: many filters, one launch sequence ----------------- */
+
+/* A batch updates and resamples n device-resident filters that share one
+ * evaluator (map and sensor model) with one fixed sequence of launches: a
+ * persistent evaluation kernel walks every member's work items, and every
+ * per-particle kernel covers all members at once.  Each member gives the
+ * same bits as when it is updated alone with the same tile (chunk length).
+ * Members keep their own particles (any count) and scans; after changing a
+ * member's particles, scan or snapshot outside the batch, call
+ * tsl_batch_prepare.  No counterpart in the reference (one filter per
+ * process, src/mcl_3d.cpp:378-425). */
+typedef struct tsl_batch tsl_batch;
+
+int tsl_batch_create(tsl_evaluator* ev, tsl_filter* const* filters, size_t n, tsl_batch** out);
+int tsl_batch_destroy(tsl_batch* b);
+
+/* Tile of the batched evaluation (k particles per wavefront, 1 or 2; points
+ * per lane 1, 2 or 4; chunk_len points per chunk, 0 = each member's
+ * automatic choice).  Re-prepares the batch. */
+int tsl_batch_set_tile(tsl_batch* b, int k, int points_per_lane, uint32_t chunk_len);
+
+/* Rebuild the members' device descriptors and reserve their buffers. */
+int tsl_batch_prepare(tsl_batch* b);
+
+/* Restore every member from its snapshot (tsl_filter_snapshot). */
+int tsl_batch_restore_async(tsl_batch* b, void* stream);
+
+/* Sensor update (evaluate + normalise + pose sums) of every member. */
+int tsl_batch_update_async(tsl_batch* b, void* stream);
+
+/* Systematic resampling of every member, member i with draw u0[i]; the
+ * plan is built on the device from each member's device-side count. */
+int tsl_batch_resample_async(tsl_batch* b, const float* u0, void* stream);
+
+/* Motion update of every member, member i with motions[i]. */
+int tsl_batch_motion_async(tsl_batch* b, const tsl_motion* motions, void* stream);
+
+/* [restore +] update + resample of every member. */
+int tsl_batch_step_async(tsl_batch* b, const float* u0, int restore, void* stream);
+
+/* Capture one step ([restore +] update + resample, draws read from device
+ * memory) into a HIP graph on `stream` (a created stream, not NULL);
+ * tsl_batch_launch replays it with new draws u0[n]. */
+int tsl_batch_capture(tsl_batch* b, int restore, void* stream);
+int tsl_batch_launch(tsl_batch* b, const float* u0, void* stream);
+
+/* Wait for `stream`, then report every member's pose, particle count and
+ * status (0 ok, 1 no valid particle, 2 resampling capacity, 3 empty, 4 draw
+ * does not advance); returns the first member error, if any. */
+int tsl_batch_sync(tsl_batch* b, void* stream, tsl_pose* poses, uint64_t* counts, int* status);
 
 /* ---- streams and timing (plumbing for benchmarks) ---------------------- */
 int tsl_stream_create(void** stream);

@@ -558,3 +558,113 @@ ORC_EXPORT void orc_cumsum(const float* weights, size_t M, double* out)
         out[m] = s;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Motion update: ParticleCloud::apply_model (src/particle_cloud.cpp:499-617)
+ * with getAngleFromMat (src/util/util.cpp:86-111).  The six draws per particle
+ * come from the library's counter-based stream (not from the reference's
+ * std::mt19937, which no parallel implementation reproduces): SplitMix64
+ * (Steele, Lea, Flood, OOPSLA 2014) mixes (seed, particle, draw, half), 53 bits
+ * give a uniform in (0, 1), Box-Muller's cosine branch a standard normal, and
+ * draw = z * stddev + mean as libstdc++'s normal_distribution forms it.
+ * Parity: the transcendental functions are glibc's here and the device
+ * math library's there, so results agree within a few float ulps; pinned by
+ * closed forms (pure translation, pure yaw, rotation composition) in the
+ * tests. */
+/* ------------------------------------------------------------------------ */
+static uint64_t orc_splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+ORC_EXPORT double orc_motion_normal(uint64_t seed, uint64_t particle, uint32_t draw)
+{
+    uint64_t base = orc_splitmix64(seed ^ orc_splitmix64(particle));
+    uint64_t h0 = orc_splitmix64(base + (uint64_t)(2u * draw));
+    uint64_t h1 = orc_splitmix64(base + (uint64_t)(2u * draw + 1u));
+    double u = ((double)(h0 >> 11) + 0.5) * 0x1.0p-53;
+    double v = ((double)(h1 >> 11) + 0.5) * 0x1.0p-53;
+    return sqrt(-2.0 * log(u)) * cos(6.283185307179586 * v);
+}
+
+ORC_EXPORT void orc_motion(const orc_particle* in, size_t n, const double mean[6], const double stddev[6],
+                           uint64_t seed, orc_particle* out)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const float* last = in[i].first;
+        double d[6];
+        for (int k = 0; k < 6; ++k)
+            d[k] = orc_motion_normal(seed, i, (uint32_t)k) * stddev[k] + mean[k];
+        /* :519-545: FLOAT_T sin_roll = sin(roll) with roll a double */
+        float sin_roll = (float)sin(d[3]), cos_roll = (float)cos(d[3]);
+        float sin_pitch = (float)sin(d[4]), cos_pitch = (float)cos(d[4]);
+        float sin_yaw = (float)sin(d[5]), cos_yaw = (float)cos(d[5]);
+        float tf_odom[16];
+        tf_odom[0] = cos_pitch * cos_yaw;
+        tf_odom[4] = cos_pitch * sin_yaw;
+        tf_odom[8] = -sin_pitch;
+        tf_odom[3] = (float)d[0];
+        tf_odom[1] = sin_roll * sin_pitch * cos_yaw - cos_roll * sin_yaw;
+        tf_odom[5] = sin_roll * sin_pitch * sin_yaw + cos_roll * cos_yaw;
+        tf_odom[9] = sin_roll * cos_pitch;
+        tf_odom[7] = (float)d[1];
+        tf_odom[2] = cos_roll * sin_pitch * cos_yaw + sin_roll * sin_yaw;
+        tf_odom[6] = cos_roll * sin_pitch * sin_yaw - sin_roll * cos_yaw;
+        tf_odom[10] = cos_roll * cos_pitch;
+        tf_odom[11] = (float)d[2];
+        /* :549-581: particle rotation, no translation */
+        float sin_alpha, cos_alpha, sin_beta, cos_beta, sin_gamma, cos_gamma;
+        orc_sincosf(last[3], &sin_alpha, &cos_alpha);
+        orc_sincosf(last[4], &sin_beta, &cos_beta);
+        orc_sincosf(last[5], &sin_gamma, &cos_gamma);
+        float tp[16];
+        tp[0] = cos_beta * cos_gamma;
+        tp[4] = cos_beta * sin_gamma;
+        tp[8] = -sin_beta;
+        tp[3] = 0;
+        tp[1] = sin_alpha * sin_beta * cos_gamma - cos_alpha * sin_gamma;
+        tp[5] = sin_alpha * sin_beta * sin_gamma + cos_alpha * cos_gamma;
+        tp[9] = sin_alpha * cos_beta;
+        tp[7] = 0;
+        tp[2] = cos_alpha * sin_beta * cos_gamma + sin_alpha * sin_gamma;
+        tp[6] = cos_alpha * sin_beta * sin_gamma - sin_alpha * cos_gamma;
+        tp[10] = cos_alpha * cos_beta;
+        tp[11] = 0;
+        /* :585-600 */
+        float tf[16];
+        for (int r = 0; r < 3; ++r) {
+            tf[4 * r + 0] = tp[4 * r] * tf_odom[0] + tp[4 * r + 1] * tf_odom[4] + tp[4 * r + 2] * tf_odom[8];
+            tf[4 * r + 1] = tp[4 * r] * tf_odom[1] + tp[4 * r + 1] * tf_odom[5] + tp[4 * r + 2] * tf_odom[9];
+            tf[4 * r + 2] = tp[4 * r] * tf_odom[2] + tp[4 * r + 1] * tf_odom[6] + tp[4 * r + 2] * tf_odom[10];
+            tf[4 * r + 3] = tp[4 * r] * tf_odom[3] + tp[4 * r + 1] * tf_odom[7] + tp[4 * r + 2] * tf_odom[11] +
+                            tp[4 * r + 3];
+        }
+        /* getAngleFromMat (util.cpp:86-111) */
+        float roll, pitch, yaw;
+        if (fabs(tf[8]) >= 1) {
+            yaw = 0;
+            double delta = atan2(tf[9], tf[10]);
+            if (tf[8] < 0) {
+                pitch = M_PI / 2.0;
+                roll = delta;
+            } else {
+                pitch = -M_PI / 2.0;
+                roll = delta;
+            }
+        } else {
+            pitch = -asin(tf[8]);
+            roll = atan2(tf[9] / cos(pitch), tf[10] / cos(pitch));
+            yaw = atan2(tf[4] / cos(pitch), tf[0] / cos(pitch));
+        }
+        out[i].first[0] = last[0] + tf[3];
+        out[i].first[1] = last[1] + tf[7];
+        out[i].first[2] = last[2] + tf[11];
+        out[i].first[3] = roll;
+        out[i].first[4] = pitch;
+        out[i].first[5] = yaw;
+        out[i].second = in[i].second;
+    }
+}

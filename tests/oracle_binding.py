@@ -58,6 +58,8 @@ _sigs = {
     "orc_resample_systematic": (ctypes.c_long, [_P, _S, _F, _P, _S]),
     "orc_systematic_u": (None, [_S, _F, _P, _S]),
     "orc_cumsum": (None, [_P, _S, _P]),
+    "orc_motion_normal": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]),
+    "orc_motion": (None, [_P, _S, _P, _P, ctypes.c_uint64, _P]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(olib, _n)
@@ -209,3 +211,14 @@ def cumsum(weights):
     out = np.empty(w.shape[0], np.float64)
     olib.orc_cumsum(_ptr(w), w.shape[0], _ptr(out))
     return out
+
+
+def motion(particles, mean, stddev, seed):
+    """ParticleCloud::apply_model with the counter-based draws."""
+    p = np.ascontiguousarray(particles, np.float32)
+    out = np.empty_like(p)
+    m = np.ascontiguousarray(mean, np.float64)
+    sd = np.ascontiguousarray(stddev, np.float64)
+    olib.orc_motion(_ptr(p), p.shape[0], _ptr(m), _ptr(sd), int(seed), _ptr(out))
+    return out
+

@@ -1,0 +1,172 @@
+"""Batched update: many device-resident filters on one map in one launch
+sequence.  Each member must give exactly the bits it gives alone (same
+chunk length), eager and replayed from a captured HIP graph, and match the
+oracle within the weight tolerance."""
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+
+pytestmark = pytest.mark.gpu
+
+WEIGHT_RTOL = 1e-5
+CHUNK = 320
+
+
+@pytest.fixture(scope="module")
+def tsl():
+    import tsdf_localization_amd as t
+    assert t.device_count() >= 1, "no HIP device"
+    return t
+
+
+@pytest.fixture(scope="module")
+def room():
+    from tsdf_localization_amd import synth
+    return synth.room()
+
+
+def _members(room, sizes, seed=5):
+    """(particles, points) per member: uneven particle counts, different
+    scan subsets."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, (P, N) in enumerate(sizes):
+        pidx = rng.choice(room.particles.shape[0], P, replace=P > room.particles.shape[0])
+        parts = room.particles[pidx].copy()
+        parts[:, 6] = 1.0 / P
+        pts = room.points[rng.permutation(room.points.shape[0])[:N]]
+        out.append((parts, pts))
+    return out
+
+
+def _alone(tsl, ev, parts, pts, tf, u0, k, pp):
+    f = tsl.DeviceFilter(ev, parts.shape[0])
+    f.set_particles(parts)
+    f.set_scan(pts, tf)
+    f.set_tile(k, CHUNK, pp)
+    f.set_order(0)
+    s = tsl.Stream()
+    f.update_async(s)
+    pose, _ = f.sync(s)
+    weighted = f.get_particles()
+    f.resample_async(u0, s)
+    f.sync(s)
+    return weighted, f.get_particles(), pose
+
+
+SIZES = [(37, 300), (1000, 1500), (257, 64), (2500, 900), (64, 1)]
+
+
+@pytest.mark.parametrize("k,pp", [(1, 1), (2, 1), (1, 4)])
+def test_batch_member_equals_alone(tsl, room, k, pp):
+    ev = tsl.TSDFEvaluator(room.map)
+    mem = _members(room, SIZES)
+    u0 = np.array([np.float32(0.37 / p.shape[0]) for p, _ in mem], np.float32)
+    filters = []
+    for parts, pts in mem:
+        f = tsl.DeviceFilter(ev, parts.shape[0])
+        f.set_particles(parts)
+        f.set_scan(pts, room.tf)
+        filters.append(f)
+    b = tsl.Batch(ev, filters)
+    b.set_tile(k, CHUNK, pp)
+    s = tsl.Stream()
+    b.update_async(s)
+    poses_u, counts_u, status_u = b.sync(s)
+    weighted = [f.get_particles() for f in filters]
+    b.resample_async(u0, s)
+    poses, counts, status = b.sync(s)
+    assert np.all(status == 0)
+    om = ob.OracleMap.from_host_map(room.map)
+    for i, (parts, pts) in enumerate(mem):
+        w_alone, r_alone, pose_alone = _alone(tsl, ev, parts, pts, room.tf, u0[i], k, pp)
+        assert np.array_equal(weighted[i].view(np.uint32), w_alone.view(np.uint32)), f"member {i} weights"
+        got = filters[i].get_particles()
+        assert np.array_equal(got.view(np.uint32), r_alone.view(np.uint32)), f"member {i} resampled"
+        assert counts[i] == got.shape[0]
+        assert poses_u[i]["position"] == pose_alone["position"]
+        _, w64 = om.evaluate(parts, pts, room.tf)
+        ref = w64 / w64.sum()
+        assert np.max(np.abs(weighted[i][:, 6].astype(np.float64) - ref) / ref) <= WEIGHT_RTOL
+        idx = ob.resample(weighted[i][:, 6], u0[i])
+        assert np.array_equal(got, weighted[i][idx])
+
+
+def test_batch_graph_replays_match_eager(tsl, room):
+    ev = tsl.TSDFEvaluator(room.map)
+    mem = _members(room, SIZES, seed=9)
+    filters = []
+    for parts, pts in mem:
+        f = tsl.DeviceFilter(ev, parts.shape[0])
+        f.set_particles(parts)
+        f.set_scan(pts, room.tf)
+        f.snapshot()
+        filters.append(f)
+    b = tsl.Batch(ev, filters)
+    b.set_tile(1, CHUNK, 1)
+    s = tsl.Stream()
+    rng = np.random.default_rng(3)
+    draws = [np.array([np.float32(rng.uniform(0, 1.0 / p.shape[0])) for p, _ in mem], np.float32) for _ in range(3)]
+    eager = []
+    for u0 in draws:
+        b.step_async(u0, restore=True, stream=s)
+        b.sync(s)
+        eager.append([f.get_particles() for f in filters])
+    b.capture(s, restore=True)
+    for rep in range(2):
+        for u0, ref in zip(draws, eager):
+            b.launch(u0, s)
+            _, counts, status = b.sync(s)
+            assert np.all(status == 0)
+            for i, f in enumerate(filters):
+                got = f.get_particles()
+                assert np.array_equal(got.view(np.uint32), ref[i].view(np.uint32)), (rep, i)
+            # eager work between replays
+            b.update_async(s)
+            b.sync(s)
+
+
+def test_filter_motion_matches_oracle(tsl, room):
+    """Device apply_model vs the oracle with the same counter-based draws.
+    Tolerance: the transcendental functions come from different math
+    libraries, so positions agree within 4e-6 m and angles within 2e-6 rad."""
+    ev = tsl.TSDFEvaluator(room.map)
+    parts = room.particles.copy()
+    f = tsl.DeviceFilter(ev, parts.shape[0])
+    f.set_particles(parts)
+    m = tsl.evaluator.motion_from_delta(0.4, 0.05, 0.01, 0.01, -0.02, 0.15, 0.02, 0.01, seed=1234)
+    s = tsl.Stream()
+    f.motion_async(m, s)
+    s.synchronize()
+    got = f.get_particles()
+    ref = ob.motion(parts, list(m.mean), list(m.stddev), m.seed)
+    assert np.max(np.abs(got[:, :3] - ref[:, :3])) <= 4e-6
+    ang = np.angle(np.exp(1j * (got[:, 3:6].astype(np.float64) - ref[:, 3:6])))
+    assert np.max(np.abs(ang)) <= 2e-6
+    assert np.array_equal(got[:, 6], ref[:, 6])
+    frac_exact = np.mean(np.all(got == ref, axis=1))
+    print(f"bit-identical particles: {frac_exact:.4f}")
+
+
+def test_batch_motion_member_equals_alone(tsl, room):
+    ev = tsl.TSDFEvaluator(room.map)
+    mem = _members(room, SIZES, seed=21)
+    motions = [tsl.evaluator.motion_from_delta(0.1 * i, 0.02, 0.0, 0.0, 0.0, 0.05 * i, 0.01, 0.005, seed=100 + i)
+               for i in range(len(mem))]
+    filters = []
+    for parts, pts in mem:
+        f = tsl.DeviceFilter(ev, parts.shape[0])
+        f.set_particles(parts)
+        f.set_scan(pts, room.tf)
+        filters.append(f)
+    b = tsl.Batch(ev, filters)
+    s = tsl.Stream()
+    b.motion_async(motions, s)
+    b.sync(s)
+    for i, (parts, pts) in enumerate(mem):
+        alone = tsl.DeviceFilter(ev, parts.shape[0])
+        alone.set_particles(parts)
+        alone.motion_async(motions[i], s)
+        s.synchronize()
+        assert np.array_equal(filters[i].get_particles().view(np.uint32), alone.get_particles().view(np.uint32)), i

@@ -133,6 +133,66 @@ def test_ragged_sizes(tsl, room, room_oracle, P, N):
     assert np.max(rel_err(raw, w64)) <= WEIGHT_RTOL
 
 
+def _filter_weights(tsl, ev, parts, pts, tf, k, pp, chunk, order=0):
+    f = tsl.DeviceFilter(ev, parts.shape[0])
+    f.set_particles(parts)
+    f.set_scan(pts, tf)
+    f.set_tile(k, chunk, pp)
+    f.set_order(order)
+    s = tsl.Stream()
+    f.update_async(s)
+    f.sync(s)
+    return f.get_particles()[:, 6]
+
+
+def _edge_scene(tsl):
+    from tsdf_localization_amd import synth
+    sc = synth.Scene()
+    sc.add_box((-5, -5, -5), (20, 20, 0.3))
+    sc.add_box((-5, -5, -5), (0.25, 20, 20))
+    sc.add_cylinder(2.0, 2.0, 0.0, 2.5, 0.4)
+    m = sc.tsdf_map((0, 0, 0), (6.4, 6.4, 3.2), 0.05, 300, dense=False)
+    rng = np.random.default_rng(17)
+    P = 200
+    parts = np.zeros((P, 7), np.float32)
+    parts[:, 0] = rng.uniform(-3, 4, P)
+    parts[:, 1] = rng.uniform(-3, 4, P)
+    parts[:, 2] = rng.uniform(-2, 2, P)
+    parts[:, 5] = rng.uniform(-math.pi, math.pi, P)
+    parts[:, 6] = 1.0 / P
+    parts[:8, 0] += 2e7  # a few particles in the general tier
+    pts = rng.uniform(-8, 8, (1300, 3)).astype(np.float32)
+    tf = synth.tf_from_xyzrpy(0.1, 0.0, 0.4, 0.0, 0.02, 0.03)
+    return m, parts, pts, tf
+
+
+@pytest.mark.parametrize("scene", ["room", "edges"])
+def test_tile_shapes_bit_identical(tsl, room, room_oracle, scene):
+    """Every (particle order, particles per wavefront, points per lane)
+    shape gives the same weights bit for bit for a given chunk length, on the
+    fast tier (room) and with the wrap/general tiers mixed in (edges), within
+    1e-5 of the oracle."""
+    if scene == "room":
+        m, parts, pts, tf, om = room.map, room.particles, room.points, room.tf, room_oracle
+    else:
+        m, parts, pts, tf = _edge_scene(tsl)
+        om = ob.OracleMap.from_host_map(m)
+    ev = tsl.TSDFEvaluator(m)
+    _, w64 = om.evaluate(parts, pts, tf)
+    ref_norm = w64 / w64.sum()
+    for chunk in (320, 0):
+        first = None
+        for order in (0, 1):
+            for k in (1, 2):
+                for pp in (1, 2, 4):
+                    w = _filter_weights(tsl, ev, parts, pts, tf, k, pp, chunk, order)
+                    assert np.max(rel_err(w, ref_norm)) <= WEIGHT_RTOL, (order, k, pp, chunk)
+                    if first is None:
+                        first = w
+                    else:
+                        assert np.array_equal(w.view(np.uint32), first.view(np.uint32)), (order, k, pp, chunk)
+
+
 def test_per_point_tile_same_result(tsl, room, room_oracle):
     ev = tsl.TSDFEvaluator(room.map, per_point=True)
     raw = ev.evaluate_raw(room.particles, room.points, room.tf)

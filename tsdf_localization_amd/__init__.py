@@ -6,11 +6,12 @@ runs in hand-written HIP kernels in libtsl.so behind the C ABI of
 include/tsl.h.  This package is the Python mirror of the reference's
 evaluator/resampler interface over that ABI.
 """
-from .evaluator import (CudaSubVoxelMap, DeviceFilter, Event, ParticleCloud, Stream, SystematicResampler,
+from .evaluator import (Batch, CudaSubVoxelMap, DeviceFilter, Event, ParticleCloud, Stream, SystematicResampler,
                         TSDFEvaluator, coef_init, likelihood_init, likelihood_value, pose_matrices,
                         resample_systematic)
 from ._lib import TslError, device_count
+from . import evaluator  # noqa: F401  (motion helpers: evaluator.motion_from_delta, ...)
 
-__all__ = ["CudaSubVoxelMap", "DeviceFilter", "Event", "ParticleCloud", "Stream", "SystematicResampler",
+__all__ = ["Batch", "CudaSubVoxelMap", "DeviceFilter", "Event", "ParticleCloud", "Stream", "SystematicResampler",
            "TSDFEvaluator", "TslError", "coef_init", "device_count", "likelihood_init", "likelihood_value",
            "pose_matrices", "resample_systematic"]

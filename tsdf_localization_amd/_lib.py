@@ -43,6 +43,11 @@ class Pose(ctypes.Structure):
                 ("covariance", ctypes.c_double * 36)]
 
 
+class Motion(ctypes.Structure):
+    """tsl_motion: the six normal distributions of a motion step and the noise seed."""
+    _fields_ = [("mean", ctypes.c_double * 6), ("stddev", ctypes.c_double * 6), ("seed", ctypes.c_uint64)]
+
+
 assert ctypes.sizeof(MapCoef) == 120
 
 if not os.path.exists(LIB_PATH):
@@ -89,9 +94,25 @@ _SIGS = {
     "tsl_filter_sync": (_I, [_P, _P, ctypes.POINTER(Pose), ctypes.POINTER(_S)]),
     "tsl_filter_snapshot": (_I, [_P]),
     "tsl_filter_restore_async": (_I, [_P, _P]),
-    "tsl_filter_set_tile": (_I, [_P, _I, ctypes.c_uint32]),
-    "tsl_filter_autotune": (_I, [_P, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_uint32),
-                                 ctypes.POINTER(_F)]),
+    "tsl_filter_set_tile": (_I, [_P, _I, _I, ctypes.c_uint32]),
+    "tsl_filter_autotune": (_I, [_P, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                 ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_F)]),
+    "tsl_filter_set_order": (_I, [_P, _I]),
+    "tsl_batch_create": (_I, [_P, _P, _S, ctypes.POINTER(_P)]),
+    "tsl_batch_destroy": (_I, [_P]),
+    "tsl_batch_set_tile": (_I, [_P, _I, _I, ctypes.c_uint32]),
+    "tsl_batch_prepare": (_I, [_P]),
+    "tsl_batch_restore_async": (_I, [_P, _P]),
+    "tsl_batch_update_async": (_I, [_P, _P]),
+    "tsl_batch_resample_async": (_I, [_P, _P, _P]),
+    "tsl_batch_step_async": (_I, [_P, _P, _I, _P]),
+    "tsl_batch_capture": (_I, [_P, _I, _P]),
+    "tsl_batch_launch": (_I, [_P, _P, _P]),
+    "tsl_batch_sync": (_I, [_P, _P, _P, _P, _P]),
+    "tsl_batch_motion_async": (_I, [_P, _P, _P]),
+    "tsl_motion_from_delta": (_I, [_F, _F, _F, _F, _F, _F, _F, _F, _P, ctypes.c_uint64, ctypes.POINTER(Motion)]),
+    "tsl_motion_from_odom": (_I, [_F, _F, _F, _P, ctypes.c_uint64, ctypes.POINTER(Motion)]),
+    "tsl_filter_motion_async": (_I, [_P, ctypes.POINTER(Motion), _P]),
     "tsl_filter_set_timing": (_I, [_P, _I]),
     "tsl_filter_eval_timing": (_I, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
                                     ctypes.POINTER(ctypes.c_uint64)]),

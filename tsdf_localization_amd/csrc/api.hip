@@ -47,9 +47,6 @@ uint32_t grid_threshold(uint64_t dim, float res)
     return lo;
 }
 
-int launch_resample_buffers(const tsl_particle* d_in, size_t M, float u0, size_t cap, double* d_s,
-                            float* d_u, int32_t* d_idx, tsl_particle* d_out, uint32_t* d_count,
-                            uint32_t* d_status, uint32_t* d_first_bad, hipStream_t stream);
 int launch_set_count(tsl_filter* f, uint32_t n, hipStream_t stream);
 
 int filter_reserve_particles(tsl_filter* f, size_t n)
@@ -81,10 +78,63 @@ int filter_reserve_points(tsl_filter* f, size_t n)
 
 static FilterState* host_state(tsl_filter* f) { return static_cast<FilterState*>(f->host_state); }
 
+// Device descriptor of the filter, uploaded when any of its buffers or its
+// scan changed since the last upload.
+int filter_member(tsl_filter* f, Member** d_member, uint32_t chunk_len, uint32_t chunks)
+{
+    TSL_TRY(f->plan.reserve(sizeof(RsPlan)));
+    TSL_TRY(f->member.reserve(sizeof(Member)));
+    Member h;
+    std::memset(&h, 0, sizeof(h));
+    h.particles = f->particles.as<tsl_particle>();
+    h.alt = f->particles_alt.as<tsl_particle>();
+    h.L = f->L.as<float>();
+    h.points = f->points.as<float4>();
+    h.partial = f->partial.as<float>();
+    h.raw = f->raw.as<float>();
+    h.inv = f->last_order ? f->pinv.as<uint32_t>() : nullptr;
+    h.blocks = f->blocks.as<double>();
+    h.st = f->state.as<FilterState>();
+    h.cumsum = f->cumsum.as<double>();
+    h.ubuf = f->ubuf.as<float>();
+    h.idx_out = nullptr;
+    h.plan = f->plan.as<RsPlan>();
+    h.u0_dev = nullptr;
+    std::memcpy(h.tf, f->tf, sizeof(h.tf));
+    h.N = (uint32_t)f->num_points;
+    h.chunk_len = chunks ? chunk_len : f->member_host.chunk_len;
+    h.chunks = chunks ? chunks : f->member_host.chunks;
+    h.stride = (uint32_t)f->capacity;
+    h.snapshot = f->snapshot.as<tsl_particle>();
+    h.snapshot_count = (uint32_t)f->snapshot_count;
+    const size_t ucap = f->ubuf.bytes / sizeof(float);
+    h.cap_limit = (uint32_t)std::min(f->capacity, ucap);
+    if (f->member_dirty || std::memcmp(&h, &f->member_host, sizeof(h)) != 0) {
+        TSL_HIP_CHECK(hipMemcpy(f->member.ptr, &h, sizeof(h), hipMemcpyHostToDevice));
+        f->member_host = h;
+        f->member_dirty = false;
+    }
+    *d_member = f->member.as<Member>();
+    return TSL_OK;
+}
+
+int status_error(uint32_t status)
+{
+    switch (status) {
+    case kStOk: return TSL_OK;
+    case kStNoValid: return set_error(TSL_ERR_NO_VALID_PARTICLE, "No particle is valid!");
+    case kStCapacity:
+        return set_error(TSL_ERR_CAPACITY, "resampling: weights sum beyond 1 + 1e-5 or output beyond capacity");
+    case kStEmpty: return set_error(TSL_ERR_INVALID_ARGUMENT, "filter holds no particles");
+    case kStNoAdvance: return set_error(TSL_ERR_UNSUPPORTED, "systematic draw does not advance");
+    default: return set_error(TSL_ERR_HIP, "unknown filter status %u", status);
+    }
+}
+
 // pose from the ten sums: weighted mean position, circular mean of each
 // angle (tsdf_evaluator.cpp:219-221), quaternion as tf2::Quaternion::setRPY
 // (roll about x, pitch about y, yaw about z, q = qz(yaw) qy(pitch) qx(roll)).
-static void pose_from_stats(const double* st, tsl_pose* pose)
+void pose_from_stats(const double* st, tsl_pose* pose)
 {
     std::memset(pose, 0, sizeof(*pose));
     pose->position[0] = (float)st[1];
@@ -168,7 +218,7 @@ static int validate_coef(const tsl_map_coef* c)
 }
 
 static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uint16_t>& codes,
-                      const std::vector<float>& palette)
+                      const std::vector<float>& palette, uint16_t init_code)
 {
     const tsl_map_coef& c = m->coef;
     // brick layout when every block offset is an aligned multiple of sd^3
@@ -192,14 +242,17 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
         }
         upload = &phys;
     }
+    // one sentinel element after the voxels holds the code of init_value:
+    // rejected lookups read it instead of branching
     TSL_TRY(m->grid_occ.reserve(sizeof(int32_t) * c.grid_occ_size));
-    TSL_TRY(m->codes.reserve(sizeof(uint16_t) * (phys_size ? phys_size : 1)));
+    TSL_TRY(m->codes.reserve(sizeof(uint16_t) * (phys_size + 1)));
     TSL_TRY(m->palette.reserve(sizeof(float) * (palette.empty() ? 1 : palette.size())));
     TSL_HIP_CHECK(hipMemcpy(m->grid_occ.ptr, grid_occ, sizeof(int32_t) * c.grid_occ_size,
                             hipMemcpyHostToDevice));
     if (phys_size)
         TSL_HIP_CHECK(hipMemcpy(m->codes.ptr, upload->data(), sizeof(uint16_t) * phys_size,
                                 hipMemcpyHostToDevice));
+    TSL_HIP_CHECK(hipMemcpy(m->codes.as<uint16_t>() + phys_size, &init_code, sizeof(uint16_t), hipMemcpyHostToDevice));
     if (!palette.empty())
         TSL_HIP_CHECK(hipMemcpy(m->palette.ptr, palette.data(), sizeof(float) * palette.size(),
                                 hipMemcpyHostToDevice));
@@ -260,6 +313,34 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     d.data_size = c.data_size;
     d.brick = brick ? 1u : 0u;
     d.sd3 = (uint32_t)sd3;
+    d.sentinel = (uint32_t)phys_size;
+    d.sentinel_code = init_code;
+    {
+        // occupancy-rank form of the block table
+        const uint64_t nwords = (c.grid_occ_size + 63) / 64;
+        bool rank = nwords <= kMaxOccWords && sd3 < (1ull << 31);
+        std::vector<OccWord> words;
+        if (rank) {
+            words.assign(nwords, OccWord{0u, 0u, 0u, 0u});
+            uint64_t n = 0;
+            for (uint64_t b = 0; rank && b < c.grid_occ_size; ++b) {
+                if (b % 64 == 0) words[b / 64].prefix = (uint32_t)n;
+                if (grid_occ[b] < 0) continue;
+                if ((uint64_t)grid_occ[b] != n * sd3) rank = false;
+                if (b % 64 < 32) words[b / 64].lo |= 1u << (b % 32);
+                else words[b / 64].hi |= 1u << (b % 32);
+                ++n;
+            }
+        }
+        d.occ_nwords = 0;
+        d.occ_words = nullptr;
+        if (rank) {
+            TSL_TRY(m->occ_words.reserve(sizeof(OccWord) * nwords));
+            TSL_HIP_CHECK(hipMemcpy(m->occ_words.ptr, words.data(), sizeof(OccWord) * nwords, hipMemcpyHostToDevice));
+            d.occ_words = m->occ_words.as<OccWord>();
+            d.occ_nwords = (uint32_t)nwords;
+        }
+    }
     {
         const uint32_t B = (uint32_t)(sd / 4);
         d.brick_cy = 64u * B - 16u;
@@ -295,11 +376,23 @@ int tsl_map_create(const tsl_map_coef* coef, const int32_t* grid_occ, const floa
         }
         codes[i] = it->second;
     }
+    uint16_t init_code;
+    {
+        uint32_t bits;
+        std::memcpy(&bits, &coef->init_value, 4);
+        auto it = seen.find(bits);
+        if (it != seen.end()) {
+            init_code = it->second;
+        } else {
+            init_code = (uint16_t)palette.size();  // at most kMaxPalette + 1 entries
+            palette.push_back(coef->init_value);
+        }
+    }
     tsl_map* m = new (std::nothrow) tsl_map();
     if (!m) return set_error(TSL_ERR_OUT_OF_MEMORY, "out of host memory");
     (void)hipGetDevice(&m->device);
     m->coef = *coef;
-    int rc = map_upload(m, grid_occ, codes, palette);
+    int rc = map_upload(m, grid_occ, codes, palette, init_code);
     if (rc != TSL_OK) {
         delete m;
         return rc;
@@ -356,7 +449,7 @@ int tsl_map_create_tsdf(const tsl_map_coef* coef, const int32_t* grid_occ, const
     if (!m) return set_error(TSL_ERR_OUT_OF_MEMORY, "out of host memory");
     (void)hipGetDevice(&m->device);
     m->coef = *coef;
-    int rc = map_upload(m, grid_occ, codes, palette);
+    int rc = map_upload(m, grid_occ, codes, palette, init_code);
     if (rc != TSL_OK) {
         delete m;
         return rc;
@@ -424,7 +517,7 @@ int tsl_map_device_bytes(const tsl_map* m, uint64_t* bytes)
 {
     if (!m || !bytes) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
     *bytes = m->grid_occ.bytes + m->codes.bytes + m->palette.bytes + m->sub_tab.bytes + m->full.bytes +
-             m->neg_tab.bytes;
+             m->neg_tab.bytes + m->occ_words.bytes;
     return TSL_OK;
 }
 
@@ -472,6 +565,8 @@ int tsl_filter_destroy(tsl_filter* f)
     (void)hipSetDevice(f->device);
     (void)hipDeviceSynchronize();
     if (f->host_state) (void)hipHostFree(f->host_state);
+    if (f->motion_host) (void)hipHostFree(f->motion_host);
+    if (f->motion_copied) (void)hipEventDestroy(static_cast<hipEvent_t>(f->motion_copied));
     if (f->ev_start) (void)hipEventDestroy(static_cast<hipEvent_t>(f->ev_start));
     if (f->ev_stop) (void)hipEventDestroy(static_cast<hipEvent_t>(f->ev_stop));
     delete f;
@@ -536,30 +631,22 @@ int tsl_filter_resample_async(tsl_filter* f, float u0, void* stream)
     if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
     TSL_HIP_CHECK(hipSetDevice(f->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    // M must be known exactly on the host to build the draw table; after a
-    // previous resampling it is only known on the device
-    FilterState* hs = host_state(f);
-    if (!f->count_known) {
-        TSL_HIP_CHECK(hipMemcpyAsync(hs, f->state.ptr, sizeof(FilterState), hipMemcpyDeviceToHost, s));
-        TSL_HIP_CHECK(hipStreamSynchronize(s));
-        f->count_bound = hs->count;
-        f->count_known = true;
-    }
+    // the device plans the draw from its own count; the host only sizes the
+    // buffers for the largest output the current count bound allows
     const size_t M = f->count_bound;
     if (M == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "filter holds no particles");
-    size_t cap = 0;
-    TSL_TRY(resample_capacity(M, u0, &cap));
-    if (cap > f->capacity) {
-        TSL_TRY(filter_reserve_particles(f, cap));
+    const uint64_t need = resample_bound(M);
+    if (need == 0 || need > 0x7fffffffull)
+        return set_error(TSL_ERR_UNSUPPORTED, "systematic draw does not advance for %zu particles", M);
+    if (need > f->capacity || f->ubuf.bytes < sizeof(float) * need) {
+        TSL_HIP_CHECK(hipStreamSynchronize(s));  // buffers are reallocated below
+        if (need > f->capacity) TSL_TRY(filter_reserve_particles(f, need));
+        TSL_TRY(f->ubuf.reserve(sizeof(float) * need));
     }
-    TSL_TRY(f->ubuf.reserve(sizeof(float) * cap));
-    FilterState* st = f->state.as<FilterState>();
-    TSL_TRY(launch_resample_buffers(f->particles.as<tsl_particle>(), M, u0, cap, f->cumsum.as<double>(),
-                                    f->ubuf.as<float>(), nullptr, f->particles_alt.as<tsl_particle>(),
-                                    &st->count, &st->status, &st->first_bad, s));
-    std::swap(f->particles.ptr, f->particles_alt.ptr);
-    std::swap(f->particles.bytes, f->particles_alt.bytes);
-    f->count_bound = cap;
+    Member* dm;
+    TSL_TRY(filter_member(f, &dm));
+    TSL_TRY(launch_resample_members(dm, 1, (uint32_t)need, u0, true, s));
+    f->count_bound = need;
     f->count_known = false;
     return TSL_OK;
 }
@@ -592,16 +679,28 @@ int tsl_filter_restore_async(tsl_filter* f, void* stream)
     return TSL_OK;
 }
 
-int tsl_filter_set_tile(tsl_filter* f, int k, uint32_t chunk_len)
+int tsl_filter_set_tile(tsl_filter* f, int k, int points_per_lane, uint32_t chunk_len)
 {
     if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
     if (k != 0 && k != 1 && k != 2) return set_error(TSL_ERR_INVALID_ARGUMENT, "K must be 0, 1 or 2");
+    if (points_per_lane != 0 && points_per_lane != 1 && points_per_lane != 2 && points_per_lane != 4)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "points per lane must be 0, 1, 2 or 4");
     f->tile_k = k;
+    f->tile_p = points_per_lane;
     f->tile_chunk = chunk_len;
     return TSL_OK;
 }
 
-int tsl_filter_autotune(tsl_filter* f, void* stream, int reps, int* best_k, uint32_t* best_chunk, float* best_ms)
+int tsl_filter_set_order(tsl_filter* f, int mode)
+{
+    if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
+    if (mode < -1 || mode > 1) return set_error(TSL_ERR_INVALID_ARGUMENT, "order mode must be -1, 0 or 1");
+    f->order_mode = mode;
+    return TSL_OK;
+}
+
+int tsl_filter_autotune(tsl_filter* f, void* stream, int reps, int* best_order, int* best_k,
+                        int* best_points_per_lane, uint32_t* best_chunk, float* best_ms)
 {
     if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
     TSL_HIP_CHECK(hipSetDevice(f->device));
@@ -611,7 +710,8 @@ int tsl_filter_autotune(tsl_filter* f, void* stream, int reps, int* best_k, uint
         f->count_bound = hs->count;
         f->count_known = true;
     }
-    return autotune_tile(f, static_cast<hipStream_t>(stream), reps > 0 ? reps : 3, best_k, best_chunk, best_ms);
+    return autotune_tile(f, static_cast<hipStream_t>(stream), reps > 0 ? reps : 3, best_order, best_k,
+                         best_points_per_lane, best_chunk, best_ms);
 }
 
 int tsl_filter_set_timing(tsl_filter* f, int enable)
@@ -663,10 +763,9 @@ int tsl_filter_sync(tsl_filter* f, void* stream, tsl_pose* pose, size_t* count)
     f->count_bound = hs->count;
     f->count_known = true;
     if (pose) pose_from_stats(hs->stats, pose);
-    if (hs->status == 1u) return set_error(TSL_ERR_NO_VALID_PARTICLE, "No particle is valid!");
-    if (hs->status == 2u) return set_error(TSL_ERR_CAPACITY, "resampling emitted more particles than expected");
-    return TSL_OK;
+    return status_error(hs->status);
 }
+
 
 int tsl_evaluator_create(tsl_map* map, int per_point, float a_hit, float a_range, float a_max,
                          float max_range, tsl_evaluator** out)
@@ -743,7 +842,7 @@ int tsl_evaluate_raw(tsl_evaluator* ev, const tsl_particle* particles, size_t P,
     TSL_TRY(scratch_filter(ev, P, &f));
     TSL_TRY(tsl_filter_set_particles(f, particles, P));
     TSL_TRY(tsl_filter_set_scan(f, points, N, tf));
-    TSL_TRY(launch_raw(f, 0, f->raw.as<float>()));
+    TSL_TRY(launch_raw(f, 0));
     TSL_HIP_CHECK(hipDeviceSynchronize());
     TSL_HIP_CHECK(hipMemcpy(weights_out, f->raw.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));
     return TSL_OK;
@@ -803,30 +902,45 @@ int tsl_resample_systematic(const tsl_particle* in, size_t M, float u0, tsl_part
                             int32_t* out_indices, size_t* out_count)
 {
     if (!in || !out_count || M == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument or M == 0");
+    if (M > 0x7fffffffull) return set_error(TSL_ERR_UNSUPPORTED, "too many particles");
     size_t cap = 0;
     TSL_TRY(resample_capacity(M, u0, &cap));
-    DevBuf d_in, d_out, d_s, d_u, d_idx, d_st;
+    if (cap > 0xffffffffull) return set_error(TSL_ERR_UNSUPPORTED, "resampling output too large");
+    DevBuf d_in, d_out, d_s, d_u, d_idx, d_st, d_plan, d_mem;
     TSL_TRY(d_in.reserve(sizeof(tsl_particle) * M));
     TSL_TRY(d_out.reserve(sizeof(tsl_particle) * cap));
     TSL_TRY(d_s.reserve(sizeof(double) * M));
     TSL_TRY(d_u.reserve(sizeof(float) * cap));
     TSL_TRY(d_idx.reserve(sizeof(int32_t) * cap));
-    TSL_TRY(d_st.reserve(3 * sizeof(uint32_t)));
+    TSL_TRY(d_st.reserve(sizeof(FilterState)));
+    TSL_TRY(d_plan.reserve(sizeof(RsPlan)));
+    TSL_TRY(d_mem.reserve(sizeof(Member)));
     TSL_HIP_CHECK(hipMemcpy(d_in.ptr, in, sizeof(tsl_particle) * M, hipMemcpyHostToDevice));
-    uint32_t* st = d_st.as<uint32_t>();
-    const uint32_t init_state[3] = {0u, 0u, 0xffffffffu};
-    TSL_HIP_CHECK(hipMemcpy(st, init_state, sizeof(init_state), hipMemcpyHostToDevice));
-    TSL_TRY(launch_resample_buffers(d_in.as<tsl_particle>(), M, u0, cap, d_s.as<double>(), d_u.as<float>(),
-                                    d_idx.as<int32_t>(), d_out.as<tsl_particle>(), st, st + 1, st + 2, 0));
+    FilterState hs{};
+    hs.count = (uint32_t)M;
+    hs.first_bad = 0xffffffffu;
+    TSL_HIP_CHECK(hipMemcpy(d_st.ptr, &hs, sizeof(hs), hipMemcpyHostToDevice));
+    Member h;
+    std::memset(&h, 0, sizeof(h));
+    h.particles = d_in.as<tsl_particle>();
+    h.alt = d_out.as<tsl_particle>();
+    h.st = d_st.as<FilterState>();
+    h.cumsum = d_s.as<double>();
+    h.ubuf = d_u.as<float>();
+    h.idx_out = d_idx.as<int32_t>();
+    h.plan = d_plan.as<RsPlan>();
+    h.cap_limit = (uint32_t)cap;
+    TSL_HIP_CHECK(hipMemcpy(d_mem.ptr, &h, sizeof(h), hipMemcpyHostToDevice));
+    TSL_TRY(launch_resample_members(d_mem.as<Member>(), 1, (uint32_t)cap, u0, false, 0));
     TSL_HIP_CHECK(hipDeviceSynchronize());
-    uint32_t h[2];
-    TSL_HIP_CHECK(hipMemcpy(h, st, sizeof(h), hipMemcpyDeviceToHost));
-    if (h[1] != 0) return set_error(TSL_ERR_CAPACITY, "weights sum beyond 1 + 1e-5: normalise them first");
-    *out_count = h[0];
-    if (h[0] > out_capacity) return set_error(TSL_ERR_CAPACITY, "resampling emits %u particles", h[0]);
-    if (out && h[0]) TSL_HIP_CHECK(hipMemcpy(out, d_out.ptr, sizeof(tsl_particle) * h[0], hipMemcpyDeviceToHost));
-    if (out_indices && h[0])
-        TSL_HIP_CHECK(hipMemcpy(out_indices, d_idx.ptr, sizeof(int32_t) * h[0], hipMemcpyDeviceToHost));
+    TSL_HIP_CHECK(hipMemcpy(&hs, d_st.ptr, sizeof(hs), hipMemcpyDeviceToHost));
+    if (hs.status == kStCapacity) return set_error(TSL_ERR_CAPACITY, "weights sum beyond 1 + 1e-5: normalise them first");
+    if (hs.status != kStOk) return set_error(TSL_ERR_UNSUPPORTED, "resampling failed (status %u)", hs.status);
+    const uint32_t J = hs.resample_count;
+    *out_count = J;
+    if (J > out_capacity) return set_error(TSL_ERR_CAPACITY, "resampling emits %u particles", J);
+    if (out && J) TSL_HIP_CHECK(hipMemcpy(out, d_out.ptr, sizeof(tsl_particle) * J, hipMemcpyDeviceToHost));
+    if (out_indices && J) TSL_HIP_CHECK(hipMemcpy(out_indices, d_idx.ptr, sizeof(int32_t) * J, hipMemcpyDeviceToHost));
     return TSL_OK;
 }
 

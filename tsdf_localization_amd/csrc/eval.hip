@@ -21,6 +21,7 @@ namespace {
 
 constexpr int kBlock = 256;           // threads per workgroup (4 wavefronts)
 constexpr int kWaves = kBlock / 64;
+constexpr uint32_t kXcds = 8;         // accelerator complex dies of the MI355X (one L2 each)
 
 __device__ __forceinline__ float wave_sum(float v)
 {
@@ -71,40 +72,98 @@ __global__ void k_pose(const tsl_particle* __restrict__ particles, const uint32_
     for (int k = 0; k < 12; ++k) L[(size_t)i * 12 + k] = out[k];
 }
 
-// One wavefront: K particles (pose matrices wavefront-uniform) x the points
-// [j0, j1) of one chunk, 64 consecutive points per iteration.  Each
-// iteration picks, by wavefront-uniform ballot, the cheapest lookup tier
-// every lane qualifies for (fast, else wrap, else per-lane wrap/general).
-// kMode 1 writes the voxel index of every pair, kMode 2 the voxel value,
-// instead of accumulating the score, with the same tiling, tier selection
-// and addressing as the production kernel (kMode 0); trace_out is indexed
-// by the caller's point order (perm).
-template <int K, int kMode, bool kU24, bool kBrick>
-__global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __restrict__ L,
-                                                 const uint32_t* __restrict__ count,
-                                                 const float4* __restrict__ pts, uint32_t N,
-                                                 uint32_t chunk_len, float a_hit,
-                                                 float* __restrict__ partial, uint32_t stride,
-                                                 void* __restrict__ trace_out, const uint32_t* __restrict__ perm)
+// Exact lookup of one (particle, point) pair outside the fast tier: wrap
+// tier if every axis offset is within R, else the general tier.
+template <bool kU24, bool kRank>
+__device__ __forceinline__ uint64_t slow_index(const DevMap& map, const SubEntry* __restrict__ tab,
+                                               const uint64_t* __restrict__ neg, const OccWord* __restrict__ occ,
+                                               float ox, float oy, float oz)
 {
-    constexpr bool kTrace = kMode != 0;
+    if (wrap_ok(ox, oy, oz, map.range_bits)) return wrap_index<kU24, kRank>(map, tab, neg, occ, ox, oy, oz);
+    return lookup_index_general(*map.full, ox, oy, oz);
+}
+
+// Pose matrices in the evaluation order: slot s holds particle order[s];
+// inv[order[s]] = s lets the reduction read each particle's partial
+// weights back in the caller's order.
+__global__ void k_pose_ordered(const tsl_particle* __restrict__ particles, const uint32_t* __restrict__ count,
+                               const uint32_t* __restrict__ order, Tf12 tf, float* __restrict__ L,
+                               uint32_t* __restrict__ inv)
+{
+    const uint32_t n = *count;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = order[i];
+    float pose[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pose[k] = particles[o].first[k];
+    float out[12];
+    pose_matrix(pose, tf.m, out);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) L[(size_t)i * 12 + k] = out[k];
+    inv[o] = i;
+}
+
+// LDS copies of the per-map tables every lookup reads: occupancy-rank
+// words, wrap-band table, sub-voxel thresholds, value palette.
+struct LdsTables {
+    const OccWord* occ;
+    const uint64_t* neg;
+    const SubEntry* tab;
+    const float* lut;
+};
+
+template <bool kRank>
+__device__ __forceinline__ LdsTables stage_tables(const DevMap& map)
+{
     extern __shared__ uint4 smem_raw[];
-    uint64_t* neg = reinterpret_cast<uint64_t*>(smem_raw);
+    OccWord* occ = reinterpret_cast<OccWord*>(smem_raw);
+    uint64_t* neg = reinterpret_cast<uint64_t*>(occ + (kRank ? map.occ_nwords : 0u));
     SubEntry* tab = reinterpret_cast<SubEntry*>(neg + 8);
     float* lut = reinterpret_cast<float*>(tab + map.sub_entries);
+    if (kRank) {
+        const uint4* src = reinterpret_cast<const uint4*>(map.occ_words);
+        uint4* dst = reinterpret_cast<uint4*>(occ);
+        for (uint32_t i = threadIdx.x; i < map.occ_nwords; i += blockDim.x) dst[i] = src[i];
+    }
     if (threadIdx.x < 8) neg[threadIdx.x] = map.neg_tab[threadIdx.x];
     for (uint32_t i = threadIdx.x; i < map.sub_entries; i += blockDim.x) tab[i] = map.sub_tab[i];
     for (uint32_t i = threadIdx.x; i < map.palette_size; i += blockDim.x) lut[i] = map.palette[i];
     __syncthreads();
+    return LdsTables{occ, neg, tab, lut};
+}
 
-    const uint32_t P = *count;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+// LDS bytes stage_tables needs for a map.
+inline size_t lds_bytes(const DevMap& d)
+{
+    return (d.occ_nwords ? sizeof(OccWord) * d.occ_nwords : 0) + 8 * sizeof(uint64_t) +
+           sizeof(SubEntry) * d.sub_entries + sizeof(float) * d.palette_size;
+}
+
+// One wavefront: K particles (pose matrices wavefront-uniform) x the points
+// [j0, j1) of one chunk.  Each iteration covers 64*kP consecutive points,
+// lane l taking points base + l + 64u (u < kP), so kP independent gather
+// chains per particle are in flight per lane; lane l still adds its points in
+// ascending order, so the sums do not depend on kP.  Each iteration picks,
+// by wavefront-uniform ballot, the fast tier if every pair qualifies, else
+// the exact wrap/general tiers per pair.  kMode 1 writes the voxel index of
+// every pair, kMode 2 the voxel value, instead of accumulating the score,
+// with the same tiling, tier selection and addressing as the production
+// mode (kMode 0); trace_out is indexed by the caller's point order (perm).
+// kMode 0 writes the chunk's partial weight of particle p0 + k to
+// partial_row[p0 + k].
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank>
+__device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t, const float* __restrict__ L,
+                                          uint32_t P, uint32_t p0, const float4* __restrict__ pts, uint32_t N,
+                                          uint32_t j0, uint32_t j1, float a_hit, float* __restrict__ partial_row,
+                                          void* __restrict__ trace_out, const uint32_t* __restrict__ perm)
+{
+    constexpr bool kTrace = kMode != 0;
+    const OccWord* occ = t.occ;
+    const uint64_t* neg = t.neg;
+    const SubEntry* tab = t.tab;
+    const float* lut = t.lut;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t p0 = (blockIdx.x * kWaves + wave) * K;
-    if (p0 >= P) return;
-    const uint32_t j0 = blockIdx.y * chunk_len;
-    const uint32_t j1 = min(N, j0 + chunk_len);
-
     float m[K][12];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -116,83 +175,113 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0f;
 
-    for (uint32_t base = j0; base < j1; base += 64) {
-        const uint32_t j = base + lane;
-        const bool act = j < j1;
-        const float4 q = pts[act ? j : j0];
-        float ox[K], oy[K], oz[K];
+    for (uint32_t base = j0; base < j1; base += 64 * kP) {
+        float4 q[kP];
+        bool act[kP];
+#pragma unroll
+        for (int u = 0; u < kP; ++u) {
+            const uint32_t j = base + 64u * u + lane;
+            act[u] = j < j1;
+            q[u] = pts[act[u] ? j : j0];
+        }
+        float ox[kP][K], oy[kP][K], oz[kP][K];
         bool fast = true;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            // tsdf_evaluator.cpp:44-46: ((L0*x + L1*y) + L2*z) + L3, no FMA
-            const float tx = m[k][0] * q.x + m[k][1] * q.y + m[k][2] * q.z + m[k][3];
-            const float ty = m[k][4] * q.x + m[k][5] * q.y + m[k][6] * q.z + m[k][7];
-            const float tz = m[k][8] * q.x + m[k][9] * q.y + m[k][10] * q.z + m[k][11];
-            ox[k] = tx - map.min_x;
-            oy[k] = ty - map.min_y;
-            oz[k] = tz - map.min_z;
-            fast = fast & fast_ok(ox[k], oy[k], oz[k], map.range_bits);
-        }
-        float v[K];
-        uint64_t full[K];
-        if (__builtin_amdgcn_ballot_w64(!fast) == 0) {
-            int32_t s[K];
-            uint32_t small[K], pbrick[K];
-            bool carry[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                fast_block<kU24, kBrick>(map, tab, ox[k], oy[k], oz[k], s[k], small[k], pbrick[k], carry[k]);
-            uint32_t code[K];
-            bool valid[K];
+        for (int u = 0; u < kP; ++u) {
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const uint32_t sum = (uint32_t)s[k] + small[k];
-                valid[k] = (s[k] >= 0) & (sum < map.ds);
-                if (kTrace) full[k] = s[k] >= 0 ? (uint64_t)(uint32_t)s[k] + small[k] : map.data_size;
-                uint32_t addr = (uint32_t)s[k] + pbrick[k];
-                if (kBrick && __builtin_expect(carry[k], 0)) addr = phys_from_linear(map, sum);
-                code[k] = 0;
-                if (valid[k]) code[k] = map.codes[addr];
+                // tsdf_evaluator.cpp:44-46: ((L0*x + L1*y) + L2*z) + L3, no FMA
+                const float tx = m[k][0] * q[u].x + m[k][1] * q[u].y + m[k][2] * q[u].z + m[k][3];
+                const float ty = m[k][4] * q[u].x + m[k][5] * q[u].y + m[k][6] * q[u].z + m[k][7];
+                const float tz = m[k][8] * q[u].x + m[k][9] * q[u].y + m[k][10] * q[u].z + m[k][11];
+                ox[u][k] = tx - map.min_x;
+                oy[u][k] = ty - map.min_y;
+                oz[u][k] = tz - map.min_z;
+                fast = fast & fast_ok(ox[u][k], oy[u][k], oz[u][k], map.range_bits);
             }
+        }
+        if (__builtin_amdgcn_ballot_w64(!fast) == 0) {
+            float v[kP][K];
+            uint64_t full[kP][K];
+            uint32_t addr[kP][K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) v[k] = valid[k] ? lut[code[k]] : map.init;
-        } else {
-            bool wrap = true;
-#pragma unroll
-            for (int k = 0; k < K; ++k) wrap = wrap & wrap_ok(ox[k], oy[k], oz[k], map.range_bits);
-            if (__builtin_amdgcn_ballot_w64(!wrap) == 0) {
-#pragma unroll
-                for (int k = 0; k < K; ++k) full[k] = wrap_index<kU24>(map, tab, neg, ox[k], oy[k], oz[k]);
-            } else {
+            for (int u = 0; u < kP; ++u) {
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    if (wrap_ok(ox[k], oy[k], oz[k], map.range_bits))
-                        full[k] = wrap_index<kU24>(map, tab, neg, ox[k], oy[k], oz[k]);
-                    else
-                        full[k] = lookup_index_general(*map.full, ox[k], oy[k], oz[k]);
+                    int32_t s;
+                    uint32_t small, pbrick;
+                    bool carry;
+                    fast_block<kU24, kBrick, kRank>(map, tab, occ, ox[u][k], oy[u][k], oz[u][k], s, small, pbrick,
+                                                    carry);
+                    const uint32_t sum = (uint32_t)s + small;
+                    const bool valid = (s >= 0) & (sum < map.ds);
+                    if (kTrace) full[u][k] = s >= 0 ? (uint64_t)(uint32_t)s + small : map.data_size;
+                    uint32_t a = (uint32_t)s + pbrick;
+                    if (kBrick && __builtin_expect(carry, 0)) a = phys_from_linear(map, sum);
+                    addr[u][k] = valid ? a : map.sentinel;
+                }
+            }
+            // rejected pairs take the sentinel element (code of init_value)
+            // without a load
+            uint32_t code[kP][K];
+#pragma unroll
+            for (int u = 0; u < kP; ++u) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    code[u][k] = map.sentinel_code;
+                    if (addr[u][k] != map.sentinel) code[u][k] = map.codes[addr[u][k]];
                 }
             }
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                v[k] = map.init;
-                if (full[k] < map.data_size) {
-                    const uint32_t x = (uint32_t)full[k];
-                    v[k] = lut[map.codes[kBrick ? phys_from_linear(map, x) : x]];
+            for (int u = 0; u < kP; ++u) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) v[u][k] = lut[code[u][k]];
+            }
+#pragma unroll
+            for (int u = 0; u < kP; ++u) {
+                const uint32_t j = base + 64u * u + lane;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if (kMode == 1) {
+                        if (act[u] && p0 + k < P)
+                            static_cast<uint64_t*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = full[u][k];
+                    } else if (kMode == 2) {
+                        if (act[u] && p0 + k < P)
+                            static_cast<float*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = v[u][k];
+                    } else if (act[u]) {
+                        acc[k] += a_hit * v[u][k] + q[u].w;  // tsdf_evaluator.cpp:56-67
+                    }
                 }
             }
-        }
-        if (kMode == 1) {
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (act && p0 + k < P) static_cast<uint64_t*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = full[k];
-        } else if (kMode == 2) {
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (act && p0 + k < P) static_cast<float*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = v[k];
         } else {
+            // exact tiers, one point group at a time (offsets recomputed from
+            // the point, bit-identical); same per-lane summation order
+#pragma unroll 1
+            for (int u = 0; u < kP; ++u) {
+                const uint32_t j = base + 64u * u + lane;
+                const bool a = j < j1;
+                const float4 qq = pts[a ? j : j0];
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (act) acc[k] += a_hit * v[k] + q.w;  // tsdf_evaluator.cpp:56-67
+                for (int k = 0; k < K; ++k) {
+                    const float tx = m[k][0] * qq.x + m[k][1] * qq.y + m[k][2] * qq.z + m[k][3];
+                    const float ty = m[k][4] * qq.x + m[k][5] * qq.y + m[k][6] * qq.z + m[k][7];
+                    const float tz = m[k][8] * qq.x + m[k][9] * qq.y + m[k][10] * qq.z + m[k][11];
+                    const uint64_t full =
+                        slow_index<kU24, kRank>(map, tab, neg, occ, tx - map.min_x, ty - map.min_y, tz - map.min_z);
+                    float v = map.init;
+                    if (full < map.data_size) {
+                        const uint32_t x = (uint32_t)full;
+                        v = lut[map.codes[kBrick ? phys_from_linear(map, x) : x]];
+                    }
+                    if (kMode == 1) {
+                        if (a && p0 + k < P) static_cast<uint64_t*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = full;
+                    } else if (kMode == 2) {
+                        if (a && p0 + k < P) static_cast<float*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = v;
+                    } else if (a) {
+                        acc[k] += a_hit * v + qq.w;
+                    }
+                }
+            }
         }
     }
     if (kTrace) return;
@@ -201,32 +290,87 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            if (p0 + k < P) partial[(size_t)blockIdx.y * stride + p0 + k] = acc[k];
+            if (p0 + k < P) partial_row[p0 + k] = acc[k];
     }
+}
+
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank>
+__global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __restrict__ L,
+                                                 const uint32_t* __restrict__ count,
+                                                 const float4* __restrict__ pts, uint32_t N,
+                                                 uint32_t chunk_len, uint32_t chunks, float a_hit,
+                                                 float* __restrict__ partial, uint32_t stride,
+                                                 void* __restrict__ trace_out, const uint32_t* __restrict__ perm)
+{
+    const LdsTables t = stage_tables<kRank>(map);
+    const uint32_t P = *count;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // Workgroups are dealt to the 8 XCDs round-robin by index; give each XCD
+    // a contiguous range of (particle group, chunk) items so that the
+    // particles one XCD works on together are neighbours in the evaluation
+    // order (spatially sorted particles then share L2 lines).
+    const uint32_t items = gridDim.x;
+    const uint32_t xcd = blockIdx.x % kXcds, r = blockIdx.x / kXcds;
+    const uint32_t q = items / kXcds, rem = items % kXcds;
+    const uint32_t item = xcd * q + min(xcd, rem) + r;
+    const uint32_t g = item / chunks, c = item - g * chunks;
+    const uint32_t p0 = (g * kWaves + wave) * K;
+    if (p0 >= P) return;
+    const uint32_t j0 = c * chunk_len;
+    const uint32_t j1 = min(N, j0 + chunk_len);
+    eval_wave<K, kP, kMode, kU24, kBrick, kRank>(map, t, L, P, p0, pts, N, j0, j1, a_hit,
+                                                 kMode == 0 ? partial + (size_t)c * stride : nullptr,
+                                                 trace_out, perm);
+}
+
+// ---------------------------------------------------------------------------
+// Member-indexed forms (blockIdx.y = member): a single filter is a batch of
+// one.  Sums are formed exactly as in the kernels above, so a filter gives
+// the same bits alone and inside a batch: extra blocks of a wider grid add
+// zeros after the filter's own blocks.
+// ---------------------------------------------------------------------------
+
+__global__ void k_pose_m(const Member* __restrict__ members)
+{
+    const Member& m = members[blockIdx.y];
+    const uint32_t n = m.st->count;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float pose[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pose[k] = m.particles[i].first[k];
+    float tf[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) tf[k] = m.tf[k];
+    float out[12];
+    pose_matrix(pose, tf, out);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m.L[(size_t)i * 12 + k] = out[k];
 }
 
 // raw[p] = sum over chunks (ascending) of partial[c][p], in double, stored as
 // float; block partial sums of raw in double; the last block to finish adds
-// the block partials in index order into stats[0].
-__global__ __launch_bounds__(kBlock) void k_reduce(const float* __restrict__ partial, uint32_t chunks,
-                                                   uint32_t stride, float* __restrict__ raw,
-                                                   double* __restrict__ blocks, FilterState* st)
+// the block partials in a fixed order into stats[0].
+__global__ __launch_bounds__(kBlock) void k_reduce_m(const Member* __restrict__ members)
 {
+    const Member& m = members[blockIdx.y];
+    FilterState* st = m.st;
     __shared__ double sh[kWaves];
     __shared__ bool last;
     const uint32_t P = st->count;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double w = 0.0;
     if (i < P) {
+        const uint32_t slot = m.inv ? m.inv[i] : i;
         double acc = 0.0;
-        for (uint32_t c = 0; c < chunks; ++c) acc += (double)partial[(size_t)c * stride + i];
+        for (uint32_t c = 0; c < m.chunks; ++c) acc += (double)m.partial[(size_t)c * m.stride + slot];
         const float wf = (float)acc;
-        raw[i] = wf;
+        m.raw[i] = wf;
         w = (double)wf;
     }
     const double b = block_sum_d(w, sh);
     if (threadIdx.x == 0) {
-        blocks[blockIdx.x] = b;
+        m.blocks[blockIdx.x] = b;
         __threadfence();
         const uint32_t done = atomicAdd(&st->counter0, 1u);
         last = (done == gridDim.x - 1);
@@ -234,13 +378,12 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const float* __restrict__ par
     __syncthreads();
     if (last) {
         __threadfence();
-        // fixed order: thread t adds partials t, t+256, ...; then the block tree
         double t = 0.0;
-        for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) t += *(volatile const double*)&blocks[k];
+        for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) t += *(volatile const double*)&m.blocks[k];
         const double total = block_sum_d(t, sh);
         if (threadIdx.x == 0) {
             st->stats[0] = total;
-            st->status = ((float)total == 0.0f) ? 1u : 0u;
+            st->status = ((float)total == 0.0f) ? kStNoValid : kStOk;
             st->counter0 = 0;
         }
     }
@@ -249,10 +392,10 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const float* __restrict__ par
 // w_i = raw_i / (float)sum written to particles[i].second (the reference's
 // particles[index].second /= weight_sum, float division), plus the nine
 // weighted pose sums (tsdf_evaluator.cpp:198-217) in double.
-__global__ __launch_bounds__(kBlock) void k_normalize(const float* __restrict__ raw,
-                                                      tsl_particle* __restrict__ particles,
-                                                      double* __restrict__ blocks, FilterState* st)
+__global__ __launch_bounds__(kBlock) void k_normalize_m(const Member* __restrict__ members)
 {
+    const Member& m = members[blockIdx.y];
+    FilterState* st = m.st;
     __shared__ double sh[kWaves];
     __shared__ bool last;
     const uint32_t P = st->count;
@@ -262,7 +405,8 @@ __global__ __launch_bounds__(kBlock) void k_normalize(const float* __restrict__ 
 #pragma unroll
     for (int k = 0; k < 9; ++k) q[k] = 0.0;
     if (i < P && sum != 0.0f) {
-        const float w = raw[i] / sum;
+        tsl_particle* particles = m.particles;
+        const float w = m.raw[i] / sum;
         particles[i].second = w;
         const double wd = (double)w;
         q[0] = (double)particles[i].first[0] * wd;
@@ -279,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_normalize(const float* __restrict__ 
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const double b = block_sum_d(q[k], sh);
-        if (threadIdx.x == 0) blocks[(size_t)k * gridDim.x + blockIdx.x] = b;
+        if (threadIdx.x == 0) m.blocks[(size_t)k * gridDim.x + blockIdx.x] = b;
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -289,14 +433,101 @@ __global__ __launch_bounds__(kBlock) void k_normalize(const float* __restrict__ 
     __syncthreads();
     if (last) {
         __threadfence();
-        for (int q = 0; q < 9; ++q) {
+        for (int qq = 0; qq < 9; ++qq) {
             double t = 0.0;
             for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x)
-                t += *(volatile const double*)&blocks[(size_t)q * gridDim.x + k];
+                t += *(volatile const double*)&m.blocks[(size_t)qq * gridDim.x + k];
             const double total = block_sum_d(t, sh);
-            if (threadIdx.x == 0) st->stats[1 + q] = total;
+            if (threadIdx.x == 0) st->stats[1 + qq] = total;
         }
         if (threadIdx.x == 0) st->counter1 = 0;
+    }
+}
+
+// Restore every member from its snapshot.
+__global__ void k_restore_m(const Member* __restrict__ members)
+{
+    const Member& m = members[blockIdx.y];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m.snapshot_count) m.particles[i] = m.snapshot[i];
+    if (i == 0) {
+        m.st->count = m.snapshot_count;
+        m.st->status = kStOk;
+    }
+}
+
+// Work plan of a batched evaluation: member f owns items
+// [prefix[f], prefix[f+1]), one item = kWaves*K particles x one chunk.
+__global__ __launch_bounds__(1024) void k_batch_plan(const Member* __restrict__ members, uint32_t n, uint32_t K,
+                                                     BatchPlan* __restrict__ plan)
+{
+    __shared__ uint32_t wsum[16];
+    const uint32_t f = threadIdx.x;
+    uint32_t items = 0;
+    if (f < n) {
+        const Member& m = members[f];
+        const uint32_t P = m.st->count;
+        const uint32_t per = kWaves * K;
+        items = (P + per - 1) / per * m.chunks;
+    }
+    // block exclusive scan
+    const uint32_t lane = f & 63, wave = f >> 6;
+    uint32_t incl = items;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (uint32_t w = 0; w < wave; ++w) woff += wsum[w];
+    const uint32_t excl = woff + incl - items;
+    if (f < n) plan->prefix[f] = excl;
+    if (f == n - 1) {
+        plan->prefix[n] = excl + items;
+        plan->total = excl + items;
+        plan->n = n;
+    }
+}
+
+// Batched evaluation: a persistent grid walks the work items of every
+// member (each workgroup item after item, stride gridDim.x); per item the
+// same wave body as k_eval.
+template <int K, int kP, bool kU24, bool kBrick, bool kRank>
+__global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member* __restrict__ members,
+                                                       const BatchPlan* __restrict__ plan, float a_hit)
+{
+    const LdsTables t = stage_tables<kRank>(map);
+    __shared__ uint32_t pre[kMaxMembers + 1];
+    const uint32_t n = plan->n;
+    for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) pre[i] = plan->prefix[i];
+    __syncthreads();
+    const uint32_t total = pre[n];
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // gridDim.x is a multiple of the XCD count; the blocks of one XCD walk
+    // a contiguous range of the items together
+    const uint32_t per_xcd = gridDim.x / kXcds;
+    const uint32_t xcd = blockIdx.x % kXcds, r = blockIdx.x / kXcds;
+    const uint32_t q = total / kXcds, rem = total % kXcds;
+    const uint32_t first = xcd * q + min(xcd, rem), last = first + q + (xcd < rem ? 1u : 0u);
+    for (uint32_t item = first + r; item < last; item += per_xcd) {
+        uint32_t lo = 0, hi = n;  // member: pre[lo] <= item < pre[lo + 1]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pre[mid] <= item) lo = mid; else hi = mid;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        const Member& m = members[lo];
+        const uint32_t local = item - pre[lo];
+        const uint32_t g = local / m.chunks, c = local - g * m.chunks;
+        const uint32_t P = m.st->count;
+        const uint32_t p0 = (g * kWaves + wave) * K;
+        if (p0 >= P) continue;
+        const uint32_t j0 = c * m.chunk_len;
+        const uint32_t j1 = min(m.N, j0 + m.chunk_len);
+        eval_wave<K, kP, 0, kU24, kBrick, kRank>(map, t, m.L, P, p0, m.points, m.N, j0, j1, a_hit,
+                                                 m.partial + (size_t)c * m.stride, nullptr, nullptr);
     }
 }
 
@@ -304,6 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_normalize(const float* __restrict__ 
 __global__ void k_set_count(FilterState* st, uint32_t n)
 {
     st->count = n;
+    st->status = kStOk;
 }
 
 // Restore the resident set from the snapshot and reset the count.
@@ -312,17 +544,24 @@ __global__ void k_restore(const tsl_particle* __restrict__ src, tsl_particle* __
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[i];
-    if (i == 0) st->count = n;
+    if (i == 0) {
+        st->count = n;
+        st->status = kStOk;
+    }
 }
 
 // Tile shape: K particles per wavefront (template instantiations 1, 2)
 // and the scan chunk a wavefront walks.  Automatic choice: K = 1 (fewest
 // scalar registers, full occupancy on gfx950) and enough point chunks to
 // give the 256 CUs at least four full rounds of wavefronts.
+}  // namespace
+
 Tile choose_tile(const tsl_filter* f, size_t P, size_t N)
 {
     Tile t;
     t.K = f->tile_k ? f->tile_k : 1;
+    t.P = f->tile_p ? f->tile_p : 1;
+    t.order = f->order_mode >= 0 ? f->order_mode : (P >= 8192 ? 1 : 0);
     const size_t groups = (P + t.K - 1) / t.K;
     size_t len;
     if (f->tile_chunk) {
@@ -346,38 +585,80 @@ Tile choose_tile(const tsl_filter* f, size_t P, size_t N)
     return t;
 }
 
+namespace {
+
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank>
+int launch_eval_t(dim3 grid, size_t lds, hipStream_t stream, const tsl_filter* f, const Tile& tile, size_t N,
+                  float* partial, void* trace_out)
+{
+    const tsl_map* map = f->ev->map;
+    FilterState* st = f->state.as<FilterState>();
+    hipLaunchKernelGGL((k_eval<K, kP, kMode, kU24, kBrick, kRank>), grid, dim3(kBlock), lds, stream, map->dev,
+                       f->L.as<float>(), &st->count, f->points.as<float4>(), (uint32_t)N, tile.chunk_len,
+                       tile.chunks, f->ev->a_hit, partial, (uint32_t)f->capacity, trace_out,
+                       f->perm.as<uint32_t>());
+    TSL_HIP_CHECK(hipGetLastError());
+    return TSL_OK;
+}
+
+template <int K, int kP, int kMode>
+int launch_eval_flags(bool u24, bool brick, bool rank, dim3 grid, size_t lds, hipStream_t stream,
+                      const tsl_filter* f, const Tile& tile, size_t N, float* partial, void* trace_out)
+{
+#define TSL_EVAL_ARGS grid, lds, stream, f, tile, N, partial, trace_out
+    if (u24) {
+        if (brick) {
+            if (rank) return launch_eval_t<K, kP, kMode, true, true, true>(TSL_EVAL_ARGS);
+            return launch_eval_t<K, kP, kMode, true, true, false>(TSL_EVAL_ARGS);
+        }
+        if (rank) return launch_eval_t<K, kP, kMode, true, false, true>(TSL_EVAL_ARGS);
+        return launch_eval_t<K, kP, kMode, true, false, false>(TSL_EVAL_ARGS);
+    }
+    if (brick) {
+        if (rank) return launch_eval_t<K, kP, kMode, false, true, true>(TSL_EVAL_ARGS);
+        return launch_eval_t<K, kP, kMode, false, true, false>(TSL_EVAL_ARGS);
+    }
+    if (rank) return launch_eval_t<K, kP, kMode, false, false, true>(TSL_EVAL_ARGS);
+    return launch_eval_t<K, kP, kMode, false, false, false>(TSL_EVAL_ARGS);
+#undef TSL_EVAL_ARGS
+}
+
+// Launch the evaluation kernel for a tile shape.  The trace modes always use
+// one point per lane per iteration (the lookup code is the same).
 template <int kMode>
 int launch_eval_kernel(const Tile& tile, const tsl_filter* f, size_t P, size_t N, hipStream_t stream,
                        float* partial, void* trace_out)
 {
     const tsl_map* map = f->ev->map;
-    FilterState* st = f->state.as<FilterState>();
     const size_t groups = (P + tile.K - 1) / tile.K;
-    dim3 grid((unsigned)((groups + kWaves - 1) / kWaves), tile.chunks);
-    const size_t lds = 8 * sizeof(uint64_t) + sizeof(SubEntry) * map->dev.sub_entries +
-                       sizeof(float) * map->dev.palette_size;
+    const size_t items = (groups + kWaves - 1) / kWaves * tile.chunks;
+    if (items > 0x7fffffffull) return set_error(TSL_ERR_UNSUPPORTED, "evaluation grid too large");
+    const dim3 grid((unsigned)items);
+    const bool rank = map->dev.occ_nwords != 0;
+    const size_t lds = lds_bytes(map->dev);
     const bool u24 = map->dev.up_dim_x < (1u << 23) && map->dev.up_dim_2 < (1u << 23) &&
                      map->dev.sub_dim_2 < (1 << 23);
     const bool brick = map->dev.brick != 0;
-#define TSL_LAUNCH_EVAL(KK, U24, BR)                                                                            \
-    hipLaunchKernelGGL((k_eval<KK, kMode, U24, BR>), grid, dim3(kBlock), lds, stream, map->dev, f->L.as<float>(), \
-                       &st->count, f->points.as<float4>(), (uint32_t)N, tile.chunk_len, f->ev->a_hit, partial,     \
-                       (uint32_t)f->capacity, trace_out, f->perm.as<uint32_t>())
-    const int sel = tile.K * 4 + (u24 ? 2 : 0) + (brick ? 1 : 0);
+    const int pp = kMode == 0 ? tile.P : 1;
+    const int sel = tile.K * 8 + pp;
+#define TSL_EVAL_FLAGS u24, brick, rank, grid, lds, stream, f, tile, N, partial, trace_out
     switch (sel) {
-    case 4: TSL_LAUNCH_EVAL(1, false, false); break;
-    case 5: TSL_LAUNCH_EVAL(1, false, true); break;
-    case 6: TSL_LAUNCH_EVAL(1, true, false); break;
-    case 7: TSL_LAUNCH_EVAL(1, true, true); break;
-    case 8: TSL_LAUNCH_EVAL(2, false, false); break;
-    case 9: TSL_LAUNCH_EVAL(2, false, true); break;
-    case 10: TSL_LAUNCH_EVAL(2, true, false); break;
-    case 11: TSL_LAUNCH_EVAL(2, true, true); break;
-    default: return set_error(TSL_ERR_INVALID_ARGUMENT, "tile K must be 1 or 2 (got %d)", tile.K);
+    case 9: return launch_eval_flags<1, 1, kMode>(TSL_EVAL_FLAGS);
+    case 17: return launch_eval_flags<2, 1, kMode>(TSL_EVAL_FLAGS);
+    default: break;
     }
-#undef TSL_LAUNCH_EVAL
-    TSL_HIP_CHECK(hipGetLastError());
-    return TSL_OK;
+    if (kMode == 0) {
+        switch (sel) {
+        case 10: return launch_eval_flags<1, 2, 0>(TSL_EVAL_FLAGS);
+        case 12: return launch_eval_flags<1, 4, 0>(TSL_EVAL_FLAGS);
+        case 18: return launch_eval_flags<2, 2, 0>(TSL_EVAL_FLAGS);
+        case 20: return launch_eval_flags<2, 4, 0>(TSL_EVAL_FLAGS);
+        default: break;
+        }
+    }
+#undef TSL_EVAL_FLAGS
+    return set_error(TSL_ERR_INVALID_ARGUMENT, "tile must have K in {1, 2} and points per lane in {1, 2, 4} (got %d, %d)",
+                     tile.K, tile.P);
 }
 
 }  // namespace
@@ -394,17 +675,34 @@ int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const floa
     return TSL_OK;
 }
 
+// Pose matrices for an evaluation: caller order, or spatial order with the
+// inverse map for the reduction.
+static int launch_pose_for_tile(tsl_filter* f, const Tile& tile, hipStream_t stream)
+{
+    const size_t P = f->count_bound;
+    FilterState* st = f->state.as<FilterState>();
+    Tf12 t;
+    for (int k = 0; k < 16; ++k) t.m[k] = f->tf[k];
+    const unsigned g = (unsigned)((P + 255) / 256);
+    if (tile.order) {
+        TSL_TRY(launch_particle_order(f, stream));
+        hipLaunchKernelGGL(k_pose_ordered, dim3(g), dim3(256), 0, stream, f->particles.as<tsl_particle>(),
+                           &st->count, f->porder.as<uint32_t>(), t, f->L.as<float>(), f->pinv.as<uint32_t>());
+    } else {
+        hipLaunchKernelGGL(k_pose, dim3(g), dim3(256), 0, stream, f->particles.as<tsl_particle>(), &st->count,
+                           (uint32_t)P, t, f->L.as<float>());
+    }
+    TSL_HIP_CHECK(hipGetLastError());
+    f->last_order = tile.order;
+    return TSL_OK;
+}
+
 static int launch_eval_core(tsl_filter* f, hipStream_t stream, Tile& tile)
 {
     const size_t P = f->count_bound;
     const size_t N = f->num_points;
-    FilterState* st = f->state.as<FilterState>();
-    Tf12 t;
-    for (int k = 0; k < 16; ++k) t.m[k] = f->tf[k];
-    hipLaunchKernelGGL(k_pose, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, stream,
-                       f->particles.as<tsl_particle>(), &st->count, (uint32_t)P, t, f->L.as<float>());
-    TSL_HIP_CHECK(hipGetLastError());
     tile = choose_tile(f, P, N);
+    TSL_TRY(launch_pose_for_tile(f, tile, stream));
     TSL_TRY(f->partial.reserve(sizeof(float) * (size_t)tile.chunks * f->capacity));
     if (f->timing) {
         if (f->timing_pending) {
@@ -429,46 +727,58 @@ static int launch_eval_core(tsl_filter* f, hipStream_t stream, Tile& tile)
 }
 
 // Time every candidate tile shape on the current particles and scan; keep
-// the fastest.  Candidates: K in {1, 2} x {automatic chunks, 1/2/4/8
-// chunks of the scan}.
-int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_k, uint32_t* best_chunk, float* best_ms)
+// the fastest.  Candidates: particle order {caller, spatial} x K in {1, 2}
+// x points per lane in {1, 2, 4} x {automatic chunks, 1/2/4/8 chunks of the
+// scan}.
+int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_order, int* best_k, int* best_p,
+                  uint32_t* best_chunk, float* best_ms)
 {
     const size_t P = f->count_bound;
     const size_t N = f->num_points;
     if (P == 0 || N == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "autotune needs particles and a scan");
-    const int saved_k = f->tile_k;
+    const int saved_k = f->tile_k, saved_p = f->tile_p, saved_order = f->order_mode;
     const uint32_t saved_chunk = f->tile_chunk;
-    TSL_TRY(launch_pose_matrices(f->particles.as<tsl_particle>(), (uint32_t)P, f->tf, f->L.as<float>(), stream));
     hipEvent_t a, b;
     TSL_HIP_CHECK(hipEventCreate(&a));
     TSL_HIP_CHECK(hipEventCreate(&b));
     float best = 1e30f;
-    int bk = 2;
+    int bo = 0, bk = 1, bp = 1;
     uint32_t bc = 0;
     const int ks[2] = {1, 2};
+    const int ps[3] = {1, 2, 4};
     const uint32_t div[5] = {0, 1, 2, 4, 8};
     int rc = TSL_OK;
-    for (int ki = 0; ki < 2 && rc == TSL_OK; ++ki) {
-        for (int ci = 0; ci < 5 && rc == TSL_OK; ++ci) {
-            f->tile_k = ks[ki];
-            f->tile_chunk = div[ci] ? (uint32_t)((N + div[ci] - 1) / div[ci]) : 0u;
-            const Tile tile = choose_tile(f, P, N);
-            rc = f->partial.reserve(sizeof(float) * (size_t)tile.chunks * f->capacity);
-            if (rc != TSL_OK) break;
-            rc = launch_eval_kernel<0>(tile, f, P, N, stream, f->partial.as<float>(), nullptr);  // warm-up
-            if (rc != TSL_OK) break;
-            (void)hipEventRecord(a, stream);
-            for (int r = 0; r < reps && rc == TSL_OK; ++r)
-                rc = launch_eval_kernel<0>(tile, f, P, N, stream, f->partial.as<float>(), nullptr);
-            (void)hipEventRecord(b, stream);
-            (void)hipEventSynchronize(b);
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, a, b);
-            ms /= (float)(reps > 0 ? reps : 1);
-            if (ms < best) {
-                best = ms;
-                bk = ks[ki];
-                bc = f->tile_chunk;
+    for (int order = 0; order < 2 && rc == TSL_OK; ++order) {
+        f->order_mode = order;
+        Tile pt = choose_tile(f, P, N);
+        rc = launch_pose_for_tile(f, pt, stream);
+        for (int ki = 0; ki < 2 && rc == TSL_OK; ++ki) {
+            for (int pi = 0; pi < 3 && rc == TSL_OK; ++pi) {
+                for (int ci = 0; ci < 5 && rc == TSL_OK; ++ci) {
+                    f->tile_k = ks[ki];
+                    f->tile_p = ps[pi];
+                    f->tile_chunk = div[ci] ? (uint32_t)((N + div[ci] - 1) / div[ci]) : 0u;
+                    const Tile tile = choose_tile(f, P, N);
+                    rc = f->partial.reserve(sizeof(float) * (size_t)tile.chunks * f->capacity);
+                    if (rc != TSL_OK) break;
+                    rc = launch_eval_kernel<0>(tile, f, P, N, stream, f->partial.as<float>(), nullptr);  // warm-up
+                    if (rc != TSL_OK) break;
+                    (void)hipEventRecord(a, stream);
+                    for (int r = 0; r < reps && rc == TSL_OK; ++r)
+                        rc = launch_eval_kernel<0>(tile, f, P, N, stream, f->partial.as<float>(), nullptr);
+                    (void)hipEventRecord(b, stream);
+                    (void)hipEventSynchronize(b);
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, a, b);
+                    ms /= (float)(reps > 0 ? reps : 1);
+                    if (ms < best) {
+                        best = ms;
+                        bo = order;
+                        bk = ks[ki];
+                        bp = ps[pi];
+                        bc = f->tile_chunk;
+                    }
+                }
             }
         }
     }
@@ -476,12 +786,18 @@ int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_k, uint
     (void)hipEventDestroy(b);
     if (rc != TSL_OK) {
         f->tile_k = saved_k;
+        f->tile_p = saved_p;
         f->tile_chunk = saved_chunk;
+        f->order_mode = saved_order;
         return rc;
     }
+    f->order_mode = bo;
     f->tile_k = bk;
+    f->tile_p = bp;
     f->tile_chunk = bc;
+    if (best_order) *best_order = bo;
     if (best_k) *best_k = bk;
+    if (best_p) *best_p = bp;
     if (best_chunk) *best_chunk = bc;
     if (best_ms) *best_ms = best;
     return TSL_OK;
@@ -493,29 +809,28 @@ int launch_update(tsl_filter* f, hipStream_t stream)
     if (P == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "filter holds no particles");
     Tile tile;
     TSL_TRY(launch_eval_core(f, stream, tile));
-    FilterState* st = f->state.as<FilterState>();
     const unsigned nb = (unsigned)((P + kBlock - 1) / kBlock);
     TSL_TRY(f->blocks.reserve(sizeof(double) * 9 * (size_t)nb));
-    hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(kBlock), 0, stream, f->partial.as<float>(), tile.chunks,
-                       (uint32_t)f->capacity, f->raw.as<float>(), f->blocks.as<double>(), st);
+    Member* dm;
+    TSL_TRY(filter_member(f, &dm, tile.chunk_len, tile.chunks));
+    hipLaunchKernelGGL(k_reduce_m, dim3(nb), dim3(kBlock), 0, stream, dm);
     TSL_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_normalize, dim3(nb), dim3(kBlock), 0, stream, f->raw.as<float>(),
-                       f->particles.as<tsl_particle>(), f->blocks.as<double>(), st);
+    hipLaunchKernelGGL(k_normalize_m, dim3(nb), dim3(kBlock), 0, stream, dm);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
 }
 
-int launch_raw(tsl_filter* f, hipStream_t stream, float* raw_out_device)
+int launch_raw(tsl_filter* f, hipStream_t stream)
 {
     const size_t P = f->count_bound;
     if (P == 0) return TSL_OK;
     Tile tile;
     TSL_TRY(launch_eval_core(f, stream, tile));
-    FilterState* st = f->state.as<FilterState>();
     const unsigned nb = (unsigned)((P + kBlock - 1) / kBlock);
     TSL_TRY(f->blocks.reserve(sizeof(double) * 9 * (size_t)nb));
-    hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(kBlock), 0, stream, f->partial.as<float>(), tile.chunks,
-                       (uint32_t)f->capacity, raw_out_device, f->blocks.as<double>(), st);
+    Member* dm;
+    TSL_TRY(filter_member(f, &dm, tile.chunk_len, tile.chunks));
+    hipLaunchKernelGGL(k_reduce_m, dim3(nb), dim3(kBlock), 0, stream, dm);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
 }
@@ -545,6 +860,100 @@ int launch_restore(tsl_filter* f, hipStream_t stream)
 int launch_set_count(tsl_filter* f, uint32_t n, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, stream, f->state.as<FilterState>(), n);
+    TSL_HIP_CHECK(hipGetLastError());
+    return TSL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched update: every member's pose matrices, the item plan, one
+// persistent evaluation launch over all members, reduction and
+// normalisation, each a single launch for the whole batch.
+// ---------------------------------------------------------------------------
+namespace {
+
+// stream == nullptr && grid_out: report the persistent grid size only.
+template <int K, int kP, bool kU24, bool kBrick, bool kRank>
+int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid_out)
+{
+    const size_t lds = lds_bytes(b.map->dev);
+    if (grid_out) {
+        int per_cu = 0;
+        TSL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&k_eval_batch<K, kP, kU24, kBrick, kRank>), kBlock, lds));
+        if (per_cu < 1) per_cu = 1;
+        int cus = 0;
+        TSL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b.map->device));
+        uint64_t grid = (uint64_t)per_cu * (uint64_t)cus;
+        const uint64_t need = (b.max_items + kXcds - 1) / kXcds * kXcds;
+        if (grid > need) grid = need;
+        grid = (grid + kXcds - 1) / kXcds * kXcds;
+        if (grid < kXcds) grid = kXcds;
+        *grid_out = (uint32_t)grid;
+        return TSL_OK;
+    }
+    hipLaunchKernelGGL((k_eval_batch<K, kP, kU24, kBrick, kRank>), dim3(b.grid), dim3(kBlock), lds, stream,
+                       b.map->dev, b.members, b.plan, b.a_hit);
+    TSL_HIP_CHECK(hipGetLastError());
+    return TSL_OK;
+}
+
+template <int K, int kP>
+int launch_eval_batch_flags(const BatchLaunch& b, hipStream_t stream, uint32_t* grid_out)
+{
+    const DevMap& d = b.map->dev;
+    const bool u24 = d.up_dim_x < (1u << 23) && d.up_dim_2 < (1u << 23) && d.sub_dim_2 < (1 << 23);
+    const bool brick = d.brick != 0, rank = d.occ_nwords != 0;
+    if (u24) {
+        if (brick) return rank ? launch_eval_batch_t<K, kP, true, true, true>(b, stream, grid_out)
+                               : launch_eval_batch_t<K, kP, true, true, false>(b, stream, grid_out);
+        return rank ? launch_eval_batch_t<K, kP, true, false, true>(b, stream, grid_out)
+                    : launch_eval_batch_t<K, kP, true, false, false>(b, stream, grid_out);
+    }
+    if (brick) return rank ? launch_eval_batch_t<K, kP, false, true, true>(b, stream, grid_out)
+                           : launch_eval_batch_t<K, kP, false, true, false>(b, stream, grid_out);
+    return rank ? launch_eval_batch_t<K, kP, false, false, true>(b, stream, grid_out)
+                : launch_eval_batch_t<K, kP, false, false, false>(b, stream, grid_out);
+}
+
+int dispatch_eval_batch(const BatchLaunch& b, hipStream_t stream, uint32_t* grid_out)
+{
+    switch (b.K * 8 + b.P) {
+    case 9: return launch_eval_batch_flags<1, 1>(b, stream, grid_out);
+    case 10: return launch_eval_batch_flags<1, 2>(b, stream, grid_out);
+    case 12: return launch_eval_batch_flags<1, 4>(b, stream, grid_out);
+    case 17: return launch_eval_batch_flags<2, 1>(b, stream, grid_out);
+    case 18: return launch_eval_batch_flags<2, 2>(b, stream, grid_out);
+    case 20: return launch_eval_batch_flags<2, 4>(b, stream, grid_out);
+    default: return set_error(TSL_ERR_INVALID_ARGUMENT, "batch tile K=%d points per lane=%d unsupported", b.K, b.P);
+    }
+}
+
+}  // namespace
+
+int batch_grid(const BatchLaunch& b, uint32_t* grid) { return dispatch_eval_batch(b, nullptr, grid); }
+
+int launch_batch_restore(const BatchLaunch& b, hipStream_t stream)
+{
+    const unsigned gx = (b.max_cap + 255) / 256 ? (b.max_cap + 255) / 256 : 1;
+    hipLaunchKernelGGL(k_restore_m, dim3(gx, b.n), dim3(256), 0, stream, b.members);
+    TSL_HIP_CHECK(hipGetLastError());
+    return TSL_OK;
+}
+
+int launch_batch_update(const BatchLaunch& b, hipStream_t stream)
+{
+    if (b.n == 0) return TSL_OK;
+    if (b.n > kMaxMembers) return set_error(TSL_ERR_UNSUPPORTED, "at most %u filters per batch", kMaxMembers);
+    const unsigned gx = (b.max_cap + 255) / 256 ? (b.max_cap + 255) / 256 : 1;
+    hipLaunchKernelGGL(k_pose_m, dim3(gx, b.n), dim3(256), 0, stream, b.members);
+    TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(1024), 0, stream, b.members, b.n, (uint32_t)b.K, b.plan);
+    TSL_HIP_CHECK(hipGetLastError());
+    if (b.grid == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "batch grid not sized");
+    TSL_TRY(dispatch_eval_batch(b, stream, nullptr));
+    hipLaunchKernelGGL(k_reduce_m, dim3(gx, b.n), dim3(kBlock), 0, stream, b.members);
+    TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_normalize_m, dim3(gx, b.n), dim3(kBlock), 0, stream, b.members);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
 }

@@ -9,15 +9,20 @@
 //
 // Equivalently output j is particle m(j) = min{m : s_m > U_j} and the number
 // emitted is J = #{j : U_j < s_{M-1}}.  Both sequences are sequential
-// recurrences with rounding, so the kernel
-//   1. computes a candidate for each in parallel (a block scan of the
+// recurrences with rounding, so the kernels
+//   1. compute a candidate for each in parallel (a block scan of the
 //      weights in double; U_j from a per-binade table of constant float
-//      steps built on the host),
-//   2. checks every element against one step of the recurrence from its
+//      steps),
+//   2. check every element against one step of the recurrence from its
 //      predecessor (s_m == s_{m-1} + w_m, U_j == float(U_{j-1} + 1/M)); if all
 //      checks pass, the candidate equals the sequential result by induction,
-//   3. otherwise recomputes serially from the first failing element,
-// then assigns outputs by binary search and gathers the particles.
+//   3. otherwise recompute serially from the first failing element,
+// then assign outputs by binary search and gather the particles.
+//
+// Every kernel is indexed by filter ("member"): the plan (M, the U run
+// table, the number of candidates) is built on the device from the
+// filter's device-side count, so a chain of updates and resamplings needs
+// no host round trip and can be captured in a graph.
 #include "tsl_internal.h"
 
 #include <cmath>
@@ -25,22 +30,118 @@
 
 namespace tsl {
 
-struct USeg {
-    uint32_t j0;   // first index of the run
-    float u;       // U_{j0}
-    double step;   // constant difference inside the run (exact)
-};
-
 namespace {
 
 constexpr int kRsThreads = 1024;
-constexpr int kMaxSeg = 128;
 
-// Passed by value as a kernel argument (2 KB), so no host buffer has to
-// outlive the launch.
-struct USegTable {
-    USeg seg[kMaxSeg];
-};
+__host__ __device__ __forceinline__ float rec_step(float u, double inv_m) { return (float)((double)u + inv_m); }
+
+// Candidates needed for weights summing to at most 1 + 1e-5: the number of
+// U_j below 1 + 1e-5, plus one; 0 if the draw does not advance.  Walks the
+// runs of constant float step without materialising them.
+__host__ __device__ uint64_t cap_walk(uint64_t M, float u0, double inv_m)
+{
+    const double stop = 1.0 + 1e-5;
+    uint64_t j = 0;
+    float u = u0;
+    while ((double)u < stop) {
+        const float n1 = rec_step(u, inv_m);
+        const double d = (double)n1 - (double)u;
+        if (!(d > 0.0)) return 0;
+        uint64_t L = 1;
+        if (u >= 1.1754943508222875e-38f) {
+            int e;
+            (void)frexp((double)u, &e);
+            const double top = ldexp(1.0, e);  // next power of two above u
+            double steps = ceil((top - (double)u) / d) - 1.0;
+            if (steps < 1.0) steps = 1.0;
+            const double lim = ceil((stop - (double)u) / d);
+            if (steps > lim) steps = lim < 1.0 ? 1.0 : lim;
+            L = (uint64_t)steps;
+            const double last = (double)u + (double)(L - 1) * d;
+            if ((double)rec_step((float)last, inv_m) != last + d || (double)rec_step(n1, inv_m) - (double)n1 != d)
+                L = 1;
+        }
+        j += L;
+        u = (float)((double)u + (double)L * d);
+        if (j > 4 * M + 1024) break;
+    }
+    return j + 1;
+}
+
+// Run table of the U recurrence: inside a binade the float step is constant
+// (rounding of U + 1/M depends only on the binade once U is on the float
+// grid), so U_j = U_{j0} + (j - j0)*step exactly.  Runs end before the next
+// power of two; a step that differs from its successor (round-to-even ties,
+// U = 0, subnormals) becomes a run of length 1.  The kernels re-check every
+// element, so the table only needs to be right to be fast, not to be exact.
+__device__ uint32_t build_segments(double inv_m, float u0, uint32_t cap, USeg* seg)
+{
+    uint32_t n = 0, j = 0;
+    float u = u0;
+    while (j < cap && n < (uint32_t)kMaxSeg) {
+        const float n1 = rec_step(u, inv_m);
+        const double d = (double)n1 - (double)u;
+        uint64_t L = 1;
+        if (u >= 1.1754943508222875e-38f && d > 0.0) {
+            int e;
+            (void)frexp((double)u, &e);
+            const double top = ldexp(1.0, e);
+            const double n2 = (double)rec_step(n1, inv_m) - (double)n1;
+            if (n2 == d) {
+                double steps = ceil((top - (double)u) / d) - 1.0;
+                if (steps < 1.0) steps = 1.0;
+                if (steps > (double)(cap - j)) steps = (double)(cap - j);
+                L = (uint64_t)steps;
+                const double last = (double)u + (double)(L - 1) * d;
+                if ((double)rec_step((float)last, inv_m) != last + d) L = 1;
+            }
+        }
+        seg[n].j0 = j;
+        seg[n].u = u;
+        seg[n].step = d;
+        ++n;
+        j += (uint32_t)L;
+        u = (float)((double)u + (double)L * d);
+    }
+    return n;  // candidates beyond the table are repaired serially
+}
+
+// Plan (one thread per member): M from the device count, cap, run table.
+// A member whose update failed (status set) or that holds no particles is
+// skipped (cap = 0) and keeps its particles.
+__global__ void k_rs_plan(Member* __restrict__ members, float u0_arg)
+{
+    if (threadIdx.x != 0) return;
+    Member& m = members[blockIdx.x];
+    FilterState* st = m.st;
+    RsPlan* p = m.plan;
+    const uint32_t M = st->count;
+    const float u0 = m.u0_dev ? *m.u0_dev : u0_arg;
+    st->first_bad = 0xffffffffu;
+    p->M = M;
+    p->u0 = u0;
+    p->cap = 0;
+    p->nseg = 0;
+    if (st->status != kStOk) return;
+    if (M == 0) {
+        st->status = kStEmpty;
+        return;
+    }
+    const double inv_m = 1.0 / (double)M;
+    const uint64_t cap = cap_walk(M, u0, inv_m);
+    if (cap == 0) {
+        st->status = kStNoAdvance;
+        return;
+    }
+    if (cap > m.cap_limit) {
+        st->status = kStCapacity;
+        return;
+    }
+    p->inv_m = inv_m;
+    p->nseg = build_segments(inv_m, u0, (uint32_t)cap, p->seg);
+    p->cap = (uint32_t)cap;
+}
 
 __device__ __forceinline__ double wave_incl_scan(double v, int lane)
 {
@@ -52,12 +153,17 @@ __device__ __forceinline__ double wave_incl_scan(double v, int lane)
     return v;
 }
 
-// Phase A (one workgroup): s_m, the double running sum, candidate by block
-// scan, checked element by element, repaired serially from the first
-// mismatch.
-__global__ __launch_bounds__(kRsThreads) void k_rs_scan(const tsl_particle* __restrict__ in, uint32_t M,
-                                                        double* __restrict__ s)
+// Phase A (one workgroup per member): s_m, the double running sum,
+// candidate by block scan, checked element by element, repaired serially
+// from the first mismatch.
+__global__ __launch_bounds__(kRsThreads) void k_rs_scan(const Member* __restrict__ members)
 {
+    const Member& mb = members[blockIdx.x];
+    const RsPlan* p = mb.plan;
+    if (p->cap == 0) return;
+    const uint32_t M = p->M;
+    const tsl_particle* __restrict__ in = mb.particles;
+    double* __restrict__ s = mb.cumsum;
     __shared__ double wave_tot[kRsThreads / 64];
     __shared__ double carry_sh;
     __shared__ uint32_t first_bad;
@@ -115,50 +221,58 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scan(const tsl_particle* __re
     }
 }
 
-__device__ __forceinline__ float u_candidate(const USegTable& segs, uint32_t nseg, uint32_t j)
+__device__ __forceinline__ float u_candidate(const USeg* __restrict__ seg, uint32_t nseg, uint32_t j)
 {
     uint32_t lo = 0, hi = nseg;  // last run with j0 <= j
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (segs.seg[mid].j0 <= j) lo = mid; else hi = mid;
+        if (seg[mid].j0 <= j) lo = mid; else hi = mid;
     }
-    const USeg sg = segs.seg[lo];
+    const USeg sg = seg[lo];
     return (float)((double)sg.u + (double)(j - sg.j0) * sg.step);
 }
 
-// Phase B (grid): U_j candidates from the run table, each checked against one
-// step of the recurrence from its predecessor.
-__global__ void k_rs_u(const USegTable segs, uint32_t nseg, uint32_t cap, double inv_m, float u0,
-                       float* __restrict__ U, uint32_t* __restrict__ first_bad)
+// Phase B (grid x candidates, y members): U_j candidates from the run
+// table, each checked against one step of the recurrence from its
+// predecessor.
+__global__ void k_rs_u(const Member* __restrict__ members)
 {
+    const Member& mb = members[blockIdx.y];
+    const RsPlan* p = mb.plan;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cap) return;
-    const float u = u_candidate(segs, nseg, j);
-    U[j] = u;
-    const float expect = j == 0 ? u0 : (float)((double)u_candidate(segs, nseg, j - 1) + inv_m);
-    if (u != expect) atomicMin(first_bad, j);
+    if (j >= p->cap) return;
+    const float u = u_candidate(p->seg, p->nseg, j);
+    mb.ubuf[j] = u;
+    const float expect = j == 0 ? p->u0 : (float)((double)u_candidate(p->seg, p->nseg, j - 1) + p->inv_m);
+    if (u != expect) atomicMin(&mb.st->first_bad, j);
 }
 
-// Phase C (one thread): serial repair of U from the first mismatch, then the
-// number emitted J = #{j : U_j < s_{M-1}} (U is non-decreasing).
-__global__ void k_rs_count(uint32_t M, const double* __restrict__ s, float* __restrict__ U, uint32_t cap,
-                           double inv_m, float u0, uint32_t* __restrict__ first_bad, uint32_t* __restrict__ count_out,
-                           uint32_t* __restrict__ status_out)
+// Phase C (one thread per member): serial repair of U from the first
+// mismatch, then the number emitted J = #{j : U_j < s_{M-1}} (U is
+// non-decreasing).
+__global__ void k_rs_count(const Member* __restrict__ members)
 {
     if (threadIdx.x != 0) return;
-    const uint32_t fb = *first_bad;
+    const Member& mb = members[blockIdx.x];
+    RsPlan* p = mb.plan;
+    const uint32_t cap = p->cap;
+    if (cap == 0) return;
+    FilterState* st = mb.st;
+    float* __restrict__ U = mb.ubuf;
+    const double inv_m = p->inv_m;
+    const uint32_t fb = st->first_bad;
     if (fb != 0xffffffffu) {
-        float u = fb == 0 ? u0 : (float)((double)U[fb - 1] + inv_m);
+        float u = fb == 0 ? p->u0 : (float)((double)U[fb - 1] + inv_m);
         for (uint32_t j = fb; j < cap; ++j) {
             U[j] = u;
             u = (float)((double)u + inv_m);
         }
     }
-    *first_bad = 0xffffffffu;
-    const double s_final = s[M - 1];
-    if (cap == 0 || (double)U[cap - 1] < s_final) {
-        *status_out = 2u;
-        *count_out = 0u;
+    st->first_bad = 0xffffffffu;
+    const double s_final = mb.cumsum[p->M - 1];
+    if ((double)U[cap - 1] < s_final) {  // weights sum beyond 1 + 1e-5
+        st->status = kStCapacity;
+        p->cap = 0;
         return;
     }
     uint32_t lo = 0, hi = cap;  // first j with U_j >= s_final
@@ -166,133 +280,75 @@ __global__ void k_rs_count(uint32_t M, const double* __restrict__ s, float* __re
         const uint32_t mid = (lo + hi) >> 1;
         if ((double)U[mid] < s_final) lo = mid + 1; else hi = mid;
     }
-    *status_out = 0u;
-    *count_out = lo;
+    st->resample_count = lo;
 }
 
-// Phase D (grid): output j is particle min{m : s_m > U_j}.
-__global__ void k_rs_assign(const tsl_particle* __restrict__ in, uint32_t M, const double* __restrict__ s,
-                            const float* __restrict__ U, const uint32_t* __restrict__ count,
-                            const uint32_t* __restrict__ status, int32_t* __restrict__ idx_out,
-                            tsl_particle* __restrict__ out)
+// Phase D (grid x outputs, y members): output j is particle
+// min{m : s_m > U_j}, written to alt (and its index to idx_out).
+__global__ void k_rs_assign(const Member* __restrict__ members)
 {
+    const Member& mb = members[blockIdx.y];
+    const RsPlan* p = mb.plan;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (*status != 0u || j >= *count) return;
-    const double u = (double)U[j];
+    if (p->cap == 0 || j >= mb.st->resample_count) return;
+    const uint32_t M = p->M;
+    const double* __restrict__ s = mb.cumsum;
+    const double u = (double)mb.ubuf[j];
     uint32_t lo = 0, hi = M;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (s[mid] > u) hi = mid; else lo = mid + 1;
     }
     const uint32_t m = lo < M ? lo : M - 1;
-    if (idx_out) idx_out[j] = (int32_t)m;
-    if (out) out[j] = in[m];
+    if (mb.idx_out) mb.idx_out[j] = (int32_t)m;
+    mb.alt[j] = mb.particles[m];
 }
 
-float rec_step(float u, double inv_m) { return (float)((double)u + inv_m); }
+// Phase E (grid x outputs, y members): the emitted particles replace the
+// resident set and its count.
+__global__ void k_rs_copy_back(const Member* __restrict__ members)
+{
+    const Member& mb = members[blockIdx.y];
+    if (mb.plan->cap == 0) return;
+    const uint32_t J = mb.st->resample_count;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < J) mb.particles[j] = mb.alt[j];
+    if (blockIdx.x == 0 && threadIdx.x == 0) mb.st->count = J;
+}
 
 }  // namespace
 
-// Run table of the U recurrence: inside a binade the float step is constant
-// (rounding of U + 1/M depends only on the binade once U is on the float
-// grid), so U_j = U_{j0} + (j - j0)*step exactly.  Runs end before the next
-// power of two; a step that differs from its successor (round-to-even ties,
-// U = 0, subnormals) becomes a run of length 1.  The kernel re-checks every
-// element, so this table only needs to be right to be fast, not to be exact.
-static int build_segments(size_t M, float u0, uint32_t cap, std::vector<USeg>& segs)
-{
-    const double inv_m = 1.0 / (double)M;
-    segs.clear();
-    uint32_t j = 0;
-    float u = u0;
-    while (j < cap) {
-        if ((int)segs.size() >= kMaxSeg) break;  // remainder repaired serially on the device
-        const float n1 = rec_step(u, inv_m);
-        const double d = (double)n1 - (double)u;
-        if (!(d > 0.0)) return set_error(TSL_ERR_UNSUPPORTED, "systematic draw does not advance (M=%zu)", M);
-        uint64_t L = 1;
-        if (u >= 1.1754943508222875e-38f) {
-            int e;
-            std::frexp((double)u, &e);
-            const double top = std::ldexp(1.0, e);  // next power of two above u
-            const double n2 = (double)rec_step(n1, inv_m) - (double)n1;
-            if (n2 == d) {
-                double steps = std::ceil((top - (double)u) / d) - 1.0;
-                if (steps < 1.0) steps = 1.0;
-                if (steps > (double)(cap - j)) steps = (double)(cap - j);
-                L = (uint64_t)steps;
-                const double last = (double)u + (double)(L - 1) * d;
-                if ((double)rec_step((float)last, inv_m) != last + d) L = 1;
-            }
-        }
-        segs.push_back(USeg{j, u, d});
-        j += (uint32_t)L;
-        u = (float)((double)u + (double)L * d);
-    }
-    return TSL_OK;
-}
-
-// Number of U values below 1 + 1e-5, plus one: an upper bound on J for
-// normalised weights (their double sum is within ~1e-7 of 1).
 int resample_capacity(size_t M, float u0, size_t* cap_out)
 {
     if (M == 0) {
         *cap_out = 0;
         return TSL_OK;
     }
-    const double inv_m = 1.0 / (double)M;
-    const double stop = 1.0 + 1e-5;
-    // walk the runs without materialising them
-    size_t j = 0;
-    float u = u0;
-    while ((double)u < stop) {
-        const float n1 = rec_step(u, inv_m);
-        const double d = (double)n1 - (double)u;
-        if (!(d > 0.0)) return set_error(TSL_ERR_UNSUPPORTED, "systematic draw does not advance (M=%zu)", M);
-        uint64_t L = 1;
-        if (u >= 1.1754943508222875e-38f) {
-            int e;
-            std::frexp((double)u, &e);
-            const double top = std::ldexp(1.0, e);
-            double steps = std::ceil((top - (double)u) / d) - 1.0;
-            if (steps < 1.0) steps = 1.0;
-            const double lim = std::ceil((stop - (double)u) / d);
-            if (steps > lim) steps = lim < 1.0 ? 1.0 : lim;
-            L = (uint64_t)steps;
-            const double last = (double)u + (double)(L - 1) * d;
-            if ((double)rec_step((float)last, inv_m) != last + d || (double)rec_step(n1, inv_m) - (double)n1 != d) L = 1;
-        }
-        j += L;
-        u = (float)((double)u + (double)L * d);
-        if (j > 4 * M + 1024) break;
-    }
-    *cap_out = j + 1;
+    const uint64_t cap = cap_walk(M, u0, 1.0 / (double)M);
+    if (cap == 0) return set_error(TSL_ERR_UNSUPPORTED, "systematic draw does not advance (M=%zu)", M);
+    *cap_out = cap;
     return TSL_OK;
 }
 
-int launch_resample_buffers(const tsl_particle* d_in, size_t M, float u0, size_t cap, double* d_s,
-                            float* d_u, int32_t* d_idx, tsl_particle* d_out, uint32_t* d_count,
-                            uint32_t* d_status, uint32_t* d_first_bad, hipStream_t stream)
+int launch_resample_members(Member* d_members, uint32_t n, uint32_t grid_cap, float u0_arg, bool copy_back,
+                            hipStream_t stream)
 {
-    if (M == 0 || M > 0x7fffffffu || cap > 0xffffffffu)
-        return set_error(TSL_ERR_INVALID_ARGUMENT, "resample: M=%zu out of range", M);
-    std::vector<USeg> segs;
-    TSL_TRY(build_segments(M, u0, (uint32_t)cap, segs));
-    USegTable table{};
-    for (size_t k = 0; k < segs.size(); ++k) table.seg[k] = segs[k];
-    const double inv_m = 1.0 / (double)M;
-    const unsigned gc = (unsigned)((cap + 255) / 256);
-    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(kRsThreads), 0, stream, d_in, (uint32_t)M, d_s);
+    if (n == 0) return TSL_OK;
+    const unsigned gx = (grid_cap + 255) / 256 ? (grid_cap + 255) / 256 : 1;
+    hipLaunchKernelGGL(k_rs_plan, dim3(n), dim3(64), 0, stream, d_members, u0_arg);
     TSL_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_rs_u, dim3(gc ? gc : 1), dim3(256), 0, stream, table, (uint32_t)segs.size(), (uint32_t)cap,
-                       inv_m, u0, d_u, d_first_bad);
+    hipLaunchKernelGGL(k_rs_scan, dim3(n), dim3(kRsThreads), 0, stream, d_members);
     TSL_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_rs_count, dim3(1), dim3(64), 0, stream, (uint32_t)M, d_s, d_u, (uint32_t)cap, inv_m, u0,
-                       d_first_bad, d_count, d_status);
+    hipLaunchKernelGGL(k_rs_u, dim3(gx, n), dim3(256), 0, stream, d_members);
     TSL_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_rs_assign, dim3(gc ? gc : 1), dim3(256), 0, stream, d_in, (uint32_t)M, d_s, d_u, d_count,
-                       d_status, d_idx, d_out);
+    hipLaunchKernelGGL(k_rs_count, dim3(n), dim3(64), 0, stream, d_members);
     TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_rs_assign, dim3(gx, n), dim3(256), 0, stream, d_members);
+    TSL_HIP_CHECK(hipGetLastError());
+    if (copy_back) {
+        hipLaunchKernelGGL(k_rs_copy_back, dim3(gx, n), dim3(256), 0, stream, d_members);
+        TSL_HIP_CHECK(hipGetLastError());
+    }
     return TSL_OK;
 }
 

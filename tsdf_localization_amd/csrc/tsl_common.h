@@ -1,10 +1,14 @@
 // Arithmetic shared by host and device code of the MI355X sensor update.
 //
-// Everything here must give bit-identical results on the host (x86-64, gcc
-// or clang) and on gfx950: IEEE float/double +,-,*,/ only, no FMA contraction
-// (all sources are compiled with -ffp-contract=off), no library math calls.
+// Everything on the sensor-update path here gives bit-identical results on
+// the host (x86-64, gcc or clang) and on gfx950: IEEE float/double +,-,*,/
+// only, no FMA contraction (all sources are compiled with -ffp-contract=off),
+// no library math calls.  The motion model at the end is the exception: its
+// double sin, cos, asin, atan2, log and sqrt come from each side's math
+// library, so host and device agree to within a few float ulps.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -139,6 +143,104 @@ TSL_HD void pose_matrix(const float* pose, const float* tf, float* L)
         L[4 * r + 2] = p[4 * r] * tf[2] + p[4 * r + 1] * tf[6] + p[4 * r + 2] * tf[10];
         L[4 * r + 3] = p[4 * r] * tf[3] + p[4 * r + 1] * tf[7] + p[4 * r + 2] * tf[11] + p[4 * r + 3];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Motion noise: counter-based draws, so that particle i of a filter gets the
+// same noise whatever else runs with it.  SplitMix64 (Steele, Lea, Flood,
+// "Fast splittable pseudorandom number generators", OOPSLA 2014) mixes
+// (seed, particle, draw, half) into 64 bits; 53 of them give a uniform in
+// (0, 1); Box-Muller (cosine branch) turns two uniforms into one standard
+// normal.
+// ---------------------------------------------------------------------------
+TSL_HD uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+TSL_HD double motion_uniform(uint64_t seed, uint64_t particle, uint32_t draw, uint32_t half)
+{
+    const uint64_t h = splitmix64(splitmix64(seed ^ splitmix64(particle)) + (uint64_t)(2u * draw + half));
+    return ((double)(h >> 11) + 0.5) * 0x1.0p-53;
+}
+
+TSL_HD double motion_normal(uint64_t seed, uint64_t particle, uint32_t draw)
+{
+    const double u = motion_uniform(seed, particle, draw, 0);
+    const double v = motion_uniform(seed, particle, draw, 1);
+    return sqrt(-2.0 * log(u)) * cos(6.283185307179586 * v);
+}
+
+// ParticleCloud::apply_model (src/particle_cloud.cpp:499-617) for one
+// particle given its six draws d = (dx, dy, dz, roll, pitch, yaw) (double,
+// as std::normal_distribution<> returns them): the noisy odometry step is
+// rotated into the particle's frame (translation added, rotation composed)
+// and the new angles are read back as getAngleFromMat does
+// (src/util/util.cpp:86-111).  Float arithmetic left to right, no FMA.
+TSL_HD void apply_motion(const float last[6], const double d[6], float out[6])
+{
+    const float sin_roll = (float)sin(d[3]), cos_roll = (float)cos(d[3]);
+    const float sin_pitch = (float)sin(d[4]), cos_pitch = (float)cos(d[4]);
+    const float sin_yaw = (float)sin(d[5]), cos_yaw = (float)cos(d[5]);
+    float o[12];
+    o[0] = cos_pitch * cos_yaw;
+    o[4] = cos_pitch * sin_yaw;
+    o[8] = -sin_pitch;
+    o[3] = (float)d[0];
+    o[1] = sin_roll * sin_pitch * cos_yaw - cos_roll * sin_yaw;
+    o[5] = sin_roll * sin_pitch * sin_yaw + cos_roll * cos_yaw;
+    o[9] = sin_roll * cos_pitch;
+    o[7] = (float)d[1];
+    o[2] = cos_roll * sin_pitch * cos_yaw + sin_roll * sin_yaw;
+    o[6] = cos_roll * sin_pitch * sin_yaw - sin_roll * cos_yaw;
+    o[10] = cos_roll * cos_pitch;
+    o[11] = (float)d[2];
+    float sa, ca, sb, cb, sg, cg;
+    sincosf_exact(last[3], &sa, &ca);
+    sincosf_exact(last[4], &sb, &cb);
+    sincosf_exact(last[5], &sg, &cg);
+    float p[12];
+    p[0] = cb * cg;
+    p[4] = cb * sg;
+    p[8] = -sb;
+    p[3] = 0.0f;
+    p[1] = sa * sb * cg - ca * sg;
+    p[5] = sa * sb * sg + ca * cg;
+    p[9] = sa * cb;
+    p[7] = 0.0f;
+    p[2] = ca * sb * cg + sa * sg;
+    p[6] = ca * sb * sg - sa * cg;
+    p[10] = ca * cb;
+    p[11] = 0.0f;
+    float t[12];
+    for (int r = 0; r < 3; ++r) {
+        t[4 * r + 0] = p[4 * r] * o[0] + p[4 * r + 1] * o[4] + p[4 * r + 2] * o[8];
+        t[4 * r + 1] = p[4 * r] * o[1] + p[4 * r + 1] * o[5] + p[4 * r + 2] * o[9];
+        t[4 * r + 2] = p[4 * r] * o[2] + p[4 * r + 1] * o[6] + p[4 * r + 2] * o[10];
+        t[4 * r + 3] = p[4 * r] * o[3] + p[4 * r + 1] * o[7] + p[4 * r + 2] * o[11] + p[4 * r + 3];
+    }
+    // getAngleFromMat on the 3x4 part (entries 0, 4, 8, 9, 10 of the 4x4)
+    float roll, pitch, yaw;
+    if (fabs((double)t[8]) >= 1.0) {
+        yaw = 0.0f;
+        const double delta = atan2((double)t[9], (double)t[10]);
+        pitch = t[8] < 0.0f ? (float)(3.141592653589793 / 2.0) : (float)(-3.141592653589793 / 2.0);
+        roll = (float)delta;
+    } else {
+        pitch = (float)(-asin((double)t[8]));
+        const double cpd = cos((double)pitch);
+        roll = (float)atan2((double)t[9] / cpd, (double)t[10] / cpd);
+        yaw = (float)atan2((double)t[4] / cpd, (double)t[0] / cpd);
+    }
+    out[0] = last[0] + t[3];
+    out[1] = last[1] + t[7];
+    out[2] = last[2] + t[11];
+    out[3] = roll;
+    out[4] = pitch;
+    out[5] = yaw;
 }
 
 }  // namespace tsl

@@ -15,6 +15,7 @@ namespace tsl {
 constexpr uint32_t kMaxPalette = 4096;  // 16-bit voxel codes index at most this many values
 constexpr uint32_t kFastLimit = 8388608u;  // 2^23: offsets below this take the 32-bit lookup
 constexpr uint32_t kMaxSubTable = 1024;    // buckets of the sub-voxel threshold table
+constexpr uint32_t kMaxOccWords = 2048;    // occupancy-rank words kept in LDS (131072 blocks, 32 KB)
 
 // ---------------------------------------------------------------------------
 // Error state
@@ -86,6 +87,17 @@ struct SubTable {
     std::vector<SubEntry> entries;
 };
 
+// One 64-block word of the occupancy bitmap with the number of allocated
+// blocks before it.  When block offsets follow the reference's setData
+// order (offset of the n-th allocated block in linear order = n * sub_dim^3,
+// cuda_sub_voxel_map.tcc:202-211), grid_occ[up] == rank(up) * sub_dim^3 for
+// allocated blocks, and rank(up) = prefix + popcount(bits below up).
+struct OccWord {
+    uint32_t lo, hi;   // bits of blocks 64w .. 64w+63
+    uint32_t prefix;   // allocated blocks in words 0 .. w-1
+    uint32_t pad;
+};
+
 // Every field the exact tiers of the lookup need (64-bit reference sizes,
 // the wrap-band constants).  Lives in device memory; the kernels read it
 // only when a wavefront leaves the fast tier.
@@ -127,6 +139,13 @@ struct DevMap {
     uint32_t sd3;                     // sub_dim^3
     uint32_t brick_cy, brick_cz;      // 64B - 16, 64B^2 - 64
     uint32_t mag_sd, sh_sd, mag_sd2, sh_sd2, mag_sd3, sh_sd3;  // x / d = umulhi(x, mag) >> sh, x < 2^31
+    // occupancy-rank form of the block table (set when the offsets follow
+    // setData order and the bitmap fits the LDS budget)
+    const OccWord* occ_words;
+    uint32_t occ_nwords;              // 0: use grid_occ
+    uint32_t sentinel;                // element of codes holding the code of init_value
+    uint32_t sentinel_code;           // that code
+    uint32_t pad1;
 };
 
 // Smallest u in [0, 2^23] with trunc(fl(fl(u)/res)) >= dim; host IEEE division.
@@ -138,6 +157,83 @@ void magic_divisor(uint32_t d, uint32_t* mag, uint32_t* sh);
 uint32_t sub_lookup_host(const SubTable& t, float a);
 void negative_band_constants(const tsl_map_coef& c, DevMapFull& d);
 
+// Device-side bookkeeping of a filter (one per filter, in device memory).
+// status values
+constexpr uint32_t kStOk = 0, kStNoValid = 1, kStCapacity = 2, kStEmpty = 3, kStNoAdvance = 4;
+
+struct FilterState {
+    uint32_t count;        // particles in the resident set
+    uint32_t status;       // kSt*
+    uint32_t counter0;     // last-block-done counters, reset by the last block
+    uint32_t counter1;
+    uint32_t resample_count;
+    uint32_t first_bad;    // resampling: first U candidate that failed its check
+    uint32_t pad[2];
+    double stats[10];      // sum, Σwx, Σwy, Σwz, Σw sin/cos roll, pitch, yaw
+};
+
+// Systematic-draw plan of one filter, built on the device from its count
+// and u0: U_j run table (see resample.hip) and the number of candidates.
+constexpr int kMaxSeg = 128;
+struct USeg {
+    uint32_t j0;   // first index of the run
+    float u;       // U_{j0}
+    double step;   // constant difference inside the run (exact)
+};
+struct RsPlan {
+    uint32_t M;     // particles resampled
+    uint32_t cap;   // candidates U_0 .. U_{cap-1}; 0 = skip this resampling
+    uint32_t nseg;
+    float u0;
+    double inv_m;
+    USeg seg[kMaxSeg];
+};
+
+// Device descriptor of one filter for the member-indexed kernels (a single
+// filter is a batch of one).
+struct Member {
+    tsl_particle* particles;   // resident set (in/out)
+    tsl_particle* alt;         // resampling output before it is copied back
+    float* L;                  // pose matrices
+    const float4* points;      // prepared scan
+    float* partial;            // chunk partial weights [chunks][stride]
+    float* raw;                // raw weights
+    const uint32_t* inv;       // slot of particle i in the evaluation order (NULL: identity)
+    double* blocks;            // reduction block partials
+    FilterState* st;
+    double* cumsum;            // resampling running sum
+    float* ubuf;               // resampling U candidates
+    int32_t* idx_out;          // resampled source indices (optional)
+    RsPlan* plan;
+    const float* u0_dev;       // uniform draw in device memory (or NULL: kernel argument)
+    const tsl_particle* snapshot;  // restore source (batched benchmarks)
+    float tf[16];              // scanner -> base
+    uint32_t N, chunk_len, chunks, stride;
+    uint32_t cap_limit;        // room of particles/alt/ubuf for resampling output
+    uint32_t snapshot_count;
+    uint32_t pad[2];
+};
+
+constexpr uint32_t kMaxMembers = 1024;  // filters in one batch
+
+// Item table of a batched evaluation, built on the device each step.
+struct BatchPlan {
+    uint32_t n, total, pad[2];
+    uint32_t prefix[kMaxMembers + 1];
+};
+
+// Upper bound on the resampling output for M particles whose weights sum
+// to at most 1 + 1e-5: U_j >= j (1/M - 2^-24) for every u0 (each float step
+// loses at most half an ulp below 2), so at most (1 + 1e-5) / (1/M - 2^-24)
+// + 2 candidates are needed.  0 if M >= 2^24 (the draw may not advance).
+inline uint64_t resample_bound(uint64_t M)
+{
+    if (M == 0) return 0;
+    const double d = 1.0 / (double)M - 5.9604644775390625e-08;
+    if (!(d > 0.0)) return 0;
+    return (uint64_t)((1.0 + 1e-5) / d) + 3;  // one more for the rounding of this quotient
+}
+
 }  // namespace tsl
 
 // ---------------------------------------------------------------------------
@@ -146,7 +242,7 @@ void negative_band_constants(const tsl_map_coef& c, DevMapFull& d);
 struct tsl_map {
     int device = 0;
     tsl_map_coef coef{};
-    tsl::DevBuf grid_occ, codes, palette, sub_tab, neg_tab, full;
+    tsl::DevBuf grid_occ, codes, palette, sub_tab, neg_tab, full, occ_words;
     tsl::DevMap dev{};
 };
 
@@ -175,6 +271,7 @@ struct tsl_filter {
     uint64_t eval_pairs = 0;  // particle*point evaluations launched
     bool timing_pending = false;
     int tile_k = 0;            // particles per wavefront (0 = automatic)
+    int tile_p = 0;            // points per lane per iteration (0 = automatic)
     uint32_t tile_chunk = 0;   // scan points per wavefront (0 = automatic)
     size_t num_points = 0;
     float tf[16] = {};
@@ -187,47 +284,83 @@ struct tsl_filter {
     tsl::DevBuf points_in;   // tsl_point staging (caller order)
     tsl::DevBuf perm;        // uint32[num_points]: caller index of sorted point j
     tsl::DevBuf sort_keys, sort_vals, sort_temp;
+    tsl::DevBuf pkeys, pvals, ptemp;  // particle-order sort
+    tsl::DevBuf porder;      // uint32: particle evaluated in slot s
+    tsl::DevBuf pinv;        // uint32: slot of particle i
+    int order_mode = -1;     // particle evaluation order: 0 caller, 1 spatial, -1 automatic
+    int last_order = 0;      // order of the last evaluation (partial weights by slot)
     tsl::DevBuf blocks;      // double block partials
     tsl::DevBuf state;       // FilterState
     tsl::DevBuf cumsum;      // double[capacity]
     tsl::DevBuf ubuf;        // float U table
+    tsl::DevBuf plan;        // RsPlan
+    tsl::DevBuf motion;      // tsl_motion of the pending motion update
+    void* motion_host = nullptr;    // pinned staging
+    void* motion_copied = nullptr;  // hipEvent_t: staging free again
+    bool motion_pending = false;
+    tsl::DevBuf member;      // Member (device descriptor of this filter)
+    tsl::Member member_host{};  // last uploaded descriptor
+    bool member_dirty = true;
     void* host_state = nullptr;  // pinned FilterState mirror
 };
 
 namespace tsl {
 
-// Device-side bookkeeping of a filter (one per filter, in device memory).
-struct FilterState {
-    uint32_t count;        // particles in the resident set
-    uint32_t status;       // 0 ok, 1 zero weight sum, 2 resample capacity exceeded
-    uint32_t counter0;     // last-block-done counters, reset by the last block
-    uint32_t counter1;
-    uint32_t resample_count;
-    uint32_t first_bad;    // resampling: first U candidate that failed its check
-    uint32_t pad[2];
-    double stats[10];      // sum, Σwx, Σwy, Σwz, Σw sin/cos roll, pitch, yaw
-};
-
 // Kernel launchers (eval.hip)
 int launch_update(tsl_filter* f, hipStream_t stream);
-int launch_raw(tsl_filter* f, hipStream_t stream, float* raw_out_device);
+int launch_raw(tsl_filter* f, hipStream_t stream);
 int launch_trace(tsl_filter* f, hipStream_t stream, void* out_device, int values);
 int launch_restore(tsl_filter* f, hipStream_t stream);
-int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_k, uint32_t* best_chunk, float* best_ms);
+int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_order, int* best_k, int* best_p,
+                  uint32_t* best_chunk, float* best_ms);
 
 struct Tile {
-    int K;
+    int K;               // particles per wavefront
+    int P;               // points per lane per iteration
+    int order;           // 1: particles evaluated in spatial order
     uint32_t chunk_len;
     uint32_t chunks;
 };
 int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const float tf[16],
                          float* d_out, hipStream_t stream);
 int launch_prep_points(tsl_filter* f, hipStream_t stream);
+int launch_particle_order(tsl_filter* f, hipStream_t stream);
 
-// Resampling (resample.hip)
-struct USegments;
+// Resampling (resample.hip): members[0..n) resampled in one sequence of
+// launches; grid_cap = max cap_limit over the members.
 int resample_capacity(size_t M, float u0, size_t* cap);
-int launch_resample(tsl_filter* f, float u0, hipStream_t stream);
+int launch_resample_members(Member* d_members, uint32_t n, uint32_t grid_cap, float u0_arg, bool copy_back,
+                            hipStream_t stream);
+// Pose estimate from the ten weighted sums of a FilterState (api.hip).
+void pose_from_stats(const double* st, tsl_pose* pose);
+// Map a FilterState status to the library's error (TSL_OK if none).
+int status_error(uint32_t status);
+
+// Batched update (eval.hip)
+struct BatchLaunch {
+    const tsl_map* map = nullptr;
+    float a_hit = 0.9f;
+    Member* members = nullptr;   // device array [n]
+    BatchPlan* plan = nullptr;   // device
+    uint32_t n = 0;
+    int K = 1, P = 1;
+    uint64_t max_items = 0;      // upper bound of the evaluation items over all members
+    uint32_t max_cap = 0;        // largest member capacity (grid bound of the per-particle kernels)
+    uint32_t max_rs = 0;         // largest resampling output room
+    uint32_t grid = 0;           // persistent evaluation grid (multiple of the XCD count)
+};
+int launch_batch_update(const BatchLaunch& b, hipStream_t stream);
+int batch_grid(const BatchLaunch& b, uint32_t* grid);
+int launch_batch_restore(const BatchLaunch& b, hipStream_t stream);
+Tile choose_tile(const tsl_filter* f, size_t P, size_t N);
+
+// Upload the filter's device descriptor if it changed; chunk_len/chunks
+// describe the partial-weight layout of the last evaluation (0 = keep).
+int filter_member(tsl_filter* f, Member** d_member, uint32_t chunk_len = 0, uint32_t chunks = 0);
+
+// Motion update of members[0..n) with motions[0..n) (motion.hip).
+int launch_motion_members(const Member* d_members, const tsl_motion* d_motions, uint32_t n, uint32_t grid_cap,
+                          hipStream_t stream);
 
 // Filter buffer management (api.hip)
 int filter_reserve_particles(tsl_filter* f, size_t n);

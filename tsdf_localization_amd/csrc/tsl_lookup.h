@@ -77,6 +77,28 @@ __device__ __forceinline__ int32_t mul_i(int32_t a, int32_t b)
     return a * b;
 }
 
+// Block offset of block up (< grid_occ_size) from the LDS copy of the
+// occupancy-rank table: rank * sub_dim^3 if allocated, else -1.  Equals
+// grid_occ[up] for maps whose offsets follow setData order.
+__device__ __forceinline__ int32_t rank_offset(const OccWord* __restrict__ occ, uint32_t up, uint32_t sd3)
+{
+    const OccWord w = occ[up >> 6];
+    const uint32_t r = up & 63u;
+    const uint32_t word = r < 32u ? w.lo : w.hi;
+    const bool set = ((word >> (r & 31u)) & 1u) != 0u;
+    const uint32_t m_lo = r < 32u ? (1u << r) - 1u : 0xffffffffu;
+    const uint32_t m_hi = r < 32u ? 0u : (1u << (r - 32u)) - 1u;
+    const uint32_t below = (uint32_t)__popc(w.lo & m_lo) + (uint32_t)__popc(w.hi & m_hi);
+    return set ? (int32_t)((w.prefix + below) * sd3) : -1;
+}
+
+template <bool kRank>
+__device__ __forceinline__ int32_t block_offset(const DevMap& m, const OccWord* __restrict__ occ, uint32_t up)
+{
+    if (kRank) return rank_offset(occ, up, m.sd3);
+    return m.grid_occ[up];
+}
+
 // Fast tier: 0 <= o < R on every axis (one unsigned compare of the bit
 // pattern per axis, since floats >= +0 order like their bits).
 __device__ __forceinline__ bool fast_ok(float ox, float oy, float oz, uint32_t range_bits)
@@ -129,15 +151,16 @@ __device__ __forceinline__ uint32_t phys_from_linear(const DevMap& m, uint32_t x
 // reference index, and (bricked layout) the in-block physical offset, with
 // carry set when an in-block coordinate equals sub_dim (the division rounded
 // up to the block edge; the reference index then lands in the next row).
-template <bool kU24, bool kBrick>
-__device__ __forceinline__ void fast_block(const DevMap& m, const SubEntry* __restrict__ tab, float ox, float oy,
-                                           float oz, int32_t& s, uint32_t& small, uint32_t& pbrick, bool& carry)
+template <bool kU24, bool kBrick, bool kRank>
+__device__ __forceinline__ void fast_block(const DevMap& m, const SubEntry* __restrict__ tab,
+                                           const OccWord* __restrict__ occ, float ox, float oy, float oz, int32_t& s,
+                                           uint32_t& small, uint32_t& pbrick, bool& carry)
 {
     const FastAxis ax = fast_axis(m, tab, ox), ay = fast_axis(m, tab, oy), az = fast_axis(m, tab, oz);
     const bool grid = (ax.i < m.gthr_x) & (ay.i < m.gthr_y) & (az.i < m.gthr_z);
     const uint32_t up = ax.i + mul_u<kU24>(ay.i, m.up_dim_x) + mul_u<kU24>(az.i, m.up_dim_2);
     s = -1;
-    if (grid & (up < m.occ)) s = m.grid_occ[up];
+    if (grid & (up < m.occ)) s = block_offset<kRank>(m, occ, up);
     small = ax.k + mul_u<kU24>(ay.k, (uint32_t)m.sub_dim) + mul_u<kU24>(az.k, (uint32_t)m.sub_dim_2);
     if (kBrick) {
         pbrick = brick_offset(m, ax.k, ay.k, az.k);
@@ -150,9 +173,10 @@ __device__ __forceinline__ void fast_block(const DevMap& m, const SubEntry* __re
 
 // Wrap tier for one lane (|o| < R on every axis): the reference's index,
 // data_size when rejected.  neg is the LDS copy of the wrap-band table.
-template <bool kU24>
+template <bool kU24, bool kRank>
 __device__ __forceinline__ uint64_t wrap_index(const DevMap& m, const SubEntry* __restrict__ tab,
-                                               const uint64_t* __restrict__ neg, float ox, float oy, float oz)
+                                               const uint64_t* __restrict__ neg, const OccWord* __restrict__ occ,
+                                               float ox, float oy, float oz)
 {
     const int32_t ix = (int32_t)ox, iy = (int32_t)oy, iz = (int32_t)oz;
     const bool nx = ox <= -1.0f, ny = oy <= -1.0f, nz = oz <= -1.0f;
@@ -161,7 +185,7 @@ __device__ __forceinline__ uint64_t wrap_index(const DevMap& m, const SubEntry* 
                       (((uint32_t)iz < m.gthr_z) | (nz & ((m.gneg & 4u) != 0u)));
     const int32_t up = ix + mul_i<kU24>(iy, (int32_t)m.up_dim_x) + mul_i<kU24>(iz, (int32_t)m.up_dim_2);
     int32_t s = -1;
-    if (grid & ((uint32_t)up < m.occ)) s = m.grid_occ[up];
+    if (grid & ((uint32_t)up < m.occ)) s = block_offset<kRank>(m, occ, (uint32_t)up);
     const float fx = ox - (float)ix, fy = oy - (float)iy, fz = oz - (float)iz;
     int32_t ex = (int32_t)sub_index(m.sub_base, m.sub_shift, tab, fabsf(fx));
     int32_t ey = (int32_t)sub_index(m.sub_base, m.sub_shift, tab, fabsf(fy));

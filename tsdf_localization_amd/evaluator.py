@@ -16,7 +16,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import MapCoef, Pose, check, lib, ptr
+from ._lib import MapCoef, Motion, Pose, check, lib, ptr
 
 
 def _particles(a):
@@ -336,24 +336,36 @@ class DeviceFilter:
     def resample_async(self, u0, stream=None):
         check(lib.tsl_filter_resample_async(self.handle, float(u0), None if stream is None else stream.handle))
 
+    def motion_async(self, motion, stream=None):
+        """ParticleCloud::apply_model on the device (motion: a tsl_motion)."""
+        check(lib.tsl_filter_motion_async(self.handle, ctypes.byref(motion), _stream(stream)))
+
     def snapshot(self):
         check(lib.tsl_filter_snapshot(self.handle))
 
     def restore_async(self, stream=None):
         check(lib.tsl_filter_restore_async(self.handle, None if stream is None else stream.handle))
 
-    def set_tile(self, k=0, chunk_len=0):
-        check(lib.tsl_filter_set_tile(self.handle, int(k), int(chunk_len)))
+    def set_tile(self, k=0, chunk_len=0, points_per_lane=0):
+        check(lib.tsl_filter_set_tile(self.handle, int(k), int(points_per_lane), int(chunk_len)))
+
+    def set_order(self, mode=-1):
+        """Particle evaluation order: 0 caller, 1 spatial, -1 automatic."""
+        check(lib.tsl_filter_set_order(self.handle, int(mode)))
 
     def autotune(self, stream=None, reps=3):
-        """Pick the fastest tile shape on the current particles and scan;
-        returns (K, chunk_len (0 = automatic), ms per evaluation)."""
+        """Pick the fastest evaluation shape on the current particles and
+        scan; returns (order, K, points per lane, chunk_len (0 = automatic),
+        ms per evaluation)."""
+        o = ctypes.c_int()
         k = ctypes.c_int()
+        pp = ctypes.c_int()
         c = ctypes.c_uint32()
         ms = ctypes.c_float()
         check(lib.tsl_filter_autotune(self.handle, None if stream is None else stream.handle, int(reps),
-                                      ctypes.byref(k), ctypes.byref(c), ctypes.byref(ms)))
-        return k.value, c.value, ms.value
+                                      ctypes.byref(o), ctypes.byref(k), ctypes.byref(pp), ctypes.byref(c),
+                                      ctypes.byref(ms)))
+        return o.value, k.value, pp.value, c.value, ms.value
 
     def set_timing(self, enable=True):
         check(lib.tsl_filter_set_timing(self.handle, int(bool(enable))))
@@ -377,3 +389,108 @@ class DeviceFilter:
         if getattr(self, "handle", None):
             lib.tsl_filter_destroy(self.handle)
             self.handle = None
+
+
+def _stream(stream):
+    return None if stream is None else stream.handle
+
+
+DEFAULT_MOTION_PARAMS = (0.1,) * 12  # a_1 .. a_12 (particle_cloud.h:60-71)
+
+
+def motion_from_delta(dx, dy, dz, roll, pitch, yaw, lin_base_noise=0.0, ang_base_noise=0.0,
+                      a=DEFAULT_MOTION_PARAMS, seed=0):
+    """Distributions of ParticleCloud::motion_update (particle_cloud.cpp:456-497)."""
+    aa = np.ascontiguousarray(a, np.float32)
+    m = Motion()
+    check(lib.tsl_motion_from_delta(float(dx), float(dy), float(dz), float(roll), float(pitch), float(yaw),
+                                    float(lin_base_noise), float(ang_base_noise), ptr(aa), int(seed), ctypes.byref(m)))
+    return m
+
+
+def motion_from_odom(linear_velocity, angular_velocity, time_diff, a=DEFAULT_MOTION_PARAMS, seed=0):
+    """Distributions of ParticleCloud::motionUpdate(odom) (particle_cloud.cpp:153-207)."""
+    aa = np.ascontiguousarray(a, np.float32)
+    m = Motion()
+    check(lib.tsl_motion_from_odom(float(linear_velocity), float(angular_velocity), float(time_diff), ptr(aa),
+                                   int(seed), ctypes.byref(m)))
+    return m
+
+
+def motion(mean, stddev, seed=0):
+    """A tsl_motion from explicit means and standard deviations."""
+    m = Motion()
+    for k in range(6):
+        m.mean[k] = float(mean[k])
+        m.stddev[k] = float(stddev[k])
+    m.seed = int(seed)
+    return m
+
+
+class Batch:
+    """Many DeviceFilters sharing one evaluator, updated and resampled by one
+    fixed sequence of launches (optionally a captured HIP graph)."""
+
+    def __init__(self, evaluator, filters):
+        self.evaluator = evaluator
+        self.filters = list(filters)
+        arr = (ctypes.c_void_p * len(self.filters))(*[f.handle.value for f in self.filters])
+        self.handle = ctypes.c_void_p()
+        check(lib.tsl_batch_create(evaluator.handle, arr, len(self.filters), ctypes.byref(self.handle)))
+
+    def __len__(self):
+        return len(self.filters)
+
+    def _u0(self, u0):
+        u = np.ascontiguousarray(u0, np.float32).reshape(-1)
+        if u.size != len(self.filters):
+            raise ValueError("one draw per filter")
+        return u
+
+    def set_tile(self, k=1, chunk_len=0, points_per_lane=1):
+        check(lib.tsl_batch_set_tile(self.handle, int(k), int(points_per_lane), int(chunk_len)))
+
+    def prepare(self):
+        check(lib.tsl_batch_prepare(self.handle))
+
+    def restore_async(self, stream=None):
+        check(lib.tsl_batch_restore_async(self.handle, _stream(stream)))
+
+    def update_async(self, stream=None):
+        check(lib.tsl_batch_update_async(self.handle, _stream(stream)))
+
+    def resample_async(self, u0, stream=None):
+        u = self._u0(u0)
+        check(lib.tsl_batch_resample_async(self.handle, ptr(u), _stream(stream)))
+
+    def step_async(self, u0, restore=False, stream=None):
+        u = self._u0(u0)
+        check(lib.tsl_batch_step_async(self.handle, ptr(u), int(bool(restore)), _stream(stream)))
+
+    def capture(self, stream, restore=False):
+        check(lib.tsl_batch_capture(self.handle, int(bool(restore)), _stream(stream)))
+
+    def launch(self, u0, stream):
+        u = self._u0(u0)
+        check(lib.tsl_batch_launch(self.handle, ptr(u), _stream(stream)))
+
+    def motion_async(self, motions, stream=None):
+        arr = (Motion * len(self.filters))(*motions)
+        check(lib.tsl_batch_motion_async(self.handle, arr, _stream(stream)))
+
+    def sync(self, stream=None, raise_on_error=True):
+        """(poses, counts, status) of every member after `stream` drains."""
+        n = len(self.filters)
+        poses = (Pose * n)()
+        counts = np.zeros(n, np.uint64)
+        status = np.zeros(n, np.int32)
+        rc = lib.tsl_batch_sync(self.handle, _stream(stream), poses, ptr(counts), ptr(status))
+        if raise_on_error:
+            check(rc)
+        return [_pose_dict(p) for p in poses], counts, status
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.tsl_batch_destroy(self.handle)
+            self.handle = None
+

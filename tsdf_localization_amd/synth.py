@@ -8,6 +8,9 @@ of this project:
   building()  sparse 1024x1024x128 @ 0.05 m int16 TSDF of a multi-room building;
               128x2048 scan reduced to ~20k points; 100k particles uniform over
               free space, yaw uniform, roll/pitch within +-3 deg
+  batched()   64 independent filters on one 512x512x96 @ 0.05 m map of a
+              multi-room floor: particle counts log-uniform in [500, 50k],
+              scans of 2k-30k points, each filter tracking its own robot
 """
 import ctypes
 import math
@@ -222,3 +225,58 @@ def building(seed=0x5EED0002, num_particles=100_000, size=(51.2, 51.2, 6.4), res
     return Workload("building", m, pts, rings, parts, tf, true_pose,
                     {"scene_objects": sc.count(), "raw_points": int(raw.shape[0]), "allocated_blocks": blocks,
                      "block_fraction": blocks / m.grid_occ.size})
+
+
+@dataclass
+class Robot:
+    particles: np.ndarray
+    points: np.ndarray
+    true_pose: np.ndarray
+
+
+@dataclass
+class BatchedWorkload:
+    name: str
+    map: CudaSubVoxelMap
+    tf: np.ndarray
+    robots: list
+    meta: dict = field(default_factory=dict)
+
+
+def batched(seed=0x5EED0003, filters=64, size=(25.6, 25.6, 4.8), resolution=0.05, truncation_mm=300,
+            particles=(500, 50_000), points=(2_000, 30_000), beams=128, azimuths=2048, filter_cell=0.05,
+            room_size=8.0, objects_per_room=4, ceiling_fraction=0.3, wall_h=3.0):
+    """One shared map, `filters` robots at seeded free poses; robot i has
+    log-uniform particle count in `particles` (around its pose: 0.2 m, 5 deg)
+    and a scan of a uniform point count in `points` (its own ray cast,
+    voxel-filtered, subsampled; fewer if the cast returns fewer)."""
+    sx, sy, sz = size
+    sc = Scene()
+    _s.synth_scene_build_building(sc.h, seed, sx, sy, wall_h, room_size, objects_per_room, ceiling_fraction)
+    m = sc.tsdf_map((0, 0, 0), (sx, sy, sz), resolution, truncation_mm, dense=False)
+    rng = np.random.default_rng(seed)
+    tf = tf_from_xyzrpy(**SCANNER_TF)
+    robots = []
+    lo_p, hi_p = particles
+    for i in range(filters):
+        for _ in range(1000):
+            x, y = rng.uniform(1.5, sx - 1.5), rng.uniform(1.5, sy - 1.5)
+            if sc.sdf([[x, y, 1.0]])[0] > 0.6:
+                break
+        true_pose = np.array([x, y, 1.0, 0.0, 0.0, rng.uniform(-math.pi, math.pi)])
+        P = int(round(math.exp(rng.uniform(math.log(lo_p), math.log(hi_p)))))
+        N = int(rng.integers(points[0], points[1] + 1))
+        raw, rings = sc.raycast(compose_pose(true_pose, tf), beams, azimuths, -22.5, 22.5, 60.0, 0.01,
+                                seed + 17 * i)
+        pts, rings = voxel_filter(raw, rings, filter_cell, 1.0)
+        if pts.shape[0] > N:
+            pts = pts[np.sort(rng.choice(pts.shape[0], N, replace=False))]
+        parts = particles_normal(seed + i, P, true_pose,
+                                 [0.2, 0.2, 0.1, math.radians(5), math.radians(2), math.radians(5)])
+        robots.append(Robot(parts, np.ascontiguousarray(pts, np.float32), true_pose))
+    blocks = int(np.count_nonzero(m.grid_occ >= 0))
+    return BatchedWorkload("batched", m, tf, robots,
+                           {"allocated_blocks": blocks, "block_fraction": blocks / m.grid_occ.size,
+                            "particles_total": int(sum(r.particles.shape[0] for r in robots)),
+                            "evals_per_step": int(sum(r.particles.shape[0] * r.points.shape[0] for r in robots))})
+
