@@ -229,6 +229,22 @@ int tsl_filter_get_particles(tsl_filter* f, tsl_particle* particles, size_t capa
 int tsl_filter_set_scan(tsl_filter* f, const tsl_point* points, size_t n,
                         const float tf_matrix[16]);
 
+/* The scan reduction of TSDFEvaluator::evaluateParticles
+ * (src/evaluation/tsdf_evaluator.cpp:304-376) on the device, then as
+ * tsl_filter_set_scan: raw points with their ring numbers (the PointCloud2
+ * fields x, y, z and ring); points closer than 1 m are dropped, the rest are
+ * deduplicated by (ring, centre of their `cell`-sized voxel) keeping the
+ * first, and ordered by ring then index.  The reference's ring iterator
+ * advances only on kept points (:319-331); that is reproduced: the k-th
+ * kept point takes ring[k].  TSL_ERR_INVALID_ARGUMENT if a kept point's ring
+ * is outside [0, 64). */
+int tsl_filter_set_scan_raw(tsl_filter* f, const tsl_point* raw, const int16_t* ring, size_t n, float cell,
+                            const float tf_matrix[16]);
+
+/* The filter's scan points in the order they were given (after the
+ * reduction, for tsl_filter_set_scan_raw). */
+int tsl_filter_get_scan(tsl_filter* f, tsl_point* out, size_t capacity, size_t* n);
+
 /* Sensor update on `stream`: pose matrices, evaluation, normalisation and
  * pose sums; particles' .second become the normalised weights.  Async. */
 int tsl_filter_update_async(tsl_filter* f, void* stream);

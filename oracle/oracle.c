@@ -668,3 +668,90 @@ ORC_EXPORT void orc_motion(const orc_particle* in, size_t n, const double mean[6
         out[i].second = in[i].second;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Scan reduction of TSDFEvaluator::evaluateParticles
+ * (src/evaluation/tsdf_evaluator.cpp:304-376):
+ *   - points with sqrtf(x^2 + y^2 + z^2) < 1 are dropped; the ring iterator
+ *     and the running index advance only for kept points (:319-331), so the
+ *     k-th kept point reads ring[k] and gets index k;
+ *   - centre = floor(p / cell) * cell + cell / 2 per axis (float);
+ *   - an unordered_set keyed by (ring, centre) keeps the first point
+ *     inserted per key (float equality; NaN never equal);
+ *   - output grouped by ring 0..63, each ring in index order, the original
+ *     point of each kept entry.
+ * Returns the number of points written, or -1 if a kept point's ring is
+ * outside [0, 64) (the reference indexes a 64-entry vector with it). */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int ring;
+    uint32_t index;
+    float c[3];
+    float orig[3];
+} orc_red;
+
+static int orc_red_cmp_key(const void* pa, const void* pb)
+{
+    const orc_red* a = (const orc_red*)pa;
+    const orc_red* b = (const orc_red*)pb;
+    if (a->ring != b->ring) return a->ring < b->ring ? -1 : 1;
+    for (int k = 0; k < 3; ++k) {
+        uint32_t ua, ub;
+        memcpy(&ua, &a->c[k], 4);
+        memcpy(&ub, &b->c[k], 4);
+        if (ua != ub) return ua < ub ? -1 : 1;
+    }
+    return a->index < b->index ? -1 : (a->index > b->index ? 1 : 0);
+}
+
+static int orc_red_cmp_out(const void* pa, const void* pb)
+{
+    const orc_red* a = (const orc_red*)pa;
+    const orc_red* b = (const orc_red*)pb;
+    if (a->ring != b->ring) return a->ring < b->ring ? -1 : 1;
+    return a->index < b->index ? -1 : (a->index > b->index ? 1 : 0);
+}
+
+ORC_EXPORT long orc_reduce_scan(const float* xyz, const int16_t* ring, size_t n, float cell, float* out_xyz)
+{
+    const float half = cell / 2;
+    orc_red* v = (orc_red*)malloc(sizeof(orc_red) * (n ? n : 1));
+    size_t kept = 0;
+    for (size_t j = 0; j < n; ++j) {
+        float x = xyz[3 * j], y = xyz[3 * j + 1], z = xyz[3 * j + 2];
+        float dist = sqrtf(x * x + y * y + z * z);
+        if (dist < 1.0)
+            continue;
+        orc_red r;
+        r.ring = ring[kept]; /* the ring iterator lags behind skipped points */
+        r.index = (uint32_t)kept;
+        r.c[0] = floorf(x / cell) * cell + half;
+        r.c[1] = floorf(y / cell) * cell + half;
+        r.c[2] = floorf(z / cell) * cell + half;
+        r.orig[0] = x;
+        r.orig[1] = y;
+        r.orig[2] = z;
+        if (r.ring < 0 || r.ring >= 64) {
+            free(v);
+            return -1;
+        }
+        v[kept++] = r;
+    }
+    qsort(v, kept, sizeof(orc_red), orc_red_cmp_key);
+    size_t w = 0;
+    for (size_t i = 0; i < kept; ++i) {
+        int nan = v[i].c[0] != v[i].c[0] || v[i].c[1] != v[i].c[1] || v[i].c[2] != v[i].c[2];
+        int first = i == 0 || nan || v[i].ring != v[i - 1].ring || v[i].c[0] != v[i - 1].c[0] ||
+                    v[i].c[1] != v[i - 1].c[1] || v[i].c[2] != v[i - 1].c[2];
+        if (first)
+            v[w++] = v[i];
+    }
+    qsort(v, w, sizeof(orc_red), orc_red_cmp_out);
+    for (size_t i = 0; i < w; ++i) {
+        out_xyz[3 * i] = v[i].orig[0];
+        out_xyz[3 * i + 1] = v[i].orig[1];
+        out_xyz[3 * i + 2] = v[i].orig[2];
+    }
+    free(v);
+    return (long)w;
+}

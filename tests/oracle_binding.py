@@ -60,6 +60,7 @@ _sigs = {
     "orc_cumsum": (None, [_P, _S, _P]),
     "orc_motion_normal": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]),
     "orc_motion": (None, [_P, _S, _P, _P, ctypes.c_uint64, _P]),
+    "orc_reduce_scan": (ctypes.c_long, [_P, _P, _S, _F, _P]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(olib, _n)
@@ -221,4 +222,16 @@ def motion(particles, mean, stddev, seed):
     sd = np.ascontiguousarray(stddev, np.float64)
     olib.orc_motion(_ptr(p), p.shape[0], _ptr(m), _ptr(sd), int(seed), _ptr(out))
     return out
+
+
+def reduce_scan(points, rings, cell=0.064):
+    """Scan reduction of TSDFEvaluator::evaluateParticles; None if a ring is
+    out of range."""
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    r = np.ascontiguousarray(rings, np.int16).reshape(-1)
+    out = np.empty((max(p.shape[0], 1), 3), np.float32)
+    n = olib.orc_reduce_scan(_ptr(p), _ptr(r), p.shape[0], float(cell), _ptr(out))
+    if n < 0:
+        return None
+    return out[:n].copy()
 

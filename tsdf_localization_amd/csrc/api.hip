@@ -898,6 +898,49 @@ int tsl_resample_capacity(size_t M, float u0, size_t* capacity)
     return resample_capacity(M, u0, capacity);
 }
 
+int tsl_filter_set_scan_raw(tsl_filter* f, const tsl_point* raw, const int16_t* ring, size_t n, float cell,
+                            const float tf[16])
+{
+    if (!f || !tf || (n && (!raw || !ring))) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n > 0x7fffffffull) return set_error(TSL_ERR_UNSUPPORTED, "too many points");
+    if (!(cell > 0.0f)) return set_error(TSL_ERR_INVALID_ARGUMENT, "reduction cell size must be positive");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    TSL_TRY(filter_reserve_points(f, n));
+    TSL_TRY(f->raw_in.reserve(sizeof(tsl_point) * (n ? n : 1)));
+    TSL_TRY(f->raw_ring.reserve(sizeof(int16_t) * (n ? n : 1) + 16));  // rings, then two aligned counters
+    uint32_t* counters = reinterpret_cast<uint32_t*>(f->raw_ring.as<char>() + ((sizeof(int16_t) * (n ? n : 1) + 3) & ~(size_t)3));
+    const uint32_t zero[2] = {0u, 0u};
+    TSL_HIP_CHECK(hipMemcpy(counters, zero, sizeof(zero), hipMemcpyHostToDevice));
+    uint32_t host[2] = {0u, 0u};
+    if (n) {
+        TSL_HIP_CHECK(hipMemcpy(f->raw_in.ptr, raw, sizeof(tsl_point) * n, hipMemcpyHostToDevice));
+        TSL_HIP_CHECK(hipMemcpy(f->raw_ring.ptr, ring, sizeof(int16_t) * n, hipMemcpyHostToDevice));
+        TSL_TRY(launch_reduce_scan(f, f->raw_in.as<tsl_point>(), f->raw_ring.as<int16_t>(), (uint32_t)n, cell,
+                                   f->points_in.as<tsl_point>(), counters, counters + 1, 0));
+        TSL_HIP_CHECK(hipDeviceSynchronize());
+        TSL_HIP_CHECK(hipMemcpy(host, counters, sizeof(host), hipMemcpyDeviceToHost));
+    }
+    if (host[1]) return set_error(TSL_ERR_INVALID_ARGUMENT, "a kept point's ring is outside [0, 64)");
+    f->num_points = host[0];
+    std::memcpy(f->tf, tf, sizeof(float) * 16);
+    TSL_TRY(launch_prep_points(f, 0));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    return TSL_OK;
+}
+
+int tsl_filter_get_scan(tsl_filter* f, tsl_point* out, size_t capacity, size_t* n)
+{
+    if (!f || !n) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    *n = f->num_points;
+    if (f->num_points > capacity) return set_error(TSL_ERR_CAPACITY, "scan holds %zu points", f->num_points);
+    if (f->num_points && out)
+        TSL_HIP_CHECK(hipMemcpy(out, f->points_in.ptr, sizeof(tsl_point) * f->num_points, hipMemcpyDeviceToHost));
+    return TSL_OK;
+}
+
 int tsl_resample_systematic(const tsl_particle* in, size_t M, float u0, tsl_particle* out, size_t out_capacity,
                             int32_t* out_indices, size_t* out_count)
 {

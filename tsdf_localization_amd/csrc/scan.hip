@@ -13,6 +13,8 @@
 
 #include "tsl_internal.h"
 
+#include <algorithm>
+
 namespace tsl {
 
 namespace {
@@ -87,7 +89,177 @@ __global__ void k_particle_keys(const tsl_particle* __restrict__ in, const uint3
     vals[i] = i;
 }
 
+// ---- scan reduction (TSDFEvaluator::evaluateParticles,
+// src/evaluation/tsdf_evaluator.cpp:304-376) ---------------------------------
+
+// keep[j]: the reference skips points closer than 1 m (sqrtf, :316-322)
+__global__ void k_red_keep(const tsl_point* __restrict__ raw, uint32_t n, uint32_t* __restrict__ keep)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const tsl_point p = raw[j];
+    const float dist = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
+    keep[j] = (dist < 1.0f) ? 0u : 1u;
+}
+
+// Kept point j becomes entry k = kidx[j] (its index in the reference); it
+// reads ring[k] because the reference's ring iterator only advances on kept
+// points (:319-331).  Entries past the kept count are padded with ring 255.
+__global__ void k_red_build(const tsl_point* __restrict__ raw, const int16_t* __restrict__ ring, uint32_t n,
+                            const uint32_t* __restrict__ keep, const uint32_t* __restrict__ kidx, float cell,
+                            float half, uint64_t* __restrict__ key_a, uint32_t* __restrict__ ring_out,
+                            uint32_t* __restrict__ cx_out, tsl_point* __restrict__ orig, uint32_t* __restrict__ vals,
+                            uint32_t* __restrict__ bad)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    vals[j] = j;
+    const uint32_t kept_total = kidx[n - 1] + keep[n - 1];
+    if (j >= kept_total) {  // padding entry (filled by the thread of the same slot)
+        key_a[j] = ~0ull;
+        ring_out[j] = 255u;
+        cx_out[j] = 0xffffffffu;
+    }
+    if (!keep[j]) return;
+    const uint32_t k = kidx[j];
+    const tsl_point p = raw[j];
+    const int r = ring[k];
+    if (r < 0 || r >= 64) atomicOr(bad, 1u);
+    const float cx = floorf(p.x / cell) * cell + half;
+    const float cy = floorf(p.y / cell) * cell + half;
+    const float cz = floorf(p.z / cell) * cell + half;
+    key_a[k] = ((uint64_t)__float_as_uint(cy) << 32) | __float_as_uint(cz);
+    ring_out[k] = (uint32_t)(r & 0xff);
+    cx_out[k] = __float_as_uint(cx);
+    orig[k] = p;
+}
+
+// key of the second (stable) pass: ring, then the x centre
+__global__ void k_red_key_b(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ ring,
+                            const uint32_t* __restrict__ cx, uint32_t n, uint64_t* __restrict__ key_b)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t e = perm[t];
+    key_b[t] = ((uint64_t)ring[e] << 32) | cx[e];
+}
+
+// first entry of each (ring, centre) group wins; NaN centres never match
+__global__ void k_red_mark(const uint32_t* __restrict__ perm, const uint64_t* __restrict__ key_a,
+                           const uint64_t* __restrict__ key_b_sorted, const uint32_t* __restrict__ ring,
+                           const uint32_t* __restrict__ cx, const uint32_t* __restrict__ keep,
+                           const uint32_t* __restrict__ kidx, uint32_t n, uint64_t* __restrict__ out_key,
+                           uint32_t* __restrict__ win)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t kept_total = kidx[n - 1] + keep[n - 1];
+    const uint32_t e = perm[t];
+    bool w = false;
+    if (t < kept_total) {
+        const uint64_t ka = key_a[e];
+        const float fx = __uint_as_float(cx[e]);
+        const float fy = __uint_as_float((uint32_t)(ka >> 32));
+        const float fz = __uint_as_float((uint32_t)ka);
+        const bool nan = (fx != fx) | (fy != fy) | (fz != fz);
+        w = t == 0 || nan || key_b_sorted[t] != key_b_sorted[t - 1] || ka != key_a[perm[t - 1]];
+    }
+    win[t] = w ? 1u : 0u;
+    out_key[t] = w ? (((uint64_t)ring[e] << 32) | e) : ~0ull;
+}
+
+__global__ void k_red_order(const uint64_t* __restrict__ sorted_key, uint32_t n, uint32_t* __restrict__ order)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    order[r] = (uint32_t)sorted_key[r];  // entry index of the r-th winner (padding past the count)
+}
+
+__global__ void k_red_gather(const uint32_t* __restrict__ order, const tsl_point* __restrict__ orig,
+                             const uint32_t* __restrict__ count, tsl_point* __restrict__ out)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= *count) return;
+    out[r] = orig[order[r]];
+}
+
 }  // namespace
+
+// Reduce a raw scan on the device: writes the reduced points to out (room
+// for n) and their number to *count_dev, and 1 to *bad_dev if a kept point's
+// ring is outside [0, 64).  Scratch buffers belong to the caller's filter.
+int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring, uint32_t n, float cell,
+                       tsl_point* out, uint32_t* count_dev, uint32_t* bad_dev, hipStream_t stream)
+{
+    if (n == 0) return TSL_OK;
+    const float half = cell / 2;
+    DevBuf& sc = f->red_scratch;
+    // layout: keep, kidx, ring, cx, vals, perm1, perm2, win, order (uint32 x n each),
+    //         key_a, key_b, key_b_sorted, out_key, out_key_sorted (uint64 x n each), orig (tsl_point x n)
+    const size_t u32 = sizeof(uint32_t) * (size_t)n, u64 = sizeof(uint64_t) * (size_t)n;
+    size_t t_scan = 0, t_sort = 0, t_sum = 0, t_keys = 0;
+    TSL_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n,
+                                                   stream));
+    TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, t_keys, (uint64_t*)nullptr, (uint64_t*)nullptr, (int)n, 0,
+                                                    64, stream));
+    TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, t_sort, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                     (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 64, stream));
+    TSL_HIP_CHECK(hipcub::DeviceReduce::Sum(nullptr, t_sum, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, stream));
+    const size_t temp = std::max(std::max(t_scan, t_keys), std::max(t_sort, t_sum));
+    const size_t total = 9 * u32 + 5 * u64 + sizeof(tsl_point) * (size_t)n + temp + 16 * 256;
+    TSL_TRY(sc.reserve(total));
+    char* base = sc.as<char>();
+    auto take = [&](size_t bytes) {
+        char* p = base;
+        base += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    uint32_t* keep = (uint32_t*)take(u32);
+    uint32_t* kidx = (uint32_t*)take(u32);
+    uint32_t* ring_e = (uint32_t*)take(u32);
+    uint32_t* cx = (uint32_t*)take(u32);
+    uint32_t* vals = (uint32_t*)take(u32);
+    uint32_t* perm1 = (uint32_t*)take(u32);
+    uint32_t* perm2 = (uint32_t*)take(u32);
+    uint32_t* win = (uint32_t*)take(u32);
+    uint32_t* order = (uint32_t*)take(u32);
+    uint64_t* key_a = (uint64_t*)take(u64);
+    uint64_t* key_a_sorted = (uint64_t*)take(u64);
+    uint64_t* key_b = (uint64_t*)take(u64);
+    uint64_t* key_b_sorted = (uint64_t*)take(u64);
+    uint64_t* out_key = (uint64_t*)take(u64);
+    tsl_point* orig = (tsl_point*)take(sizeof(tsl_point) * (size_t)n);
+    void* tmp = take(temp);
+    if ((size_t)(base - sc.as<char>()) > sc.bytes) return set_error(TSL_ERR_HIP, "scan reduction scratch layout");
+    const unsigned g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_red_keep, dim3(g), dim3(256), 0, stream, raw, n, keep);
+    TSL_HIP_CHECK(hipGetLastError());
+    size_t tb = temp;
+    TSL_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, keep, kidx, (int)n, stream));
+    hipLaunchKernelGGL(k_red_build, dim3(g), dim3(256), 0, stream, raw, ring, n, keep, kidx, cell, half, key_a, ring_e,
+                       cx, orig, vals, bad_dev);
+    TSL_HIP_CHECK(hipGetLastError());
+    // pass A: (y centre, z centre), stable; pass B: (ring, x centre), stable
+    tb = temp;
+    TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key_a, key_a_sorted, vals, perm1, (int)n, 0, 64, stream));
+    hipLaunchKernelGGL(k_red_key_b, dim3(g), dim3(256), 0, stream, perm1, ring_e, cx, n, key_b);
+    TSL_HIP_CHECK(hipGetLastError());
+    tb = temp;
+    TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key_b, key_b_sorted, perm1, perm2, (int)n, 0, 64, stream));
+    hipLaunchKernelGGL(k_red_mark, dim3(g), dim3(256), 0, stream, perm2, key_a, key_b_sorted, ring_e, cx, keep, kidx, n,
+                       out_key, win);
+    TSL_HIP_CHECK(hipGetLastError());
+    tb = temp;
+    TSL_HIP_CHECK(hipcub::DeviceReduce::Sum(tmp, tb, win, count_dev, (int)n, stream));
+    // winners by (ring, index): the entry position is the reference's index
+    tb = temp;
+    TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(tmp, tb, out_key, key_b, (int)n, 0, 64, stream));
+    hipLaunchKernelGGL(k_red_order, dim3(g), dim3(256), 0, stream, key_b, n, order);
+    TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_red_gather, dim3(g), dim3(256), 0, stream, order, orig, count_dev, out);
+    TSL_HIP_CHECK(hipGetLastError());
+    return TSL_OK;
+}
 
 int launch_particle_order(tsl_filter* f, hipStream_t stream)
 {

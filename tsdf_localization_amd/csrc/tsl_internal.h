@@ -285,6 +285,8 @@ struct tsl_filter {
     tsl::DevBuf perm;        // uint32[num_points]: caller index of sorted point j
     tsl::DevBuf sort_keys, sort_vals, sort_temp;
     tsl::DevBuf pkeys, pvals, ptemp;  // particle-order sort
+    tsl::DevBuf red_scratch;          // scan reduction
+    tsl::DevBuf raw_in, raw_ring;     // raw scan staging
     tsl::DevBuf porder;      // uint32: particle evaluated in slot s
     tsl::DevBuf pinv;        // uint32: slot of particle i
     int order_mode = -1;     // particle evaluation order: 0 caller, 1 spatial, -1 automatic
@@ -325,6 +327,8 @@ int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const floa
                          float* d_out, hipStream_t stream);
 int launch_prep_points(tsl_filter* f, hipStream_t stream);
 int launch_particle_order(tsl_filter* f, hipStream_t stream);
+int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring, uint32_t n, float cell,
+                       tsl_point* out, uint32_t* count_dev, uint32_t* bad_dev, hipStream_t stream);
 
 // Resampling (resample.hip): members[0..n) resampled in one sequence of
 // launches; grid_cap = max cap_limit over the members.

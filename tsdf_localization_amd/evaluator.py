@@ -330,6 +330,24 @@ class DeviceFilter:
         tf = _tf(tf_matrix)
         check(lib.tsl_filter_set_scan(self.handle, ptr(pts), pts.shape[0], ptr(tf)))
 
+    def set_scan_raw(self, points, rings, tf_matrix, cell=0.064):
+        """Raw scan + ring numbers through the reference's reduction
+        (tsdf_evaluator.cpp:304-376; cell = reduction_cell_size, default
+        0.064 as TSDFEvaluator's constructor, tsdf_evaluator.h:72)."""
+        pts = _points(points)
+        r = np.ascontiguousarray(rings, np.int16).reshape(-1)
+        if r.size != pts.shape[0]:
+            raise ValueError("one ring number per point")
+        tf = _tf(tf_matrix)
+        check(lib.tsl_filter_set_scan_raw(self.handle, ptr(pts), ptr(r), pts.shape[0], float(cell), ptr(tf)))
+
+    def get_scan(self):
+        n = ctypes.c_size_t()
+        lib.tsl_filter_get_scan(self.handle, None, 0, ctypes.byref(n))
+        out = np.empty((max(n.value, 1), 3), np.float32)
+        check(lib.tsl_filter_get_scan(self.handle, ptr(out), out.shape[0], ctypes.byref(n)))
+        return out[: n.value].copy()
+
     def update_async(self, stream=None):
         check(lib.tsl_filter_update_async(self.handle, None if stream is None else stream.handle))
 
