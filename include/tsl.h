@@ -375,6 +375,22 @@ int tsl_batch_launch(tsl_batch* b, const float* u0, void* stream);
  * does not advance); returns the first member error, if any. */
 int tsl_batch_sync(tsl_batch* b, void* stream, tsl_pose* poses, uint64_t* counts, int* status);
 
+/* ---- .mcl snapshot files ------------------------------------------------ */
+
+/* MCLFile::write (src/util/mcl_file.cpp:14-64): points, their rings,
+ * particles, tf[16] and pose (x, y, z, q1..q4) as text with the default
+ * stream formatting (6 significant digits, so values round-trip only to
+ * that precision, as in the reference).  Host only. */
+int tsl_mcl_write(const char* path, const tsl_point* points, const int32_t* rings, size_t num_points,
+                  const tsl_particle* particles, size_t num_particles, const float tf[16], const float pose[7]);
+
+/* MCLFile::read (src/util/mcl_file.cpp:66-113).  Reports the sizes through
+ * num_points / num_particles; TSL_ERR_CAPACITY (sizes still reported) if the
+ * buffers are too small, so a first call with zero capacities sizes them.
+ * tf and pose may be NULL.  Host only. */
+int tsl_mcl_read(const char* path, tsl_point* points, int32_t* rings, size_t points_capacity, tsl_particle* particles,
+                 size_t particles_capacity, float tf[16], float pose[7], size_t* num_points, size_t* num_particles);
+
 /* ---- streams and timing (plumbing for benchmarks) ---------------------- */
 int tsl_stream_create(void** stream);
 int tsl_stream_destroy(void* stream);

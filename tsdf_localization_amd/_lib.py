@@ -98,6 +98,8 @@ _SIGS = {
     "tsl_filter_autotune": (_I, [_P, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_F)]),
     "tsl_filter_set_order": (_I, [_P, _I]),
+    "tsl_mcl_write": (_I, [ctypes.c_char_p, _P, _P, _S, _P, _S, _P, _P]),
+    "tsl_mcl_read": (_I, [ctypes.c_char_p, _P, _P, _S, _P, _S, _P, _P, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
     "tsl_filter_set_scan_raw": (_I, [_P, _P, _P, _S, _F, _P]),
     "tsl_filter_get_scan": (_I, [_P, _P, _S, ctypes.POINTER(_S)]),
     "tsl_batch_create": (_I, [_P, _P, _S, ctypes.POINTER(_P)]),
