@@ -375,6 +375,68 @@ int tsl_batch_launch(tsl_batch* b, const float* u0, void* stream);
  * does not advance); returns the first member error, if any. */
 int tsl_batch_sync(tsl_batch* b, void* stream, tsl_pose* poses, uint64_t* counts, int* status);
 
+/* ---- serving: resident robots, continuous batching --------------------- */
+
+/* A server keeps many robots' particle sets resident on the device and
+ * runs their requests (motion step + scan + resampling draw) with
+ * continuous batching: a worker thread starts a tick with the pending
+ * requests of every robot not in flight (at most max_batch, oldest first;
+ * one request per robot per tick), completes earlier ticks as they finish
+ * and admits requests that arrive meanwhile into the next tick.  Robots are
+ * admitted against an HBM budget (TSL_ERR_ADMISSION).  A robot's result is
+ * the same alone or in any mix.  No counterpart in the reference (one robot
+ * per ROS node, src/mcl_3d.cpp). */
+typedef struct tsl_server tsl_server;
+
+typedef struct tsl_request {
+    uint64_t id;              /* caller's tag, returned with the result */
+    uint32_t robot;           /* from tsl_server_add_robot */
+    uint32_t pad;
+    tsl_motion motion;        /* odometry step (applied before the sensor update) */
+    const tsl_point* points;  /* reduced scan, copied by tsl_server_submit */
+    size_t num_points;
+    float u0;                 /* systematic-resampling draw in [0, 1/M) */
+    float pad2;
+} tsl_request;
+
+typedef struct tsl_result {
+    uint64_t id;
+    uint32_t robot;
+    int32_t status;           /* 0 ok, 1 no valid particle, 2 resampling capacity, 3 empty, 4 draw does not advance */
+    tsl_pose pose;            /* weighted mean pose of the sensor update */
+    uint64_t count;           /* particles after resampling */
+    double submit_ms, start_ms, done_ms;  /* server clock: submitted, tick started, result ready */
+    uint32_t batch;           /* requests in the tick */
+    uint32_t pad;
+} tsl_result;
+
+/* hbm_budget_bytes: device bytes the server may hold (map included; 0 = no
+ * limit); max_robots resident at once; max_particles per robot;
+ * max_batch requests per tick; chunk_len scan points per evaluation chunk
+ * (0 = 2048). */
+int tsl_server_create(tsl_evaluator* ev, uint64_t hbm_budget_bytes, uint32_t max_robots, uint64_t max_particles,
+                      uint32_t max_batch, uint32_t chunk_len, tsl_server** out);
+int tsl_server_destroy(tsl_server* s);
+
+/* Admit a robot with its particle set, the largest scan it will send and its
+ * scanner->base transform; every device buffer its ticks need is reserved
+ * here. */
+int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n, uint32_t max_points,
+                         const float tf_matrix[16], uint32_t* robot);
+int tsl_server_remove_robot(tsl_server* s, uint32_t robot);
+
+/* Thread-safe; the scan is copied before returning. */
+int tsl_server_submit(tsl_server* s, const tsl_request* request);
+
+/* Up to capacity finished results (waiting up to wait_ms for the first). */
+int tsl_server_poll(tsl_server* s, tsl_result* out, size_t capacity, size_t* n, int wait_ms);
+
+/* A robot's particles (waits for its request in flight; call with none
+ * pending for it). */
+int tsl_server_get_particles(tsl_server* s, uint32_t robot, tsl_particle* out, size_t capacity, size_t* n);
+
+int tsl_server_stats(tsl_server* s, uint64_t* used_bytes, uint64_t* ticks, uint64_t* requests);
+
 /* ---- .mcl snapshot files ------------------------------------------------ */
 
 /* MCLFile::write (src/util/mcl_file.cpp:14-64): points, their rings,

@@ -1,0 +1,143 @@
+"""Serving loop: resident robots, continuous batching, HBM admission.
+A robot's result must be the same bits alone and in a mix, equal to the
+single-filter path (motion, scan, update, resampling with the same chunk),
+and the evaluation must match the oracle."""
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+
+pytestmark = pytest.mark.gpu
+
+CHUNK = 512
+
+
+@pytest.fixture(scope="module")
+def tsl():
+    import tsdf_localization_amd as t
+    assert t.device_count() >= 1, "no HIP device"
+    return t
+
+
+@pytest.fixture(scope="module")
+def room():
+    from tsdf_localization_amd import synth
+    return synth.room()
+
+
+def _robots(room, k, seed=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        P = int(rng.integers(100, 1000))
+        parts = room.particles[rng.choice(room.particles.shape[0], P, replace=False)].copy()
+        parts[:, 6] = 1.0 / P
+        N = int(rng.integers(200, room.points.shape[0]))
+        pts = room.points[np.sort(rng.choice(room.points.shape[0], N, replace=False))]
+        out.append((parts, pts))
+    return out
+
+
+def _motion(tsl, i):
+    return tsl.evaluator.motion_from_delta(0.05 * (i % 3), 0.01, 0.0, 0.0, 0.0, 0.02 * (i % 5), 0.01, 0.005,
+                                           seed=1000 + i)
+
+
+def test_server_alone_equals_mix_equals_filter(tsl, room):
+    from tsdf_localization_amd.server import Server
+    ev = tsl.TSDFEvaluator(room.map)
+    robots = _robots(room, 8)
+    srv = Server(ev, max_robots=32, max_particles=2000, max_batch=16, chunk_len=CHUNK)
+    try:
+        ids_a = [srv.add_robot(p, 3000, room.tf) for p, _ in robots]
+        ids_b = [srv.add_robot(p, 3000, room.tf) for p, _ in robots]  # twins
+        u0 = [np.float32(0.31 / p.shape[0]) for p, _ in robots]
+        # robot 0 alone
+        srv.submit(ids_a[0], _motion(tsl, 0), robots[0][1], u0[0], request_id=1)
+        alone = srv.wait(1)[0]
+        assert alone["batch"] == 1 and alone["status"] == 0
+        # all twins together
+        for i, (p, pts) in enumerate(robots):
+            srv.submit(ids_b[i], _motion(tsl, i), pts, u0[i], request_id=100 + i)
+        mix = {r["id"]: r for r in srv.wait(len(robots))}
+        assert max(r["batch"] for r in mix.values()) > 1
+        assert mix[100]["pose"] == alone["pose"] and mix[100]["count"] == alone["count"]
+        pa = srv.get_particles(ids_a[0])
+        pb = srv.get_particles(ids_b[0])
+        assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+        for r in mix.values():
+            assert r["status"] == 0
+            assert r["submit_ms"] <= r["start_ms"] <= r["done_ms"]
+        # single-filter path with the same steps
+        for i in (0, 3, 7):
+            parts, pts = robots[i]
+            f = tsl.DeviceFilter(ev, parts.shape[0])
+            f.set_particles(parts)
+            f.set_tile(1, CHUNK, 1)
+            f.set_order(0)
+            s = tsl.Stream()
+            f.motion_async(_motion(tsl, i), s)
+            s.synchronize()
+            moved = f.get_particles()
+            f.set_scan(pts, room.tf)
+            f.update_async(s)
+            pose, _ = f.sync(s)
+            weighted = f.get_particles()
+            f.resample_async(u0[i], s)
+            f.sync(s)
+            assert np.array_equal(srv.get_particles(ids_b[i]).view(np.uint32), f.get_particles().view(np.uint32)), i
+            assert mix[100 + i]["pose"] == pose
+            # the evaluation against the oracle
+            om = ob.OracleMap.from_host_map(room.map)
+            _, w64 = om.evaluate(moved, pts, room.tf)
+            ref = w64 / w64.sum()
+            assert np.max(np.abs(weighted[:, 6].astype(np.float64) - ref) / ref) <= 1e-5
+    finally:
+        srv.close()
+
+
+def test_server_admission_by_budget(tsl, room):
+    from tsdf_localization_amd.server import Server
+    ev = tsl.TSDFEvaluator(room.map)
+    parts = room.particles[:500]
+    probe = Server(ev, max_robots=4, max_particles=1000, max_batch=4)
+    try:
+        base = probe.stats()["used_bytes"]
+        r = probe.add_robot(parts, 2000, room.tf)
+        per_robot = probe.stats()["used_bytes"] - base
+    finally:
+        probe.close()
+    srv = Server(ev, hbm_budget=base + 2 * per_robot + per_robot // 2, max_robots=8, max_particles=1000, max_batch=4)
+    try:
+        a = srv.add_robot(parts, 2000, room.tf)
+        b = srv.add_robot(parts, 2000, room.tf)
+        with pytest.raises(tsl.TslError, match="bytes"):
+            srv.add_robot(parts, 2000, room.tf)
+        srv.remove_robot(a)
+        c = srv.add_robot(parts, 2000, room.tf)
+        assert c == a
+        srv.submit(b, _motion(tsl, 1), room.points, np.float32(0.1 / 500), request_id=7)
+        srv.submit(c, _motion(tsl, 2), room.points, np.float32(0.2 / 500), request_id=8)
+        res = srv.wait(2)
+        assert sorted(r["id"] for r in res) == [7, 8] and all(r["status"] == 0 for r in res)
+    finally:
+        srv.close()
+
+
+def test_server_many_requests_per_robot_in_order(tsl, room):
+    """Requests of one robot run one after the other, in submission order."""
+    from tsdf_localization_amd.server import Server
+    ev = tsl.TSDFEvaluator(room.map)
+    robots = _robots(room, 3, seed=11)
+    srv = Server(ev, max_robots=4, max_particles=2000, max_batch=8, chunk_len=CHUNK)
+    try:
+        ids = [srv.add_robot(p, 3000, room.tf) for p, _ in robots]
+        for step in range(4):
+            for i, (p, pts) in enumerate(robots):
+                srv.submit(ids[i], _motion(tsl, step), pts, np.float32(0.5 / p.shape[0]), request_id=step * 10 + i)
+        res = srv.wait(12)
+        for i in range(3):
+            mine = [r for r in res if r["robot"] == ids[i]]
+            assert [r["id"] for r in sorted(mine, key=lambda r: r["done_ms"])] == [s * 10 + i for s in range(4)]
+    finally:
+        srv.close()

@@ -48,6 +48,20 @@ class Motion(ctypes.Structure):
     _fields_ = [("mean", ctypes.c_double * 6), ("stddev", ctypes.c_double * 6), ("seed", ctypes.c_uint64)]
 
 
+class Request(ctypes.Structure):
+    """tsl_request: one serving request."""
+    _fields_ = [("id", ctypes.c_uint64), ("robot", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("motion", Motion),
+                ("points", ctypes.c_void_p), ("num_points", ctypes.c_size_t), ("u0", ctypes.c_float),
+                ("pad2", ctypes.c_float)]
+
+
+class Result(ctypes.Structure):
+    """tsl_result: one finished serving request."""
+    _fields_ = [("id", ctypes.c_uint64), ("robot", ctypes.c_uint32), ("status", ctypes.c_int32), ("pose", Pose),
+                ("count", ctypes.c_uint64), ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double), ("batch", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+
 assert ctypes.sizeof(MapCoef) == 120
 
 if not os.path.exists(LIB_PATH):
@@ -98,6 +112,16 @@ _SIGS = {
     "tsl_filter_autotune": (_I, [_P, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_F)]),
     "tsl_filter_set_order": (_I, [_P, _I]),
+    "tsl_server_create": (_I, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                               ctypes.POINTER(_P)]),
+    "tsl_server_destroy": (_I, [_P]),
+    "tsl_server_add_robot": (_I, [_P, _P, _S, ctypes.c_uint32, _P, ctypes.POINTER(ctypes.c_uint32)]),
+    "tsl_server_remove_robot": (_I, [_P, ctypes.c_uint32]),
+    "tsl_server_submit": (_I, [_P, ctypes.POINTER(Request)]),
+    "tsl_server_poll": (_I, [_P, _P, _S, ctypes.POINTER(_S), _I]),
+    "tsl_server_get_particles": (_I, [_P, ctypes.c_uint32, _P, _S, ctypes.POINTER(_S)]),
+    "tsl_server_stats": (_I, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                              ctypes.POINTER(ctypes.c_uint64)]),
     "tsl_mcl_write": (_I, [ctypes.c_char_p, _P, _P, _S, _P, _S, _P, _P]),
     "tsl_mcl_read": (_I, [ctypes.c_char_p, _P, _P, _S, _P, _S, _P, _P, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
     "tsl_filter_set_scan_raw": (_I, [_P, _P, _P, _S, _F, _P]),

@@ -282,6 +282,23 @@ int launch_particle_order(tsl_filter* f, hipStream_t stream)
     return TSL_OK;
 }
 
+// Reserve every buffer the scan preparation of up to n points uses, so that
+// later preparations allocate nothing (and never synchronise).
+int filter_reserve_scan(tsl_filter* f, size_t n)
+{
+    if (n == 0) n = 1;
+    TSL_TRY(filter_reserve_points(f, n));
+    TSL_TRY(f->sort_keys.reserve(sizeof(uint64_t) * 2 * n));
+    TSL_TRY(f->sort_vals.reserve(sizeof(uint32_t) * n));
+    TSL_TRY(f->perm.reserve(sizeof(uint32_t) * n));
+    size_t temp = 0;
+    uint64_t* keys = f->sort_keys.as<uint64_t>();
+    TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, keys, keys + n, f->sort_vals.as<uint32_t>(),
+                                                     f->perm.as<uint32_t>(), (int)n, 0, 63, (hipStream_t)0));
+    TSL_TRY(f->sort_temp.reserve(temp));
+    return TSL_OK;
+}
+
 int launch_prep_points(tsl_filter* f, hipStream_t stream)
 {
     const uint32_t n = (uint32_t)f->num_points;

@@ -1,0 +1,526 @@
+// Serving loop: many robots' particle sets stay resident on the device;
+// requests (odometry step + scan + resampling draw) arrive from any thread;
+// a worker thread keeps the GPU busy with continuous batching -- every
+// "tick" takes the pending requests of robots that are not in flight (at
+// most max_batch, oldest first), enqueues motion update, scan preparation,
+// batched sensor update and resampling for all of them, and completes
+// earlier ticks as their events fire, so requests that arrive while others
+// run are admitted into the next tick at once.  Robots are admitted against
+// an HBM budget.  Each robot's result is the same whether it runs alone or
+// in a mix (member-indexed kernels, counter-based motion noise).
+// No counterpart in the reference (one robot per ROS node, src/mcl_3d.cpp).
+#include "tsl_internal.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+using namespace tsl;
+
+namespace {
+
+constexpr int kMaxInflight = 3;
+
+struct Robot {
+    tsl_filter* f = nullptr;
+    uint32_t max_points = 0;
+    uint64_t bytes = 0;
+    bool active = false;
+    bool busy = false;
+    tsl_point* staging = nullptr;  // pinned, max_points
+};
+
+struct Pending {
+    tsl_request req;
+    std::vector<tsl_point> points;
+    double submit_ms;
+};
+
+struct Slot {  // one tick in flight
+    std::vector<uint32_t> robots;
+    std::vector<Pending> reqs;
+    DevBuf members, plan, u0, motions, states;
+    Member* members_host = nullptr;      // pinned
+    float* u0_host = nullptr;            // pinned
+    tsl_motion* motions_host = nullptr;  // pinned
+    FilterState* states_host = nullptr;  // pinned
+    hipEvent_t done = nullptr;
+    double start_ms = 0.0;
+    bool used = false;
+};
+
+__global__ void k_gather_states_s(const Member* __restrict__ members, uint32_t n, FilterState* __restrict__ out)
+{
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < n) out[f] = *members[f].st;
+}
+
+}  // namespace
+
+struct tsl_server {
+    tsl_evaluator* ev = nullptr;
+    int device = 0;
+    uint64_t budget = 0, used = 0;
+    uint32_t max_batch = 64;
+    uint32_t chunk_len = 2048;
+    uint64_t max_particles = 0;    // per robot; sizes the shared reduction grid
+    std::vector<Robot> robots;     // fixed size (max_robots): entries never move
+    std::mutex mu;
+    std::condition_variable cv_req, cv_done;
+    std::deque<Pending> pending;
+    std::deque<tsl_result> done;
+    std::deque<Slot*> inflight;
+    std::vector<Slot*> free_slots;
+    Slot slots[kMaxInflight];
+    hipStream_t stream = nullptr;
+    std::thread worker;
+    std::atomic<bool> stop{false};
+    std::chrono::steady_clock::time_point t0;
+    int per_cu = 0, cus = 0;
+    int worker_error = TSL_OK;
+    uint64_t ticks = 0, batched_requests = 0;
+
+    double now_ms() const
+    {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+void complete(tsl_server* s, Slot* sl)
+{
+    const double t = s->now_ms();
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (size_t i = 0; i < sl->robots.size(); ++i) {
+        const FilterState& st = sl->states_host[i];
+        tsl_result r;
+        std::memset(&r, 0, sizeof(r));
+        r.id = sl->reqs[i].req.id;
+        r.robot = sl->robots[i];
+        r.status = (int)st.status;
+        r.count = st.count;
+        pose_from_stats(st.stats, &r.pose);
+        r.submit_ms = sl->reqs[i].submit_ms;
+        r.start_ms = sl->start_ms;
+        r.done_ms = t;
+        r.batch = (uint32_t)sl->robots.size();
+        Robot& rb = s->robots[sl->robots[i]];
+        rb.busy = false;
+        rb.f->count_bound = rb.f->capacity;
+        rb.f->count_known = false;
+        s->done.push_back(r);
+    }
+    sl->reqs.clear();
+    sl->robots.clear();
+    sl->used = false;
+    s->free_slots.push_back(sl);
+    s->cv_done.notify_all();
+}
+
+// Enqueue one tick for the selected requests (robots marked busy already).
+int dispatch(tsl_server* s, Slot* sl)
+{
+    hipStream_t st = s->stream;
+    const uint32_t n = (uint32_t)sl->reqs.size();
+    uint64_t max_items = 0;
+    uint32_t max_cap = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        Robot& rb = s->robots[sl->robots[i]];
+        tsl_filter* f = rb.f;
+        const Pending& pd = sl->reqs[i];
+        const size_t np = pd.points.size();
+        if (np) std::memcpy(rb.staging, pd.points.data(), sizeof(tsl_point) * np);
+        if (np) TSL_HIP_CHECK(hipMemcpyAsync(f->points_in.ptr, rb.staging, sizeof(tsl_point) * np,
+                                             hipMemcpyHostToDevice, st));
+        f->num_points = np;
+        TSL_TRY(launch_prep_points(f, st));
+        const uint32_t chunk = s->chunk_len;
+        const uint32_t chunks = (uint32_t)std::max<size_t>(1, (np + chunk - 1) / chunk);
+        Member& h = sl->members_host[i];
+        std::memset(&h, 0, sizeof(h));
+        h.particles = f->particles.as<tsl_particle>();
+        h.alt = f->particles_alt.as<tsl_particle>();
+        h.L = f->L.as<float>();
+        h.points = f->points.as<float4>();
+        h.partial = f->partial.as<float>();
+        h.raw = f->raw.as<float>();
+        h.blocks = f->blocks.as<double>();
+        h.st = f->state.as<FilterState>();
+        h.cumsum = f->cumsum.as<double>();
+        h.ubuf = f->ubuf.as<float>();
+        h.plan = f->plan.as<RsPlan>();
+        h.u0_dev = sl->u0.as<float>() + i;
+        std::memcpy(h.tf, f->tf, sizeof(h.tf));
+        h.N = (uint32_t)np;
+        h.chunk_len = chunk;
+        h.chunks = chunks;
+        h.stride = (uint32_t)f->capacity;
+        h.cap_limit = (uint32_t)f->capacity;
+        sl->u0_host[i] = pd.req.u0;
+        sl->motions_host[i] = pd.req.motion;
+        max_items += (f->capacity + 3) / 4 * chunks;
+        max_cap = std::max<uint32_t>(max_cap, (uint32_t)f->capacity);
+    }
+    TSL_HIP_CHECK(hipMemcpyAsync(sl->members.ptr, sl->members_host, sizeof(Member) * n, hipMemcpyHostToDevice, st));
+    TSL_HIP_CHECK(hipMemcpyAsync(sl->u0.ptr, sl->u0_host, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    TSL_HIP_CHECK(hipMemcpyAsync(sl->motions.ptr, sl->motions_host, sizeof(tsl_motion) * n, hipMemcpyHostToDevice, st));
+    BatchLaunch l;
+    l.map = s->ev->map;
+    l.a_hit = s->ev->a_hit;
+    l.members = sl->members.as<Member>();
+    l.plan = sl->plan.as<BatchPlan>();
+    l.n = n;
+    l.K = 1;
+    l.P = 1;
+    l.max_items = max_items;
+    l.max_cap = max_cap;
+    l.max_rs = max_cap;
+    uint64_t grid = (uint64_t)s->per_cu * (uint64_t)s->cus;
+    const uint64_t need = (max_items + 7) / 8 * 8;
+    if (grid > need) grid = need;
+    grid = std::max<uint64_t>(8, (grid + 7) / 8 * 8);
+    l.grid = (uint32_t)grid;
+    TSL_TRY(launch_motion_members(l.members, sl->motions.as<tsl_motion>(), n, max_cap, st));
+    TSL_TRY(launch_batch_update(l, st));
+    TSL_TRY(launch_resample_members(l.members, n, max_cap, 0.0f, true, st));
+    hipLaunchKernelGGL(k_gather_states_s, dim3((n + 63) / 64), dim3(64), 0, st, l.members, n,
+                       sl->states.as<FilterState>());
+    TSL_HIP_CHECK(hipGetLastError());
+    TSL_HIP_CHECK(hipMemcpyAsync(sl->states_host, sl->states.ptr, sizeof(FilterState) * n, hipMemcpyDeviceToHost, st));
+    TSL_HIP_CHECK(hipEventRecord(sl->done, st));
+    return TSL_OK;
+}
+
+void worker_main(tsl_server* s)
+{
+    (void)hipSetDevice(s->device);
+    while (true) {
+        bool progressed = false;
+        // completions, oldest first
+        while (!s->inflight.empty()) {
+            Slot* sl = s->inflight.front();
+            const hipError_t q = hipEventQuery(sl->done);
+            if (q == hipErrorNotReady) break;
+            s->inflight.pop_front();
+            if (q != hipSuccess) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->worker_error = set_error(TSL_ERR_HIP, "tick failed: %s", hipGetErrorString(q));
+            }
+            complete(s, sl);
+            progressed = true;
+        }
+        // admit the next tick
+        Slot* sl = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->stop && s->pending.empty() && s->inflight.empty()) break;
+            if (!s->free_slots.empty() && !s->pending.empty()) {
+                Slot* cand = s->free_slots.back();
+                std::vector<uint8_t> taken(s->robots.size(), 0);
+                for (auto it = s->pending.begin(); it != s->pending.end() && cand->reqs.size() < s->max_batch;) {
+                    const uint32_t r = it->req.robot;
+                    Robot& rb = s->robots[r];
+                    if (rb.active && !rb.busy && !taken[r]) {
+                        taken[r] = 1;
+                        rb.busy = true;
+                        cand->robots.push_back(r);
+                        cand->reqs.push_back(std::move(*it));
+                        it = s->pending.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
+                if (!cand->reqs.empty()) {
+                    s->free_slots.pop_back();
+                    sl = cand;
+                }
+            }
+        }
+        if (sl) {
+            sl->start_ms = s->now_ms();
+            const int rc = dispatch(s, sl);
+            if (rc != TSL_OK) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->worker_error = rc;
+            }
+            sl->used = true;
+            s->inflight.push_back(sl);
+            s->ticks += 1;
+            s->batched_requests += sl->reqs.size();
+            progressed = true;
+        }
+        if (!progressed) {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->inflight.empty()) {
+                s->cv_req.wait_for(lk, std::chrono::milliseconds(2));
+            } else {
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+            }
+        }
+    }
+}
+
+int slot_init(tsl_server* s, Slot& sl)
+{
+    const size_t n = s->max_batch;
+    TSL_TRY(sl.members.reserve(sizeof(Member) * n));
+    TSL_TRY(sl.plan.reserve(sizeof(BatchPlan)));
+    TSL_TRY(sl.u0.reserve(sizeof(float) * n));
+    TSL_TRY(sl.motions.reserve(sizeof(tsl_motion) * n));
+    TSL_TRY(sl.states.reserve(sizeof(FilterState) * n));
+    if (hipHostMalloc(reinterpret_cast<void**>(&sl.members_host), sizeof(Member) * n, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&sl.u0_host), sizeof(float) * n, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&sl.motions_host), sizeof(tsl_motion) * n, hipHostMallocDefault) !=
+            hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&sl.states_host), sizeof(FilterState) * n, hipHostMallocDefault) !=
+            hipSuccess)
+        return set_error(TSL_ERR_OUT_OF_MEMORY, "pinned host buffers");
+    TSL_HIP_CHECK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    return TSL_OK;
+}
+
+uint64_t filter_bytes(const tsl_filter* f)
+{
+    const DevBuf* bufs[] = {&f->particles, &f->particles_alt, &f->L, &f->raw, &f->partial, &f->points, &f->points_in,
+                            &f->perm, &f->sort_keys, &f->sort_vals, &f->sort_temp, &f->blocks, &f->state, &f->cumsum,
+                            &f->ubuf, &f->plan, &f->member};
+    uint64_t b = 0;
+    for (const DevBuf* d : bufs) b += d->bytes;
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsl_server_create(tsl_evaluator* ev, uint64_t hbm_budget_bytes, uint32_t max_robots, uint64_t max_particles,
+                      uint32_t max_batch, uint32_t chunk_len, tsl_server** out)
+{
+    if (!ev || !out) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (max_robots == 0 || max_particles == 0)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "max_robots and max_particles must be positive");
+    if (resample_bound(max_particles) == 0 || resample_bound(max_particles) > 0x7fffffffull)
+        return set_error(TSL_ERR_UNSUPPORTED, "max_particles too large");
+    if (max_batch == 0 || max_batch > kMaxMembers)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "max_batch must be in [1, %u]", kMaxMembers);
+    *out = nullptr;
+    TSL_HIP_CHECK(hipSetDevice(ev->device));
+    tsl_server* s = new (std::nothrow) tsl_server();
+    if (!s) return set_error(TSL_ERR_OUT_OF_MEMORY, "out of host memory");
+    s->ev = ev;
+    s->device = ev->device;
+    s->budget = hbm_budget_bytes;
+    s->max_batch = max_batch;
+    s->chunk_len = chunk_len ? (chunk_len + 63) / 64 * 64 : 2048;
+    s->max_particles = max_particles;
+    s->robots.resize(max_robots);
+    s->t0 = std::chrono::steady_clock::now();
+    uint64_t map_bytes = 0;
+    (void)tsl_map_device_bytes(ev->map, &map_bytes);
+    s->used = map_bytes;
+    int rc = TSL_OK;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = set_error(TSL_ERR_HIP, "hipStreamCreate failed");
+    for (int i = 0; i < kMaxInflight && rc == TSL_OK; ++i) {
+        rc = slot_init(s, s->slots[i]);
+        s->free_slots.push_back(&s->slots[i]);
+    }
+    if (rc == TSL_OK) {
+        // persistent grid of the batched evaluation: the occupancy of the
+        // kernel this map selects, on every compute unit
+        BatchLaunch probe;
+        probe.map = ev->map;
+        probe.K = 1;
+        probe.P = 1;
+        probe.max_items = ~0ull >> 8;
+        uint32_t g = 0;
+        rc = batch_grid(probe, &g);
+        if (rc == TSL_OK) {
+            int cus = 0;
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ev->device);
+            s->cus = cus > 0 ? cus : 1;
+            s->per_cu = (int)std::max<uint32_t>(1, g / (uint32_t)s->cus);
+        }
+    }
+    if (rc != TSL_OK) {
+        tsl_server_destroy(s);
+        return rc;
+    }
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return TSL_OK;
+}
+
+int tsl_server_destroy(tsl_server* s)
+{
+    if (!s) return TSL_OK;
+    if (s->worker.joinable()) {
+        s->stop = true;
+        s->cv_req.notify_all();
+        s->worker.join();
+    }
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (Robot& r : s->robots) {
+        if (r.f) tsl_filter_destroy(r.f);
+        if (r.staging) (void)hipHostFree(r.staging);
+    }
+    for (Slot& sl : s->slots) {
+        if (sl.members_host) (void)hipHostFree(sl.members_host);
+        if (sl.u0_host) (void)hipHostFree(sl.u0_host);
+        if (sl.motions_host) (void)hipHostFree(sl.motions_host);
+        if (sl.states_host) (void)hipHostFree(sl.states_host);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+    }
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return TSL_OK;
+}
+
+int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n, uint32_t max_points,
+                         const float tf[16], uint32_t* robot)
+{
+    if (!s || !robot || !tf || (n && !particles)) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n == 0 || max_points == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "a robot needs particles and a scan size");
+    if (n > s->max_particles)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "%zu particles exceed the server's %llu per robot", n,
+                         (unsigned long long)s->max_particles);
+    TSL_HIP_CHECK(hipSetDevice(s->device));
+    tsl_filter* f = nullptr;
+    TSL_TRY(tsl_filter_create(s->ev, n, &f));
+    int rc = tsl_filter_set_particles(f, particles, n);
+    const uint64_t cap = resample_bound(n);
+    if (rc == TSL_OK && (cap == 0 || cap > 0x7fffffffull))
+        rc = set_error(TSL_ERR_UNSUPPORTED, "systematic draw does not advance for %zu particles", n);
+    // everything a tick can touch, reserved now: ticks never allocate
+    if (rc == TSL_OK) rc = filter_reserve_particles(f, cap);
+    if (rc == TSL_OK) rc = filter_reserve_scan(f, max_points);
+    const size_t chunks = ((size_t)max_points + s->chunk_len - 1) / s->chunk_len;
+    if (rc == TSL_OK) rc = f->partial.reserve(sizeof(float) * std::max<size_t>(chunks, 1) * f->capacity);
+    if (rc == TSL_OK) rc = f->ubuf.reserve(sizeof(float) * f->capacity);
+    if (rc == TSL_OK) rc = f->plan.reserve(sizeof(RsPlan));
+    // the reduction grid of a tick spans its largest member
+    const uint64_t nb_max = (resample_bound(s->max_particles) + 255) / 256;
+    if (rc == TSL_OK) rc = f->blocks.reserve(sizeof(double) * 9 * nb_max);
+    tsl_point* staging = nullptr;
+    if (rc == TSL_OK && hipHostMalloc(reinterpret_cast<void**>(&staging), sizeof(tsl_point) * max_points,
+                                      hipHostMallocDefault) != hipSuccess)
+        rc = set_error(TSL_ERR_OUT_OF_MEMORY, "pinned scan staging");
+    if (rc == TSL_OK) std::memcpy(f->tf, tf, sizeof(float) * 16);
+    if (rc != TSL_OK) {
+        if (staging) (void)hipHostFree(staging);
+        tsl_filter_destroy(f);
+        return rc;
+    }
+    const uint64_t bytes = filter_bytes(f);
+    std::lock_guard<std::mutex> lk(s->mu);
+    uint32_t slot = 0;
+    while (slot < s->robots.size() && (s->robots[slot].active || s->robots[slot].busy)) ++slot;
+    if (slot == s->robots.size() || (s->budget && s->used + bytes > s->budget)) {
+        (void)hipHostFree(staging);
+        tsl_filter_destroy(f);
+        if (slot == s->robots.size())
+            return set_error(TSL_ERR_ADMISSION, "server holds its maximum of %zu robots", s->robots.size());
+        return set_error(TSL_ERR_ADMISSION, "robot needs %llu bytes; %llu of %llu in use",
+                         (unsigned long long)bytes, (unsigned long long)s->used, (unsigned long long)s->budget);
+    }
+    s->used += bytes;
+    Robot& r = s->robots[slot];
+    r.f = f;
+    r.max_points = max_points;
+    r.bytes = bytes;
+    r.active = true;
+    r.busy = false;
+    r.staging = staging;
+    f->count_bound = f->capacity;
+    f->count_known = false;
+    *robot = slot;
+    return TSL_OK;
+}
+
+int tsl_server_remove_robot(tsl_server* s, uint32_t robot)
+{
+    if (!s) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (robot >= s->robots.size() || !s->robots[robot].active)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "no robot %u", robot);
+    for (const Pending& p : s->pending)
+        if (p.req.robot == robot) return set_error(TSL_ERR_INVALID_ARGUMENT, "robot %u has pending requests", robot);
+    s->cv_done.wait(lk, [&] { return !s->robots[robot].busy; });
+    Robot& r = s->robots[robot];
+    r.active = false;
+    s->used -= r.bytes;
+    tsl_filter_destroy(r.f);
+    (void)hipHostFree(r.staging);
+    r.f = nullptr;
+    r.staging = nullptr;
+    return TSL_OK;
+}
+
+int tsl_server_submit(tsl_server* s, const tsl_request* req)
+{
+    if (!s || !req || (req->num_points && !req->points)) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    Pending p;
+    p.req = *req;
+    p.points.assign(req->points, req->points + req->num_points);
+    p.req.points = nullptr;
+    p.submit_ms = s->now_ms();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (req->robot >= s->robots.size() || !s->robots[req->robot].active)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "no robot %u", req->robot);
+    if (req->num_points > s->robots[req->robot].max_points)
+        return set_error(TSL_ERR_CAPACITY, "scan of %zu points exceeds the robot's %u", req->num_points,
+                         s->robots[req->robot].max_points);
+    if (req->num_points == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "empty scan");
+    s->pending.push_back(std::move(p));
+    s->cv_req.notify_one();
+    return TSL_OK;
+}
+
+int tsl_server_poll(tsl_server* s, tsl_result* out, size_t capacity, size_t* n, int wait_ms)
+{
+    if (!s || !n || (capacity && !out)) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && wait_ms > 0)
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !s->done.empty(); });
+    size_t k = 0;
+    while (k < capacity && !s->done.empty()) {
+        out[k++] = s->done.front();
+        s->done.pop_front();
+    }
+    *n = k;
+    if (s->worker_error != TSL_OK) return s->worker_error;
+    return TSL_OK;
+}
+
+int tsl_server_get_particles(tsl_server* s, uint32_t robot, tsl_particle* out, size_t capacity, size_t* n)
+{
+    if (!s || !n) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (robot >= s->robots.size() || !s->robots[robot].active)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "no robot %u", robot);
+    s->cv_done.wait(lk, [&] { return !s->robots[robot].busy; });
+    tsl_filter* f = s->robots[robot].f;
+    lk.unlock();
+    return tsl_filter_get_particles(f, out, capacity, n);
+}
+
+int tsl_server_stats(tsl_server* s, uint64_t* used_bytes, uint64_t* ticks, uint64_t* requests)
+{
+    if (!s) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (used_bytes) *used_bytes = s->used;
+    if (ticks) *ticks = s->ticks;
+    if (requests) *requests = s->batched_requests;
+    return TSL_OK;
+}
+
+}  // extern "C"

@@ -327,6 +327,7 @@ int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const floa
                          float* d_out, hipStream_t stream);
 int launch_prep_points(tsl_filter* f, hipStream_t stream);
 int launch_particle_order(tsl_filter* f, hipStream_t stream);
+int filter_reserve_scan(tsl_filter* f, size_t n);
 int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring, uint32_t n, float cell,
                        tsl_point* out, uint32_t* count_dev, uint32_t* bad_dev, hipStream_t stream);
 
