@@ -198,12 +198,123 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_serving(args):
+    """256 robots resident on the building map, requests (odometry step +
+    5k-20k point scan + draw) from a Poisson process of 200 requests/s:
+    p50/p99 latency (submission to result) under that load, and the
+    saturation throughput of the served form (every request submitted at
+    once) against plain per-request launches of the same requests."""
+    import tsdf_localization_amd as tsl
+    from tsdf_localization_amd import synth
+    from tsdf_localization_amd.server import Server
+
+    t0 = time.time()
+    w = synth.serving()
+    R = len(w.robots)
+    log(f"workload serving: {R} robots, mean {w.meta['mean_evals_per_request']:.3e} evals/request "
+        f"(generated in {time.time() - t0:.1f}s)")
+    ev = tsl.TSDFEvaluator(w.map)
+    rng = np.random.default_rng(4242)
+    n_req = args.requests
+    rate = args.rate
+    robots_of = rng.integers(0, R, n_req)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    motions = [tsl.evaluator.motion_from_delta(0.05, 0.0, 0.0, 0.0, 0.0, 0.01, 0.01, 0.005, seed=i)
+               for i in range(n_req)]
+    u0 = [np.float32(rng.uniform(0, 0.9 / w.robots[r].particles.shape[0])) for r in robots_of]
+    evals = [w.robots[r].particles.shape[0] * w.robots[r].points.shape[0] for r in robots_of]
+
+    def fresh_server():
+        srv = Server(ev, hbm_budget=args.hbm_budget_gb * (1 << 30), max_robots=R, max_particles=20_000,
+                     max_batch=args.max_batch, chunk_len=2048)
+        ids = [srv.add_robot(r.particles, 20_000, w.tf) for r in w.robots]
+        return srv, ids
+
+    # 1. latency under Poisson arrivals
+    srv, ids = fresh_server()
+    log(f"server: {srv.stats()['used_bytes'] / 2**30:.2f} GiB resident for {R} robots")
+    for i in range(min(16, n_req)):  # warm-up
+        srv.submit(ids[robots_of[i]], motions[i], w.robots[robots_of[i]].points, u0[i], request_id=i)
+    srv.wait(min(16, n_req))
+    start = time.perf_counter()
+    for i in range(n_req):
+        delay = arrivals[i] - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        srv.submit(ids[robots_of[i]], motions[i], w.robots[robots_of[i]].points, u0[i], request_id=i)
+    res = srv.wait(n_req, timeout_ms=600_000)
+    wall = time.perf_counter() - start
+    lat = np.array([r["done_ms"] - r["submit_ms"] for r in res])
+    bad = sum(r["status"] != 0 for r in res)
+    batch = np.array([r["batch"] for r in res])
+    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+    log(f"poisson {rate}/s: {n_req} requests in {wall:.2f} s, latency p50 {p50:.2f} ms p99 {p99:.2f} ms, "
+        f"mean batch {batch.mean():.1f}, failed {bad}")
+    srv.close()
+
+    # 2. saturation: every request at once
+    srv, ids = fresh_server()
+    t1 = time.perf_counter()
+    for i in range(n_req):
+        srv.submit(ids[robots_of[i]], motions[i], w.robots[robots_of[i]].points, u0[i], request_id=i)
+    res = srv.wait(n_req, timeout_ms=600_000)
+    served_s = time.perf_counter() - t1
+    served_rate = sum(evals) / served_s
+    sat_batch = float(np.mean([r["batch"] for r in res]))
+    log(f"served saturation: {served_s * 1e3:.1f} ms for {n_req} requests, {served_rate:.3e} evals/s, "
+        f"mean batch {sat_batch:.1f}")
+    srv.close()
+
+    # 3. plain per-request launches, back to back
+    filters = {}
+    stream = tsl.Stream()
+    for r in set(robots_of.tolist()):
+        f = tsl.DeviceFilter(ev, w.robots[r].particles.shape[0])
+        f.set_particles(w.robots[r].particles)
+        filters[r] = f
+    t2 = time.perf_counter()
+    for i in range(n_req):
+        f = filters[robots_of[i]]
+        f.motion_async(motions[i], stream)
+        f.set_scan(w.robots[robots_of[i]].points, w.tf)
+        f.update_async(stream)
+        f.resample_async(u0[i], stream)
+        f.sync(stream)
+    plain_s = time.perf_counter() - t2
+    plain_rate = sum(evals) / plain_s
+    log(f"plain per-request: {plain_s * 1e3:.1f} ms, {plain_rate:.3e} evals/s; served/plain {served_rate / plain_rate:.2f}x")
+    out = {
+        "metric": "particle*point evals/s",
+        "value": served_rate,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": n_req,
+        "warmup": 16,
+        "ms_per_step": served_s * 1e3 / n_req,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (building map, shared ray-cast scans, particles around each robot)",
+        "config": {"workload": "serving", "robots": R, "requests": n_req, "arrival_rate_per_s": rate,
+                   "max_batch": args.max_batch, "map": "sparse 1024x1024x128 @0.05 m int16"},
+        "latency_ms": {"p50": p50, "p99": p99, "mean_batch": float(batch.mean()), "failed": int(bad)},
+        "plain_value": plain_rate,
+        "speedup_vs_plain": served_rate / plain_rate,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="building", choices=["building", "room", "batched"])
+    ap.add_argument("--workload", default="building", choices=["building", "room", "batched", "serving"])
+    ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=200.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--max-batch", type=int, default=64, help="serving: requests per tick")
+    ap.add_argument("--hbm-budget-gb", type=int, default=64, help="serving: device bytes the server may hold")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample", type=int, default=20000)
@@ -227,6 +338,9 @@ def main():
 
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
         return
 
     import tsdf_localization_amd as tsl

@@ -280,3 +280,57 @@ def batched(seed=0x5EED0003, filters=64, size=(25.6, 25.6, 4.8), resolution=0.05
                             "particles_total": int(sum(r.particles.shape[0] for r in robots)),
                             "evals_per_step": int(sum(r.particles.shape[0] * r.points.shape[0] for r in robots))})
 
+
+@dataclass
+class ServingWorkload:
+    name: str
+    map: CudaSubVoxelMap
+    tf: np.ndarray
+    robots: list          # Robot per simulated robot (its scan: one of the shared scans)
+    scans: list           # distinct scans (robots reuse them)
+    meta: dict = field(default_factory=dict)
+
+
+def serving(seed=0x5EED0004, robots=256, particles=(2_000, 20_000), points=(5_000, 20_000), distinct_scans=32,
+            beams=64, azimuths=1024):
+    """Robots on the global-localisation building map: each with a particle
+    set (count log-uniform in `particles`, around its pose) and a scan of
+    5k-20k points; ray casts are shared between robots (distinct_scans poses,
+    robot i uses scan i mod distinct_scans) to keep generation short."""
+    w = building(num_particles=1)
+    m = w.map
+    rng = np.random.default_rng(seed)
+    sc = Scene()
+    # the same building as building(): rebuild the scene for ray casting
+    sx, sy, sz = 51.2, 51.2, 6.4
+    _s.synth_scene_build_building(sc.h, 0x5EED0002, sx, sy, 3.0, 10.0, 4, 0.3)
+    tf = tf_from_xyzrpy(**SCANNER_TF)
+    scans, poses = [], []
+    for i in range(distinct_scans):
+        N = int(rng.integers(points[0], points[1] + 1))
+        for _ in range(100):  # a free pose whose scan has at least N points
+            for _ in range(1000):
+                x, y = rng.uniform(2, sx - 2), rng.uniform(2, sy - 2)
+                if sc.sdf([[x, y, 1.0]])[0] > 0.8:
+                    break
+            pose = np.array([x, y, 1.0, 0.0, 0.0, rng.uniform(-math.pi, math.pi)])
+            raw, rings = sc.raycast(compose_pose(pose, tf), beams, azimuths, -22.5, 22.5, 60.0, 0.01, seed + i)
+            pts, rings = voxel_filter(raw, rings, 0.05, 1.0)
+            if pts.shape[0] >= N:
+                break
+        if pts.shape[0] > N:
+            pts = pts[np.sort(rng.choice(pts.shape[0], N, replace=False))]
+        scans.append(np.ascontiguousarray(pts, np.float32))
+        poses.append(pose)
+    out = []
+    for r in range(robots):
+        pose = poses[r % distinct_scans]
+        P = int(round(math.exp(rng.uniform(math.log(particles[0]), math.log(particles[1])))))
+        parts = particles_normal(seed + 7 * r, P, pose, [0.2, 0.2, 0.05, math.radians(5), math.radians(1),
+                                                         math.radians(5)])
+        out.append(Robot(parts, scans[r % distinct_scans], pose))
+    return ServingWorkload("serving", m, tf, out, scans,
+                           {"robots": robots, "distinct_scans": distinct_scans,
+                            "mean_evals_per_request": float(np.mean([r.particles.shape[0] * r.points.shape[0]
+                                                                     for r in out]))})
+
