@@ -317,11 +317,13 @@ def main():
     ap.add_argument("--hbm-budget-gb", type=int, default=64, help="serving: device bytes the server may hold")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--graph-step", action="store_true", help="also time the step replayed from a HIP graph")
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--tile-k", type=int, default=None, help="particles per wavefront (1, 2); default autotune")
     ap.add_argument("--tile-p", type=int, default=1, help="points per lane per iteration (1, 2, 4) with --tile-k")
     ap.add_argument("--order", type=int, default=-1, help="particle order with --tile-k: 0 caller, 1 spatial, -1 auto")
+    ap.add_argument("--lookup", type=int, default=-1, help="block lookup with --tile-k: 0 table, 1 LDS rank, -1 auto")
     ap.add_argument("--chunk", type=int, default=0, help="scan points per wavefront (0 = automatic)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run plain copies of this many MiB (counter calibration) and exit")
@@ -369,6 +371,7 @@ def main():
         order, k, pp, chunk = args.order, args.tile_k, args.tile_p, args.chunk
         f.set_tile(k, chunk, pp)
         f.set_order(order)
+        f.set_block_lookup(args.lookup)
     rng = np.random.default_rng(12345)
     u0s = [np.float32(rng.uniform(0.0, 1.0 / P)) for _ in range(args.warmup + args.steps)]
 
@@ -403,6 +406,27 @@ def main():
     log(f"{args.steps} steps in {total_ms:.2f} ms: {ms_per_step:.3f} ms/step, {value:.3e} evals/s; "
         f"eval kernel {kern_avg_ms:.3f} ms avg over {launches} launches ({achieved:.0f} GB/s algorithmic)")
 
+    # the same step replayed from a HIP graph (a batch of this one filter):
+    # reported beside the plain launches, which stay the measured value
+    graph_ms = None
+    if args.graph_step:
+        b = tsl.Batch(ev, [f])
+        b.set_tile(k if k in (1, 2) else 1, chunk, pp if pp in (1, 2, 4) else 1)
+        one = [np.array([u], np.float32) for u in u0s]
+        b.capture(stream, restore=True)
+        for i in range(args.warmup):
+            b.launch(one[i], stream)
+        stream.synchronize()
+        g0, g1 = tsl.Event(), tsl.Event()
+        g0.record(stream)
+        for i in range(args.steps):
+            b.launch(one[args.warmup + i], stream)
+        g1.record(stream)
+        stream.synchronize()
+        b.sync(stream)
+        graph_ms = g0.elapsed_ms(g1) / args.steps
+        log(f"graph-replayed step: {graph_ms:.3f} ms/step, {evals / graph_ms * 1e3:.3e} evals/s")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -436,6 +460,7 @@ def main():
                      "kernel": "k_eval", "kernel_avg_ms": kern_avg_ms,
                      "bytes_per_eval": BYTES_PER_EVAL, "evals_per_launch": pairs // max(launches, 1)},
         "cpu_baseline": cpu,
+        "graph_step_ms": graph_ms,
         "resampled_count": n_after,
     }
     print(json.dumps(out), flush=True)

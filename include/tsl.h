@@ -280,6 +280,13 @@ int tsl_filter_set_tile(tsl_filter* f, int k, int points_per_lane, uint32_t chun
  * reference. */
 int tsl_filter_set_order(tsl_filter* f, int mode);
 
+/* How the evaluation finds a block's voxel offset: 0 = the block table in
+ * device memory (one gather), 1 = an occupancy bitmap with running counts
+ * staged in LDS (no gather; needs offsets in setData order and a table of
+ * at most 131072 blocks), -1 = automatic.  Results are identical.  No
+ * counterpart in the reference. */
+int tsl_filter_set_block_lookup(tsl_filter* f, int mode);
+
 /* Time the candidate evaluation shapes (particle order x tile) on the
  * filter's current particles and scan (reps launches each, on `stream`,
  * synchronously) and keep the fastest; reports it through the non-NULL out

@@ -133,12 +133,13 @@ def test_ragged_sizes(tsl, room, room_oracle, P, N):
     assert np.max(rel_err(raw, w64)) <= WEIGHT_RTOL
 
 
-def _filter_weights(tsl, ev, parts, pts, tf, k, pp, chunk, order=0):
+def _filter_weights(tsl, ev, parts, pts, tf, k, pp, chunk, order=0, lookup=0):
     f = tsl.DeviceFilter(ev, parts.shape[0])
     f.set_particles(parts)
     f.set_scan(pts, tf)
     f.set_tile(k, chunk, pp)
     f.set_order(order)
+    f.set_block_lookup(lookup)
     s = tsl.Stream()
     f.update_async(s)
     f.sync(s)
@@ -182,15 +183,16 @@ def test_tile_shapes_bit_identical(tsl, room, room_oracle, scene):
     ref_norm = w64 / w64.sum()
     for chunk in (320, 0):
         first = None
-        for order in (0, 1):
+        for order, lookup in ((0, 0), (1, 0), (0, 1), (1, 1)):
             for k in (1, 2):
                 for pp in (1, 2, 4):
-                    w = _filter_weights(tsl, ev, parts, pts, tf, k, pp, chunk, order)
-                    assert np.max(rel_err(w, ref_norm)) <= WEIGHT_RTOL, (order, k, pp, chunk)
+                    w = _filter_weights(tsl, ev, parts, pts, tf, k, pp, chunk, order, lookup)
+                    key = (order, lookup, k, pp, chunk)
+                    assert np.max(rel_err(w, ref_norm)) <= WEIGHT_RTOL, key
                     if first is None:
                         first = w
                     else:
-                        assert np.array_equal(w.view(np.uint32), first.view(np.uint32)), (order, k, pp, chunk)
+                        assert np.array_equal(w.view(np.uint32), first.view(np.uint32)), key
 
 
 def test_per_point_tile_same_result(tsl, room, room_oracle):

@@ -103,18 +103,18 @@ def test_server_admission_by_budget(tsl, room):
     probe = Server(ev, max_robots=4, max_particles=1000, max_batch=4)
     try:
         base = probe.stats()["used_bytes"]
-        r = probe.add_robot(parts, 2000, room.tf)
+        r = probe.add_robot(parts, 3000, room.tf)
         per_robot = probe.stats()["used_bytes"] - base
     finally:
         probe.close()
     srv = Server(ev, hbm_budget=base + 2 * per_robot + per_robot // 2, max_robots=8, max_particles=1000, max_batch=4)
     try:
-        a = srv.add_robot(parts, 2000, room.tf)
-        b = srv.add_robot(parts, 2000, room.tf)
+        a = srv.add_robot(parts, 3000, room.tf)
+        b = srv.add_robot(parts, 3000, room.tf)
         with pytest.raises(tsl.TslError, match="bytes"):
-            srv.add_robot(parts, 2000, room.tf)
+            srv.add_robot(parts, 3000, room.tf)
         srv.remove_robot(a)
-        c = srv.add_robot(parts, 2000, room.tf)
+        c = srv.add_robot(parts, 3000, room.tf)
         assert c == a
         srv.submit(b, _motion(tsl, 1), room.points, np.float32(0.1 / 500), request_id=7)
         srv.submit(c, _motion(tsl, 2), room.points, np.float32(0.2 / 500), request_id=8)

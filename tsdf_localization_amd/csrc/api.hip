@@ -350,6 +350,34 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
             magic_divisor((uint32_t)(sd * sd), &d.mag_sd2, &d.sh_sd2);
             magic_divisor((uint32_t)sd3, &d.mag_sd3, &d.sh_sd3);
         }
+        // in-block address contribution of each axis coordinate k in [0, sd]
+        // (k = sd only on carry lanes, which recompute their address)
+        std::vector<uint32_t> ctab(3 * (sd + 1));
+        for (uint64_t k = 0; k <= sd; ++k) {
+            const uint64_t kk = k < sd ? k : 0;
+            if (brick) {
+                ctab[k] = (uint32_t)((kk & 3) + 64 * (kk >> 2));
+                ctab[(sd + 1) + k] = (uint32_t)(4 * (kk & 3) + 64 * B * (kk >> 2));
+                ctab[2 * (sd + 1) + k] = (uint32_t)(16 * (kk & 3) + 64 * B * B * (kk >> 2));
+            } else {
+                ctab[k] = (uint32_t)kk;
+                ctab[(sd + 1) + k] = (uint32_t)(kk * sd);
+                ctab[2 * (sd + 1) + k] = (uint32_t)(kk * sd * sd);
+            }
+        }
+        TSL_TRY(m->contrib.reserve(sizeof(uint32_t) * ctab.size()));
+        TSL_HIP_CHECK(hipMemcpy(m->contrib.ptr, ctab.data(), sizeof(uint32_t) * ctab.size(), hipMemcpyHostToDevice));
+        d.contrib = m->contrib.as<uint32_t>();
+        d.contrib_n = (uint32_t)ctab.size();
+        // smallest fraction in [0, 1) whose in-block coordinate reaches sd
+        d.carry_bits = 0x7f800000u;
+        for (uint32_t bits = 0x3f7fffffu;; --bits) {
+            float f;
+            std::memcpy(&f, &bits, 4);
+            if (cvt_f32_u64(f / c.resolution) < sd) break;
+            d.carry_bits = bits;
+            if (bits == 0) break;
+        }
     }
     return TSL_OK;
 }
@@ -688,6 +716,14 @@ int tsl_filter_set_tile(tsl_filter* f, int k, int points_per_lane, uint32_t chun
     f->tile_k = k;
     f->tile_p = points_per_lane;
     f->tile_chunk = chunk_len;
+    return TSL_OK;
+}
+
+int tsl_filter_set_block_lookup(tsl_filter* f, int mode)
+{
+    if (!f) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL filter");
+    if (mode < -1 || mode > 1) return set_error(TSL_ERR_INVALID_ARGUMENT, "block lookup mode must be -1, 0 or 1");
+    f->rank_mode = mode;
     return TSL_OK;
 }
 

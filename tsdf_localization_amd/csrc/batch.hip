@@ -17,6 +17,7 @@ struct tsl_batch {
     int device = 0;
     std::vector<tsl_filter*> filters;
     int order = 0, K = 1, P = 1;
+    bool rank = true;        // block offsets from the LDS occupancy-rank table when the map has one
     uint32_t chunk_len = 0;  // 0: each member's own automatic choice
     DevBuf members, plan, u0_dev, states, motions;
     std::vector<Member> host_members;
@@ -126,6 +127,7 @@ int prepare(tsl_batch* b)
     l.max_items = max_items;
     l.max_cap = (uint32_t)max_cap;
     l.max_rs = (uint32_t)max_cap;
+    l.rank = b->rank;
     TSL_TRY(batch_grid(l, &l.grid));
     for (tsl_filter* f : b->filters) {
         f->count_bound = f->capacity;  // the device count is authoritative from here on

@@ -22,6 +22,7 @@ namespace {
 constexpr int kBlock = 256;           // threads per workgroup (4 wavefronts)
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kXcds = 8;         // accelerator complex dies of the MI355X (one L2 each)
+constexpr uint32_t kTile = 512;       // scan points staged in LDS per step (8 KB), a multiple of 64 * 4
 
 __device__ __forceinline__ float wave_sum(float v)
 {
@@ -111,6 +112,7 @@ struct LdsTables {
     const uint64_t* neg;
     const SubEntry* tab;
     const float* lut;
+    const uint32_t* contrib;
 };
 
 template <bool kRank>
@@ -121,6 +123,7 @@ __device__ __forceinline__ LdsTables stage_tables(const DevMap& map)
     uint64_t* neg = reinterpret_cast<uint64_t*>(occ + (kRank ? map.occ_nwords : 0u));
     SubEntry* tab = reinterpret_cast<SubEntry*>(neg + 8);
     float* lut = reinterpret_cast<float*>(tab + map.sub_entries);
+    uint32_t* contrib = reinterpret_cast<uint32_t*>(lut + map.palette_size);
     if (kRank) {
         const uint4* src = reinterpret_cast<const uint4*>(map.occ_words);
         uint4* dst = reinterpret_cast<uint4*>(occ);
@@ -129,15 +132,16 @@ __device__ __forceinline__ LdsTables stage_tables(const DevMap& map)
     if (threadIdx.x < 8) neg[threadIdx.x] = map.neg_tab[threadIdx.x];
     for (uint32_t i = threadIdx.x; i < map.sub_entries; i += blockDim.x) tab[i] = map.sub_tab[i];
     for (uint32_t i = threadIdx.x; i < map.palette_size; i += blockDim.x) lut[i] = map.palette[i];
+    for (uint32_t i = threadIdx.x; i < map.contrib_n; i += blockDim.x) contrib[i] = map.contrib[i];
     __syncthreads();
-    return LdsTables{occ, neg, tab, lut};
+    return LdsTables{occ, neg, tab, lut, contrib};
 }
 
 // LDS bytes stage_tables needs for a map.
-inline size_t lds_bytes(const DevMap& d)
+inline size_t lds_bytes(const DevMap& d, bool rank)
 {
-    return (d.occ_nwords ? sizeof(OccWord) * d.occ_nwords : 0) + 8 * sizeof(uint64_t) +
-           sizeof(SubEntry) * d.sub_entries + sizeof(float) * d.palette_size;
+    return (rank ? sizeof(OccWord) * d.occ_nwords : 0) + 8 * sizeof(uint64_t) +
+           sizeof(SubEntry) * d.sub_entries + sizeof(float) * d.palette_size + sizeof(uint32_t) * d.contrib_n;
 }
 
 // One wavefront: K particles (pose matrices wavefront-uniform) x the points
@@ -154,9 +158,10 @@ inline size_t lds_bytes(const DevMap& d)
 // partial_row[p0 + k].
 template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank>
 __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t, const float* __restrict__ L,
-                                          uint32_t P, uint32_t p0, const float4* __restrict__ pts, uint32_t N,
-                                          uint32_t j0, uint32_t j1, float a_hit, float* __restrict__ partial_row,
-                                          void* __restrict__ trace_out, const uint32_t* __restrict__ perm)
+                                          uint32_t P, uint32_t p0, bool active, const float4* __restrict__ pts,
+                                          uint32_t N, uint32_t j0, uint32_t j1, float a_hit,
+                                          float* __restrict__ partial_row, void* __restrict__ trace_out,
+                                          const uint32_t* __restrict__ perm, float4* __restrict__ tile)
 {
     constexpr bool kTrace = kMode != 0;
     const OccWord* occ = t.occ;
@@ -167,7 +172,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
     float m[K][12];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const uint32_t p = min(p0 + k, P - 1);
+        const uint32_t p = active ? min(p0 + k, P - 1) : 0u;
 #pragma unroll
         for (int e = 0; e < 12; ++e) m[k][e] = L[(size_t)p * 12 + e];
     }
@@ -175,14 +180,22 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0f;
 
-    for (uint32_t base = j0; base < j1; base += 64 * kP) {
+    // The workgroup's waves walk the same chunk: its points are staged in
+    // LDS one tile at a time (every wave helps and waits, active or not).
+    for (uint32_t tb = j0; tb < j1; tb += kTile) {
+    const uint32_t te = min(j1, tb + kTile);
+    __syncthreads();  // the previous tile is consumed
+    for (uint32_t i = threadIdx.x; i < te - tb; i += blockDim.x) tile[i] = pts[tb + i];
+    __syncthreads();
+    if (!active) continue;
+    for (uint32_t base = tb; base < te; base += 64 * kP) {
         float4 q[kP];
         bool act[kP];
 #pragma unroll
         for (int u = 0; u < kP; ++u) {
             const uint32_t j = base + 64u * u + lane;
-            act[u] = j < j1;
-            q[u] = pts[act[u] ? j : j0];
+            act[u] = j < te;
+            q[u] = tile[act[u] ? j - tb : 0u];
         }
         float ox[kP][K], oy[kP][K], oz[kP][K];
         bool fast = true;
@@ -209,15 +222,24 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     int32_t s;
-                    uint32_t small, pbrick;
+                    uint32_t pc, kx, ky, kz;
                     bool carry;
-                    fast_block<kU24, kBrick, kRank>(map, tab, occ, ox[u][k], oy[u][k], oz[u][k], s, small, pbrick,
-                                                    carry);
-                    const uint32_t sum = (uint32_t)s + small;
-                    const bool valid = (s >= 0) & (sum < map.ds);
-                    if (kTrace) full[u][k] = s >= 0 ? (uint64_t)(uint32_t)s + small : map.data_size;
-                    uint32_t a = (uint32_t)s + pbrick;
-                    if (kBrick && __builtin_expect(carry, 0)) a = phys_from_linear(map, sum);
+                    fast_lookup<kU24, kRank>(map, tab, occ, t.contrib, ox[u][k], oy[u][k], oz[u][k], s, pc, kx, ky,
+                                             kz, carry);
+                    // a whole block row lies inside the array, so only a lane
+                    // whose index leaves its row (carry) can fall past data_size
+                    uint32_t a = (uint32_t)s + pc;
+                    bool valid = s >= 0;
+                    if (kTrace || __builtin_expect(carry, 0)) {
+                        const uint32_t small = kx + mul_u<kU24>(ky, (uint32_t)map.sub_dim) +
+                                               mul_u<kU24>(kz, (uint32_t)map.sub_dim_2);
+                        const uint32_t sum = (uint32_t)s + small;
+                        if (carry) {
+                            valid = valid & (sum < map.ds);
+                            a = kBrick ? phys_from_linear(map, sum) : sum;
+                        }
+                        if (kTrace) full[u][k] = s >= 0 ? (uint64_t)(uint32_t)s + small : map.data_size;
+                    }
                     addr[u][k] = valid ? a : map.sentinel;
                 }
             }
@@ -259,8 +281,8 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll 1
             for (int u = 0; u < kP; ++u) {
                 const uint32_t j = base + 64u * u + lane;
-                const bool a = j < j1;
-                const float4 qq = pts[a ? j : j0];
+                const bool a = j < te;
+                const float4 qq = tile[a ? j - tb : 0u];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const float tx = m[k][0] * qq.x + m[k][1] * qq.y + m[k][2] * qq.z + m[k][3];
@@ -284,7 +306,8 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
             }
         }
     }
-    if (kTrace) return;
+    }
+    if (kTrace || !active) return;
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = wave_sum(acc[k]);
     if (lane == 0) {
@@ -315,12 +338,13 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
     const uint32_t item = xcd * q + min(xcd, rem) + r;
     const uint32_t g = item / chunks, c = item - g * chunks;
     const uint32_t p0 = (g * kWaves + wave) * K;
-    if (p0 >= P) return;
+    if (g * kWaves * K >= P) return;  // the whole workgroup is idle (uniform across its waves)
     const uint32_t j0 = c * chunk_len;
     const uint32_t j1 = min(N, j0 + chunk_len);
-    eval_wave<K, kP, kMode, kU24, kBrick, kRank>(map, t, L, P, p0, pts, N, j0, j1, a_hit,
+    __shared__ float4 tile[kTile];
+    eval_wave<K, kP, kMode, kU24, kBrick, kRank>(map, t, L, P, p0, p0 < P, pts, N, j0, j1, a_hit,
                                                  kMode == 0 ? partial + (size_t)c * stride : nullptr,
-                                                 trace_out, perm);
+                                                 trace_out, perm, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -500,6 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member*
 {
     const LdsTables t = stage_tables<kRank>(map);
     __shared__ uint32_t pre[kMaxMembers + 1];
+    __shared__ float4 tile[kTile];
     const uint32_t n = plan->n;
     for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) pre[i] = plan->prefix[i];
     __syncthreads();
@@ -523,11 +548,11 @@ __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member*
         const uint32_t g = local / m.chunks, c = local - g * m.chunks;
         const uint32_t P = m.st->count;
         const uint32_t p0 = (g * kWaves + wave) * K;
-        if (p0 >= P) continue;
+        if (g * kWaves * K >= P) continue;  // uniform across the workgroup
         const uint32_t j0 = c * m.chunk_len;
         const uint32_t j1 = min(m.N, j0 + m.chunk_len);
-        eval_wave<K, kP, 0, kU24, kBrick, kRank>(map, t, m.L, P, p0, m.points, m.N, j0, j1, a_hit,
-                                                 m.partial + (size_t)c * m.stride, nullptr, nullptr);
+        eval_wave<K, kP, 0, kU24, kBrick, kRank>(map, t, m.L, P, p0, p0 < P, m.points, m.N, j0, j1, a_hit,
+                                                 m.partial + (size_t)c * m.stride, nullptr, nullptr, tile);
     }
 }
 
@@ -562,6 +587,7 @@ Tile choose_tile(const tsl_filter* f, size_t P, size_t N)
     t.K = f->tile_k ? f->tile_k : 1;
     t.P = f->tile_p ? f->tile_p : 1;
     t.order = f->order_mode >= 0 ? f->order_mode : (P >= 8192 ? 1 : 0);
+    t.rank = f->rank_mode >= 0 ? f->rank_mode : 0;
     const size_t groups = (P + t.K - 1) / t.K;
     size_t len;
     if (f->tile_chunk) {
@@ -634,8 +660,8 @@ int launch_eval_kernel(const Tile& tile, const tsl_filter* f, size_t P, size_t N
     const size_t items = (groups + kWaves - 1) / kWaves * tile.chunks;
     if (items > 0x7fffffffull) return set_error(TSL_ERR_UNSUPPORTED, "evaluation grid too large");
     const dim3 grid((unsigned)items);
-    const bool rank = map->dev.occ_nwords != 0;
-    const size_t lds = lds_bytes(map->dev);
+    const bool rank = tile.rank && map->dev.occ_nwords != 0;
+    const size_t lds = lds_bytes(map->dev, rank);
     const bool u24 = map->dev.up_dim_x < (1u << 23) && map->dev.up_dim_2 < (1u << 23) &&
                      map->dev.sub_dim_2 < (1 << 23);
     const bool brick = map->dev.brick != 0;
@@ -736,22 +762,25 @@ int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_order, 
     const size_t P = f->count_bound;
     const size_t N = f->num_points;
     if (P == 0 || N == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "autotune needs particles and a scan");
-    const int saved_k = f->tile_k, saved_p = f->tile_p, saved_order = f->order_mode;
+    const int saved_k = f->tile_k, saved_p = f->tile_p, saved_order = f->order_mode, saved_rank = f->rank_mode;
     const uint32_t saved_chunk = f->tile_chunk;
     hipEvent_t a, b;
     TSL_HIP_CHECK(hipEventCreate(&a));
     TSL_HIP_CHECK(hipEventCreate(&b));
     float best = 1e30f;
-    int bo = 0, bk = 1, bp = 1;
+    int bo = 0, br = 0, bk = 1, bp = 1;
     uint32_t bc = 0;
     const int ks[2] = {1, 2};
     const int ps[3] = {1, 2, 4};
     const uint32_t div[5] = {0, 1, 2, 4, 8};
     int rc = TSL_OK;
+    const int ranks = f->ev->map->dev.occ_nwords ? 2 : 1;
     for (int order = 0; order < 2 && rc == TSL_OK; ++order) {
         f->order_mode = order;
         Tile pt = choose_tile(f, P, N);
         rc = launch_pose_for_tile(f, pt, stream);
+        for (int rk = 0; rk < ranks && rc == TSL_OK; ++rk) {
+        f->rank_mode = rk;
         for (int ki = 0; ki < 2 && rc == TSL_OK; ++ki) {
             for (int pi = 0; pi < 3 && rc == TSL_OK; ++pi) {
                 for (int ci = 0; ci < 5 && rc == TSL_OK; ++ci) {
@@ -774,12 +803,14 @@ int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_order, 
                     if (ms < best) {
                         best = ms;
                         bo = order;
+                        br = rk;
                         bk = ks[ki];
                         bp = ps[pi];
                         bc = f->tile_chunk;
                     }
                 }
             }
+        }
         }
     }
     (void)hipEventDestroy(a);
@@ -789,9 +820,11 @@ int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_order, 
         f->tile_p = saved_p;
         f->tile_chunk = saved_chunk;
         f->order_mode = saved_order;
+        f->rank_mode = saved_rank;
         return rc;
     }
     f->order_mode = bo;
+    f->rank_mode = br;
     f->tile_k = bk;
     f->tile_p = bp;
     f->tile_chunk = bc;
@@ -875,7 +908,7 @@ namespace {
 template <int K, int kP, bool kU24, bool kBrick, bool kRank>
 int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid_out)
 {
-    const size_t lds = lds_bytes(b.map->dev);
+    const size_t lds = lds_bytes(b.map->dev, kRank);
     if (grid_out) {
         int per_cu = 0;
         TSL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -902,7 +935,7 @@ int launch_eval_batch_flags(const BatchLaunch& b, hipStream_t stream, uint32_t* 
 {
     const DevMap& d = b.map->dev;
     const bool u24 = d.up_dim_x < (1u << 23) && d.up_dim_2 < (1u << 23) && d.sub_dim_2 < (1 << 23);
-    const bool brick = d.brick != 0, rank = d.occ_nwords != 0;
+    const bool brick = d.brick != 0, rank = b.rank && d.occ_nwords != 0;
     if (u24) {
         if (brick) return rank ? launch_eval_batch_t<K, kP, true, true, true>(b, stream, grid_out)
                                : launch_eval_batch_t<K, kP, true, true, false>(b, stream, grid_out);

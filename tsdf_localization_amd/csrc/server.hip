@@ -69,6 +69,7 @@ struct tsl_server {
     uint64_t budget = 0, used = 0;
     uint32_t max_batch = 64;
     uint32_t chunk_len = 2048;
+    int K = 2;                     // particles per wavefront of the batched evaluation
     uint64_t max_particles = 0;    // per robot; sizes the shared reduction grid
     std::vector<Robot> robots;     // fixed size (max_robots): entries never move
     std::mutex mu;
@@ -165,7 +166,7 @@ int dispatch(tsl_server* s, Slot* sl)
         h.cap_limit = (uint32_t)f->capacity;
         sl->u0_host[i] = pd.req.u0;
         sl->motions_host[i] = pd.req.motion;
-        max_items += (f->capacity + 3) / 4 * chunks;
+        max_items += (f->capacity + 4 * s->K - 1) / (4 * s->K) * chunks;
         max_cap = std::max<uint32_t>(max_cap, (uint32_t)f->capacity);
     }
     TSL_HIP_CHECK(hipMemcpyAsync(sl->members.ptr, sl->members_host, sizeof(Member) * n, hipMemcpyHostToDevice, st));
@@ -177,8 +178,9 @@ int dispatch(tsl_server* s, Slot* sl)
     l.members = sl->members.as<Member>();
     l.plan = sl->plan.as<BatchPlan>();
     l.n = n;
-    l.K = 1;
+    l.K = s->K;
     l.P = 1;
+    l.rank = true;
     l.max_items = max_items;
     l.max_cap = max_cap;
     l.max_rs = max_cap;
@@ -338,8 +340,9 @@ int tsl_server_create(tsl_evaluator* ev, uint64_t hbm_budget_bytes, uint32_t max
         // kernel this map selects, on every compute unit
         BatchLaunch probe;
         probe.map = ev->map;
-        probe.K = 1;
+        probe.K = s->K;
         probe.P = 1;
+        probe.rank = true;
         probe.max_items = ~0ull >> 8;
         uint32_t g = 0;
         rc = batch_grid(probe, &g);

@@ -138,6 +138,11 @@ struct DevMap {
     uint32_t brick;                   // 1: bricked codes
     uint32_t sd3;                     // sub_dim^3
     uint32_t brick_cy, brick_cz;      // 64B - 16, 64B^2 - 64
+    // per-axis in-block contribution of in-block coordinate k (0..sub_dim)
+    // to the physical voxel address: contrib[a * (sub_dim + 1) + k]
+    const uint32_t* contrib;
+    uint32_t contrib_n;               // 3 * (sub_dim + 1)
+    uint32_t carry_bits;              // bits of the smallest fraction whose in-block coordinate is sub_dim (+inf if none)
     uint32_t mag_sd, sh_sd, mag_sd2, sh_sd2, mag_sd3, sh_sd3;  // x / d = umulhi(x, mag) >> sh, x < 2^31
     // occupancy-rank form of the block table (set when the offsets follow
     // setData order and the bitmap fits the LDS budget)
@@ -242,7 +247,7 @@ inline uint64_t resample_bound(uint64_t M)
 struct tsl_map {
     int device = 0;
     tsl_map_coef coef{};
-    tsl::DevBuf grid_occ, codes, palette, sub_tab, neg_tab, full, occ_words;
+    tsl::DevBuf grid_occ, codes, palette, sub_tab, neg_tab, full, occ_words, contrib;
     tsl::DevMap dev{};
 };
 
@@ -291,6 +296,7 @@ struct tsl_filter {
     tsl::DevBuf pinv;        // uint32: slot of particle i
     int order_mode = -1;     // particle evaluation order: 0 caller, 1 spatial, -1 automatic
     int last_order = 0;      // order of the last evaluation (partial weights by slot)
+    int rank_mode = -1;      // block offsets: 0 global table, 1 LDS rank table, -1 automatic
     tsl::DevBuf blocks;      // double block partials
     tsl::DevBuf state;       // FilterState
     tsl::DevBuf cumsum;      // double[capacity]
@@ -320,6 +326,7 @@ struct Tile {
     int K;               // particles per wavefront
     int P;               // points per lane per iteration
     int order;           // 1: particles evaluated in spatial order
+    int rank;            // 1: block offsets from the LDS occupancy-rank table (when the map has one)
     uint32_t chunk_len;
     uint32_t chunks;
 };
@@ -353,6 +360,7 @@ struct BatchLaunch {
     uint32_t max_cap = 0;        // largest member capacity (grid bound of the per-particle kernels)
     uint32_t max_rs = 0;         // largest resampling output room
     uint32_t grid = 0;           // persistent evaluation grid (multiple of the XCD count)
+    bool rank = false;           // block offsets from the LDS occupancy-rank table
 };
 int launch_batch_update(const BatchLaunch& b, hipStream_t stream);
 int batch_grid(const BatchLaunch& b, uint32_t* grid);

@@ -171,6 +171,33 @@ __device__ __forceinline__ void fast_block(const DevMap& m, const SubEntry* __re
     }
 }
 
+// Fast tier for one lane, address form: block offset s (or -1), the sum pc
+// of the three per-axis in-block address contributions (LDS table indexed by
+// the in-block coordinate), the coordinates themselves, and carry when some
+// coordinate equals sub_dim (the lane's reference index then leaves the
+// block row, and its address is recomputed from the linear index).  o >= 0
+// on every axis here, so trunc is a plain conversion and the fraction
+// o - trunc(o) is exactly fract(o).
+template <bool kU24, bool kRank>
+__device__ __forceinline__ void fast_lookup(const DevMap& m, const SubEntry* __restrict__ tab,
+                                            const OccWord* __restrict__ occ, const uint32_t* __restrict__ contrib,
+                                            float ox, float oy, float oz, int32_t& s, uint32_t& pc, uint32_t& kx,
+                                            uint32_t& ky, uint32_t& kz, bool& carry)
+{
+    const uint32_t ix = (uint32_t)ox, iy = (uint32_t)oy, iz = (uint32_t)oz;
+    const float fx = __builtin_amdgcn_fractf(ox), fy = __builtin_amdgcn_fractf(oy), fz = __builtin_amdgcn_fractf(oz);
+    kx = sub_index(m.sub_base, m.sub_shift, tab, fx);
+    ky = sub_index(m.sub_base, m.sub_shift, tab, fy);
+    kz = sub_index(m.sub_base, m.sub_shift, tab, fz);
+    const uint32_t n1 = (uint32_t)m.sub_dim + 1u;
+    pc = contrib[kx] + contrib[n1 + ky] + contrib[2u * n1 + kz];
+    carry = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz)) >= m.carry_bits;
+    const bool grid = (ix < m.gthr_x) & (iy < m.gthr_y) & (iz < m.gthr_z);
+    const uint32_t up = ix + mul_u<kU24>(iy, m.up_dim_x) + mul_u<kU24>(iz, m.up_dim_2);
+    s = -1;
+    if (grid & (up < m.occ)) s = block_offset<kRank>(m, occ, up);
+}
+
 // Wrap tier for one lane (|o| < R on every axis): the reference's index,
 // data_size when rejected.  neg is the LDS copy of the wrap-band table.
 template <bool kU24, bool kRank>

@@ -367,6 +367,10 @@ class DeviceFilter:
     def set_tile(self, k=0, chunk_len=0, points_per_lane=0):
         check(lib.tsl_filter_set_tile(self.handle, int(k), int(points_per_lane), int(chunk_len)))
 
+    def set_block_lookup(self, mode=-1):
+        """Block offsets: 0 global table, 1 LDS occupancy-rank table, -1 automatic."""
+        check(lib.tsl_filter_set_block_lookup(self.handle, int(mode)))
+
     def set_order(self, mode=-1):
         """Particle evaluation order: 0 caller, 1 spatial, -1 automatic."""
         check(lib.tsl_filter_set_order(self.handle, int(mode)))
