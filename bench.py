@@ -159,7 +159,7 @@ def bench_batched(args):
     filters = make_filters()
     b = tsl.Batch(ev, filters)
     results = {}
-    for k, chunk in ((1, 2048), (1, 4096), (2, 2048), (2, 4096), (1, 0)):
+    for k, chunk in ((1, 2048), (2, 2048), (2, 8192), (1, 0), (2, 0)):
         b.set_tile(k, chunk, 1)
         ms = timed(lambda i: b.step_async(draws[i], restore=True, stream=stream))
         b.sync(stream)
@@ -226,7 +226,7 @@ def bench_serving(args):
 
     def fresh_server():
         srv = Server(ev, hbm_budget=args.hbm_budget_gb * (1 << 30), max_robots=R, max_particles=20_000,
-                     max_batch=args.max_batch, chunk_len=2048)
+                     max_batch=args.max_batch, chunk_len=args.chunk or 2048)
         ids = [srv.add_robot(r.particles, 20_000, w.tf) for r in w.robots]
         return srv, ids
 

@@ -348,8 +348,10 @@ int tsl_batch_create(tsl_evaluator* ev, tsl_filter* const* filters, size_t n, ts
 int tsl_batch_destroy(tsl_batch* b);
 
 /* Tile of the batched evaluation (k particles per wavefront, 1 or 2; points
- * per lane 1, 2 or 4; chunk_len points per chunk, 0 = each member's
- * automatic choice).  Re-prepares the batch. */
+ * per lane 1, 2 or 4; chunk_len points per chunk, 0 = chosen per member
+ * against the whole batch: the whole scan unless the batch is too small to
+ * fill the GPU that way).  The persistent grid splits the batch's
+ * particle-point pairs evenly over its workgroups.  Re-prepares the batch. */
 int tsl_batch_set_tile(tsl_batch* b, int k, int points_per_lane, uint32_t chunk_len);
 
 /* Rebuild the members' device descriptors and reserve their buffers. */

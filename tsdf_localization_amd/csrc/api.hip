@@ -317,18 +317,19 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     d.sentinel_code = init_code;
     {
         // occupancy-rank form of the block table
-        const uint64_t nwords = (c.grid_occ_size + 63) / 64;
-        bool rank = nwords <= kMaxOccWords && sd3 < (1ull << 31);
+        // (rank < 2^17 and sub_dim^3 < 2^24 keep the device product a
+        // 24-bit multiply; rank * sub_dim^3 < 2^31 as a block offset)
+        const uint64_t nwords = (c.grid_occ_size + 31) / 32;
+        bool rank = nwords <= kMaxOccWords && sd3 < (1ull << 24);
         std::vector<OccWord> words;
         if (rank) {
-            words.assign(nwords, OccWord{0u, 0u, 0u, 0u});
+            words.assign(nwords, OccWord{0u, 0u});
             uint64_t n = 0;
             for (uint64_t b = 0; rank && b < c.grid_occ_size; ++b) {
-                if (b % 64 == 0) words[b / 64].prefix = (uint32_t)n;
+                if (b % 32 == 0) words[b / 32].prefix = (uint32_t)n;
                 if (grid_occ[b] < 0) continue;
-                if ((uint64_t)grid_occ[b] != n * sd3) rank = false;
-                if (b % 64 < 32) words[b / 64].lo |= 1u << (b % 32);
-                else words[b / 64].hi |= 1u << (b % 32);
+                if ((uint64_t)grid_occ[b] != n * sd3 || n * sd3 >= (1ull << 31)) rank = false;
+                words[b / 32].bits |= 1u << (b % 32);
                 ++n;
             }
         }
@@ -365,10 +366,18 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
                 ctab[2 * (sd + 1) + k] = (uint32_t)(kk * sd * sd);
             }
         }
-        TSL_TRY(m->contrib.reserve(sizeof(uint32_t) * ctab.size()));
-        TSL_HIP_CHECK(hipMemcpy(m->contrib.ptr, ctab.data(), sizeof(uint32_t) * ctab.size(), hipMemcpyHostToDevice));
-        d.contrib = m->contrib.as<uint32_t>();
-        d.contrib_n = (uint32_t)ctab.size();
+        const size_t ne = st.entries.size();
+        std::vector<AxisEntry> atab(3 * ne);
+        for (size_t a = 0; a < 3; ++a)
+            for (size_t b = 0; b < ne; ++b) {
+                const SubEntry& e = st.entries[b];
+                const uint64_t k0 = std::min<uint64_t>(e.count, sd), k1 = std::min<uint64_t>(e.count + 1ull, sd);
+                atab[a * ne + b] = AxisEntry{e.thr, ctab[a * (sd + 1) + k0], ctab[a * (sd + 1) + k1], e.count};
+            }
+        TSL_TRY(m->axis_tab.reserve(sizeof(AxisEntry) * atab.size()));
+        TSL_HIP_CHECK(hipMemcpy(m->axis_tab.ptr, atab.data(), sizeof(AxisEntry) * atab.size(), hipMemcpyHostToDevice));
+        d.axis_tab = m->axis_tab.as<AxisEntry>();
+        d.axis_n = (uint32_t)atab.size();
         // smallest fraction in [0, 1) whose in-block coordinate reaches sd
         d.carry_bits = 0x7f800000u;
         for (uint32_t bits = 0x3f7fffffu;; --bits) {
@@ -545,7 +554,7 @@ int tsl_map_device_bytes(const tsl_map* m, uint64_t* bytes)
 {
     if (!m || !bytes) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
     *bytes = m->grid_occ.bytes + m->codes.bytes + m->palette.bytes + m->sub_tab.bytes + m->full.bytes +
-             m->neg_tab.bytes + m->occ_words.bytes;
+             m->neg_tab.bytes + m->occ_words.bytes + m->axis_tab.bytes;
     return TSL_OK;
 }
 

@@ -106,42 +106,50 @@ __global__ void k_pose_ordered(const tsl_particle* __restrict__ particles, const
 }
 
 // LDS copies of the per-map tables every lookup reads: occupancy-rank
-// words, wrap-band table, sub-voxel thresholds, value palette.
+// words, wrap-band table, sub-voxel thresholds, value palette.  The
+// scoring kernels keep the palette pre-multiplied by a_hit (each entry is
+// the float product a_hit * value the score would form per pair, so the
+// sums are unchanged); the tracing kernels keep the raw values.
 struct LdsTables {
+    const AxisEntry* atab;
     const OccWord* occ;
     const uint64_t* neg;
     const SubEntry* tab;
     const float* lut;
-    const uint32_t* contrib;
 };
 
 template <bool kRank>
-__device__ __forceinline__ LdsTables stage_tables(const DevMap& map)
+__device__ __forceinline__ LdsTables stage_tables(const DevMap& map, bool scaled, float a_hit)
 {
     extern __shared__ uint4 smem_raw[];
-    OccWord* occ = reinterpret_cast<OccWord*>(smem_raw);
+    AxisEntry* atab = reinterpret_cast<AxisEntry*>(smem_raw);  // 16-byte entries first (aligned)
+    OccWord* occ = reinterpret_cast<OccWord*>(atab + map.axis_n);
     uint64_t* neg = reinterpret_cast<uint64_t*>(occ + (kRank ? map.occ_nwords : 0u));
     SubEntry* tab = reinterpret_cast<SubEntry*>(neg + 8);
     float* lut = reinterpret_cast<float*>(tab + map.sub_entries);
-    uint32_t* contrib = reinterpret_cast<uint32_t*>(lut + map.palette_size);
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(map.axis_tab);
+        uint4* dst = reinterpret_cast<uint4*>(atab);
+        for (uint32_t i = threadIdx.x; i < map.axis_n; i += blockDim.x) dst[i] = src[i];
+    }
     if (kRank) {
-        const uint4* src = reinterpret_cast<const uint4*>(map.occ_words);
-        uint4* dst = reinterpret_cast<uint4*>(occ);
+        const uint2* src = reinterpret_cast<const uint2*>(map.occ_words);
+        uint2* dst = reinterpret_cast<uint2*>(occ);
         for (uint32_t i = threadIdx.x; i < map.occ_nwords; i += blockDim.x) dst[i] = src[i];
     }
     if (threadIdx.x < 8) neg[threadIdx.x] = map.neg_tab[threadIdx.x];
     for (uint32_t i = threadIdx.x; i < map.sub_entries; i += blockDim.x) tab[i] = map.sub_tab[i];
-    for (uint32_t i = threadIdx.x; i < map.palette_size; i += blockDim.x) lut[i] = map.palette[i];
-    for (uint32_t i = threadIdx.x; i < map.contrib_n; i += blockDim.x) contrib[i] = map.contrib[i];
+    for (uint32_t i = threadIdx.x; i < map.palette_size; i += blockDim.x)
+        lut[i] = scaled ? a_hit * map.palette[i] : map.palette[i];
     __syncthreads();
-    return LdsTables{occ, neg, tab, lut, contrib};
+    return LdsTables{atab, occ, neg, tab, lut};
 }
 
 // LDS bytes stage_tables needs for a map.
 inline size_t lds_bytes(const DevMap& d, bool rank)
 {
-    return (rank ? sizeof(OccWord) * d.occ_nwords : 0) + 8 * sizeof(uint64_t) +
-           sizeof(SubEntry) * d.sub_entries + sizeof(float) * d.palette_size + sizeof(uint32_t) * d.contrib_n;
+    return sizeof(AxisEntry) * d.axis_n + (rank ? sizeof(OccWord) * d.occ_nwords : 0) + 8 * sizeof(uint64_t) +
+           sizeof(SubEntry) * d.sub_entries + sizeof(float) * d.palette_size;
 }
 
 // One wavefront: K particles (pose matrices wavefront-uniform) x the points
@@ -173,8 +181,10 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const uint32_t p = active ? min(p0 + k, P - 1) : 0u;
+        // wavefront-uniform: held in scalar registers
 #pragma unroll
-        for (int e = 0; e < 12; ++e) m[k][e] = L[(size_t)p * 12 + e];
+        for (int e = 0; e < 12; ++e)
+            m[k][e] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L[(size_t)p * 12 + e])));
     }
     float acc[K];
 #pragma unroll
@@ -217,41 +227,41 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
             float v[kP][K];
             uint64_t full[kP][K];
             uint32_t addr[kP][K];
+            bool ok[kP][K];
 #pragma unroll
             for (int u = 0; u < kP; ++u) {
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    int32_t s;
-                    uint32_t pc, kx, ky, kz;
-                    bool carry;
-                    fast_lookup<kU24, kRank>(map, tab, occ, t.contrib, ox[u][k], oy[u][k], oz[u][k], s, pc, kx, ky,
-                                             kz, carry);
+                    uint32_t s, pc, kx, ky, kz;
+                    bool blk, carry;
+                    fast_lookup<kU24, kRank>(map, t.atab, occ, ox[u][k], oy[u][k], oz[u][k], s, blk, pc, kx, ky, kz,
+                                             carry);
                     // a whole block row lies inside the array, so only a lane
                     // whose index leaves its row (carry) can fall past data_size
-                    uint32_t a = (uint32_t)s + pc;
-                    bool valid = s >= 0;
+                    uint32_t a = s + pc;
+                    bool valid = blk;
                     if (kTrace || __builtin_expect(carry, 0)) {
                         const uint32_t small = kx + mul_u<kU24>(ky, (uint32_t)map.sub_dim) +
                                                mul_u<kU24>(kz, (uint32_t)map.sub_dim_2);
-                        const uint32_t sum = (uint32_t)s + small;
+                        const uint32_t sum = s + small;
                         if (carry) {
                             valid = valid & (sum < map.ds);
                             a = kBrick ? phys_from_linear(map, sum) : sum;
                         }
-                        if (kTrace) full[u][k] = s >= 0 ? (uint64_t)(uint32_t)s + small : map.data_size;
+                        if (kTrace) full[u][k] = blk ? (uint64_t)s + small : map.data_size;
                     }
-                    addr[u][k] = valid ? a : map.sentinel;
+                    addr[u][k] = a;
+                    ok[u][k] = valid;
                 }
             }
-            // rejected pairs take the sentinel element (code of init_value)
-            // without a load
+            // rejected pairs take the code of init_value without a load
             uint32_t code[kP][K];
 #pragma unroll
             for (int u = 0; u < kP; ++u) {
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     code[u][k] = map.sentinel_code;
-                    if (addr[u][k] != map.sentinel) code[u][k] = map.codes[addr[u][k]];
+                    if (ok[u][k]) code[u][k] = map.codes[addr[u][k]];
                 }
             }
 #pragma unroll
@@ -271,7 +281,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
                         if (act[u] && p0 + k < P)
                             static_cast<float*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = v[u][k];
                     } else if (act[u]) {
-                        acc[k] += a_hit * v[u][k] + q[u].w;  // tsdf_evaluator.cpp:56-67
+                        acc[k] += v[u][k] + q[u].w;  // tsdf_evaluator.cpp:56-67 (v = a_hit * value)
                     }
                 }
             }
@@ -290,7 +300,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
                     const float tz = m[k][8] * qq.x + m[k][9] * qq.y + m[k][10] * qq.z + m[k][11];
                     const uint64_t full =
                         slow_index<kU24, kRank>(map, tab, neg, occ, tx - map.min_x, ty - map.min_y, tz - map.min_z);
-                    float v = map.init;
+                    float v = kTrace ? map.init : a_hit * map.init;
                     if (full < map.data_size) {
                         const uint32_t x = (uint32_t)full;
                         v = lut[map.codes[kBrick ? phys_from_linear(map, x) : x]];
@@ -300,7 +310,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
                     } else if (kMode == 2) {
                         if (a && p0 + k < P) static_cast<float*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = v;
                     } else if (a) {
-                        acc[k] += a_hit * v + qq.w;
+                        acc[k] += v + qq.w;
                     }
                 }
             }
@@ -325,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
                                                  float* __restrict__ partial, uint32_t stride,
                                                  void* __restrict__ trace_out, const uint32_t* __restrict__ perm)
 {
-    const LdsTables t = stage_tables<kRank>(map);
+    const LdsTables t = stage_tables<kRank>(map, kMode == 0, a_hit);
     const uint32_t P = *count;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // Workgroups are dealt to the 8 XCDs round-robin by index; give each XCD
@@ -480,79 +490,98 @@ __global__ void k_restore_m(const Member* __restrict__ members)
     }
 }
 
-// Work plan of a batched evaluation: member f owns items
-// [prefix[f], prefix[f+1]), one item = kWaves*K particles x one chunk.
+// Work plan of a batched evaluation: member f costs groups * kWaves*K * N
+// particle-point slots (its items are groups x chunks, one item = kWaves*K
+// particles x one chunk); cost[] is the exclusive prefix over members.
 __global__ __launch_bounds__(1024) void k_batch_plan(const Member* __restrict__ members, uint32_t n, uint32_t K,
                                                      BatchPlan* __restrict__ plan)
 {
-    __shared__ uint32_t wsum[16];
+    __shared__ uint64_t wsum[16];
     const uint32_t f = threadIdx.x;
-    uint32_t items = 0;
+    uint64_t cost = 0;
     if (f < n) {
         const Member& m = members[f];
-        const uint32_t P = m.st->count;
-        const uint32_t per = kWaves * K;
-        items = (P + per - 1) / per * m.chunks;
+        const uint64_t per = (uint64_t)kWaves * K;
+        cost = ((uint64_t)m.st->count + per - 1) / per * per * m.N;
     }
-    // block exclusive scan
+    // block exclusive scan, 64-bit values shuffled as two halves
     const uint32_t lane = f & 63, wave = f >> 6;
-    uint32_t incl = items;
+    uint64_t incl = cost;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t o = __shfl_up(incl, off, 64);
-        if (lane >= (uint32_t)off) incl += o;
+        const uint32_t olo = __shfl_up((uint32_t)incl, off, 64);
+        const uint32_t ohi = __shfl_up((uint32_t)(incl >> 32), off, 64);
+        if (lane >= (uint32_t)off) incl += ((uint64_t)ohi << 32) | olo;
     }
     if (lane == 63) wsum[wave] = incl;
     __syncthreads();
-    uint32_t woff = 0;
+    uint64_t woff = 0;
     for (uint32_t w = 0; w < wave; ++w) woff += wsum[w];
-    const uint32_t excl = woff + incl - items;
-    if (f < n) plan->prefix[f] = excl;
+    const uint64_t excl = woff + incl - cost;
+    if (f < n) plan->cost[f] = excl;
     if (f == n - 1) {
-        plan->prefix[n] = excl + items;
-        plan->total = excl + items;
+        plan->cost[n] = excl + cost;
         plan->n = n;
     }
 }
 
-// Batched evaluation: a persistent grid walks the work items of every
-// member (each workgroup item after item, stride gridDim.x); per item the
-// same wave body as k_eval.
+// Batched evaluation: a persistent grid splits the batch's total cost
+// evenly; each workgroup walks the items whose first pair falls in its
+// slice, in member, particle group, chunk order, with the same wave body
+// as k_eval.  Workgroups are dealt to the XCDs round-robin by index, so the
+// workgroups of one XCD take neighbouring slices (few members per XCD).
 template <int K, int kP, bool kU24, bool kBrick, bool kRank>
 __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member* __restrict__ members,
                                                        const BatchPlan* __restrict__ plan, float a_hit)
 {
-    const LdsTables t = stage_tables<kRank>(map);
-    __shared__ uint32_t pre[kMaxMembers + 1];
+    const LdsTables t = stage_tables<kRank>(map, true, a_hit);
     __shared__ float4 tile[kTile];
+    constexpr uint32_t per = kWaves * K;
     const uint32_t n = plan->n;
-    for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) pre[i] = plan->prefix[i];
-    __syncthreads();
-    const uint32_t total = pre[n];
+    const uint64_t total = plan->cost[n];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // gridDim.x is a multiple of the XCD count; the blocks of one XCD walk
-    // a contiguous range of the items together
-    const uint32_t per_xcd = gridDim.x / kXcds;
-    const uint32_t xcd = blockIdx.x % kXcds, r = blockIdx.x / kXcds;
-    const uint32_t q = total / kXcds, rem = total % kXcds;
-    const uint32_t first = xcd * q + min(xcd, rem), last = first + q + (xcd < rem ? 1u : 0u);
-    for (uint32_t item = first + r; item < last; item += per_xcd) {
-        uint32_t lo = 0, hi = n;  // member: pre[lo] <= item < pre[lo + 1]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (pre[mid] <= item) lo = mid; else hi = mid;
-        }
-        lo = __builtin_amdgcn_readfirstlane(lo);
-        const Member& m = members[lo];
-        const uint32_t local = item - pre[lo];
-        const uint32_t g = local / m.chunks, c = local - g * m.chunks;
+    const uint32_t per_xcd = gridDim.x / kXcds;  // gridDim.x is a multiple of the XCD count
+    const uint64_t rank = (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+    const uint64_t x0 = total * rank / gridDim.x, x1 = total * (rank + 1) / gridDim.x;
+    if (x0 >= x1) return;  // uniform across the workgroup
+    // member holding x0: cost[f] <= x0 < cost[f + 1] (so its cost is > 0)
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (plan->cost[mid] <= x0) lo = mid; else hi = mid;
+    }
+    uint32_t f = lo, g, c;
+    uint64_t base = plan->cost[f];
+    {
+        const Member& m = members[f];
+        const uint64_t unit = (uint64_t)per * m.N;  // one particle group x the whole scan
+        const uint64_t local = x0 - base;
+        g = (uint32_t)(local / unit);
+        const uint64_t step = (uint64_t)per * m.chunk_len;
+        c = (uint32_t)((local - (uint64_t)g * unit + step - 1) / step);  // first chunk starting at or after x0
+    }
+    while (f < n) {
+        const Member& m = members[f];
         const uint32_t P = m.st->count;
+        if (c >= m.chunks) {
+            c = 0;
+            ++g;
+        }
+        if (m.N == 0 || (uint64_t)g * per >= P) {
+            ++f;
+            g = 0;
+            c = 0;
+            if (f < n) base = plan->cost[f];
+            continue;
+        }
+        const uint64_t start = base + (uint64_t)g * per * m.N + (uint64_t)c * per * m.chunk_len;
+        if (start >= x1) break;
         const uint32_t p0 = (g * kWaves + wave) * K;
-        if (g * kWaves * K >= P) continue;  // uniform across the workgroup
         const uint32_t j0 = c * m.chunk_len;
         const uint32_t j1 = min(m.N, j0 + m.chunk_len);
         eval_wave<K, kP, 0, kU24, kBrick, kRank>(map, t, m.L, P, p0, p0 < P, m.points, m.N, j0, j1, a_hit,
                                                  m.partial + (size_t)c * m.stride, nullptr, nullptr, tile);
+        ++c;
     }
 }
 
@@ -609,6 +638,19 @@ Tile choose_tile(const tsl_filter* f, size_t P, size_t N)
     t.chunks = (uint32_t)((N + len - 1) / len);
     if (t.chunks == 0) t.chunks = 1;
     return t;
+}
+
+uint32_t batch_chunk(uint64_t batch_cost, uint32_t grid, int K, size_t N, uint32_t min_chunk)
+{
+    // the whole scan per item while an item (kWaves*K particles x the
+    // scan) is at most half a workgroup's share of the batch; shorter
+    // chunks (multiples of 64) only when the batch is too small for that
+    const uint64_t per = (uint64_t)kWaves * (uint64_t)(K > 0 ? K : 1);
+    const uint64_t share = batch_cost / (2ull * (grid ? grid : 1u));
+    uint64_t len = (share / per + 63) / 64 * 64;
+    len = std::max<uint64_t>(len, std::max<uint32_t>(min_chunk, 256u));
+    len = std::min<uint64_t>(len, (N + 63) / 64 * 64);
+    return (uint32_t)(len ? len : 64);
 }
 
 namespace {

@@ -15,7 +15,7 @@ namespace tsl {
 constexpr uint32_t kMaxPalette = 4096;  // 16-bit voxel codes index at most this many values
 constexpr uint32_t kFastLimit = 8388608u;  // 2^23: offsets below this take the 32-bit lookup
 constexpr uint32_t kMaxSubTable = 1024;    // buckets of the sub-voxel threshold table
-constexpr uint32_t kMaxOccWords = 2048;    // occupancy-rank words kept in LDS (131072 blocks, 32 KB)
+constexpr uint32_t kMaxOccWords = 4096;    // occupancy-rank words kept in LDS (131072 blocks, 32 KB)
 
 // ---------------------------------------------------------------------------
 // Error state
@@ -92,10 +92,18 @@ struct SubTable {
 // order (offset of the n-th allocated block in linear order = n * sub_dim^3,
 // cuda_sub_voxel_map.tcc:202-211), grid_occ[up] == rank(up) * sub_dim^3 for
 // allocated blocks, and rank(up) = prefix + popcount(bits below up).
+// Sub-voxel bucket b of one axis: the in-block coordinate is count, or
+// count + 1 when the fraction's bit pattern is >= thr; c_lo / c_hi are that
+// axis's contribution to the voxel's physical offset inside its block for
+// those two coordinates (a coordinate equal to sub_dim contributes as 0:
+// such lanes recompute their address).
+struct AxisEntry {
+    uint32_t thr, c_lo, c_hi, count;
+};
+
 struct OccWord {
-    uint32_t lo, hi;   // bits of blocks 64w .. 64w+63
+    uint32_t bits;     // bit j: block 32w + j allocated
     uint32_t prefix;   // allocated blocks in words 0 .. w-1
-    uint32_t pad;
 };
 
 // Every field the exact tiers of the lookup need (64-bit reference sizes,
@@ -138,10 +146,10 @@ struct DevMap {
     uint32_t brick;                   // 1: bricked codes
     uint32_t sd3;                     // sub_dim^3
     uint32_t brick_cy, brick_cz;      // 64B - 16, 64B^2 - 64
-    // per-axis in-block contribution of in-block coordinate k (0..sub_dim)
-    // to the physical voxel address: contrib[a * (sub_dim + 1) + k]
-    const uint32_t* contrib;
-    uint32_t contrib_n;               // 3 * (sub_dim + 1)
+    // per-axis sub-voxel buckets with the physical-address contribution of
+    // both in-block coordinates a bucket can give: axis_tab[a * sub_entries + b]
+    const AxisEntry* axis_tab;
+    uint32_t axis_n;                  // 3 * sub_entries
     uint32_t carry_bits;              // bits of the smallest fraction whose in-block coordinate is sub_dim (+inf if none)
     uint32_t mag_sd, sh_sd, mag_sd2, sh_sd2, mag_sd3, sh_sd3;  // x / d = umulhi(x, mag) >> sh, x < 2^31
     // occupancy-rank form of the block table (set when the offsets follow
@@ -221,10 +229,13 @@ struct Member {
 
 constexpr uint32_t kMaxMembers = 1024;  // filters in one batch
 
-// Item table of a batched evaluation, built on the device each step.
+// Work plan of a batched evaluation, built on the device each step from the
+// members' device counts: cost[f] is the exclusive prefix over members of
+// groups_f * kWaves*K * N_f (particle slots x scan points), cost[n] the
+// total.  Workgroups split the total cost evenly.
 struct BatchPlan {
-    uint32_t n, total, pad[2];
-    uint32_t prefix[kMaxMembers + 1];
+    uint32_t n, pad[3];
+    uint64_t cost[kMaxMembers + 1];
 };
 
 // Upper bound on the resampling output for M particles whose weights sum
@@ -247,7 +258,7 @@ inline uint64_t resample_bound(uint64_t M)
 struct tsl_map {
     int device = 0;
     tsl_map_coef coef{};
-    tsl::DevBuf grid_occ, codes, palette, sub_tab, neg_tab, full, occ_words, contrib;
+    tsl::DevBuf grid_occ, codes, palette, sub_tab, neg_tab, full, occ_words, axis_tab;
     tsl::DevMap dev{};
 };
 
@@ -364,6 +375,9 @@ struct BatchLaunch {
 };
 int launch_batch_update(const BatchLaunch& b, hipStream_t stream);
 int batch_grid(const BatchLaunch& b, uint32_t* grid);
+// Chunk length of one member of a batch whose members hold batch_cost
+// particle-point pairs in all, for a persistent grid of `grid` workgroups.
+uint32_t batch_chunk(uint64_t batch_cost, uint32_t grid, int K, size_t N, uint32_t min_chunk);
 int launch_batch_restore(const BatchLaunch& b, hipStream_t stream);
 Tile choose_tile(const tsl_filter* f, size_t P, size_t N);
 

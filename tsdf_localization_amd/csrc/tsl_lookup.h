@@ -77,19 +77,24 @@ __device__ __forceinline__ int32_t mul_i(int32_t a, int32_t b)
     return a * b;
 }
 
-// Block offset of block up (< grid_occ_size) from the LDS copy of the
-// occupancy-rank table: rank * sub_dim^3 if allocated, else -1.  Equals
-// grid_occ[up] for maps whose offsets follow setData order.
+// Occupancy-rank form of the block table (LDS copy): for block up
+// (< grid_occ_size), set = allocated and the return value = rank *
+// sub_dim^3, which equals grid_occ[up] for maps whose offsets follow setData
+// order.  The rank is below 2^17 and sub_dim^3 below 2^24, so the product is
+// a 24-bit multiply.
+__device__ __forceinline__ uint32_t rank_block(const OccWord* __restrict__ occ, uint32_t up, uint32_t sd3, bool& set)
+{
+    const OccWord w = occ[up >> 5];
+    const uint32_t r = up & 31u;
+    set = ((w.bits >> r) & 1u) != 0u;
+    return __umul24((uint32_t)__popc(w.bits & ~(0xffffffffu << r)) + w.prefix, sd3);
+}
+
 __device__ __forceinline__ int32_t rank_offset(const OccWord* __restrict__ occ, uint32_t up, uint32_t sd3)
 {
-    const OccWord w = occ[up >> 6];
-    const uint32_t r = up & 63u;
-    const uint32_t word = r < 32u ? w.lo : w.hi;
-    const bool set = ((word >> (r & 31u)) & 1u) != 0u;
-    const uint32_t m_lo = r < 32u ? (1u << r) - 1u : 0xffffffffu;
-    const uint32_t m_hi = r < 32u ? 0u : (1u << (r - 32u)) - 1u;
-    const uint32_t below = (uint32_t)__popc(w.lo & m_lo) + (uint32_t)__popc(w.hi & m_hi);
-    return set ? (int32_t)((w.prefix + below) * sd3) : -1;
+    bool set;
+    const uint32_t off = rank_block(occ, up, sd3, set);
+    return set ? (int32_t)off : -1;
 }
 
 template <bool kRank>
@@ -103,8 +108,7 @@ __device__ __forceinline__ int32_t block_offset(const DevMap& m, const OccWord* 
 // pattern per axis, since floats >= +0 order like their bits).
 __device__ __forceinline__ bool fast_ok(float ox, float oy, float oz, uint32_t range_bits)
 {
-    return (__float_as_uint(ox) < range_bits) & (__float_as_uint(oy) < range_bits) &
-           (__float_as_uint(oz) < range_bits);
+    return max(max(__float_as_uint(ox), __float_as_uint(oy)), __float_as_uint(oz)) < range_bits;
 }
 
 // Wrap tier: |o| < R on every axis.
@@ -112,20 +116,6 @@ __device__ __forceinline__ bool wrap_ok(float ox, float oy, float oz, uint32_t r
 {
     return ((__float_as_uint(ox) & 0x7fffffffu) < range_bits) & ((__float_as_uint(oy) & 0x7fffffffu) < range_bits) &
            ((__float_as_uint(oz) & 0x7fffffffu) < range_bits);
-}
-
-struct FastAxis {
-    uint32_t i;  // trunc(o)
-    uint32_t k;  // in-block coordinate
-};
-
-__device__ __forceinline__ FastAxis fast_axis(const DevMap& m, const SubEntry* __restrict__ tab, float o)
-{
-    FastAxis r;
-    r.i = (uint32_t)(int32_t)o;
-    const float f = o - (float)(int32_t)r.i;  // exact, in [0, 1)
-    r.k = sub_index(m.sub_base, m.sub_shift, tab, f);
-    return r;
 }
 
 // Offset of local voxel (lx, ly, lz), each < sub_dim, inside a bricked block:
@@ -147,55 +137,52 @@ __device__ __forceinline__ uint32_t phys_from_linear(const DevMap& m, uint32_t x
     return slot * m.sd3 + brick_offset(m, lx, ly, lz);
 }
 
-// Fast tier for one lane: block offset s (or -1), in-block part of the
-// reference index, and (bricked layout) the in-block physical offset, with
-// carry set when an in-block coordinate equals sub_dim (the division rounded
-// up to the block edge; the reference index then lands in the next row).
-template <bool kU24, bool kBrick, bool kRank>
-__device__ __forceinline__ void fast_block(const DevMap& m, const SubEntry* __restrict__ tab,
-                                           const OccWord* __restrict__ occ, float ox, float oy, float oz, int32_t& s,
-                                           uint32_t& small, uint32_t& pbrick, bool& carry)
+// One axis of the fast tier: the in-block coordinate k of fraction f and
+// its contribution to the physical in-block offset, from one LDS entry.
+__device__ __forceinline__ uint32_t axis_part(uint32_t sub_base, uint32_t sub_shift,
+                                              const AxisEntry* __restrict__ t, float f, uint32_t& k)
 {
-    const FastAxis ax = fast_axis(m, tab, ox), ay = fast_axis(m, tab, oy), az = fast_axis(m, tab, oz);
-    const bool grid = (ax.i < m.gthr_x) & (ay.i < m.gthr_y) & (az.i < m.gthr_z);
-    const uint32_t up = ax.i + mul_u<kU24>(ay.i, m.up_dim_x) + mul_u<kU24>(az.i, m.up_dim_2);
-    s = -1;
-    if (grid & (up < m.occ)) s = block_offset<kRank>(m, occ, up);
-    small = ax.k + mul_u<kU24>(ay.k, (uint32_t)m.sub_dim) + mul_u<kU24>(az.k, (uint32_t)m.sub_dim_2);
-    if (kBrick) {
-        pbrick = brick_offset(m, ax.k, ay.k, az.k);
-        carry = max(max(ax.k, ay.k), az.k) >= (uint32_t)m.sub_dim;
-    } else {
-        pbrick = small;
-        carry = false;
-    }
+    const uint32_t key = __float_as_uint(f);
+    const uint32_t b = (key > sub_base ? key - sub_base : 0u) >> sub_shift;
+    const AxisEntry e = t[b];
+    const bool hi = key >= e.thr;
+    k = e.count + (hi ? 1u : 0u);
+    return hi ? e.c_hi : e.c_lo;
 }
 
-// Fast tier for one lane, address form: block offset s (or -1), the sum pc
-// of the three per-axis in-block address contributions (LDS table indexed by
-// the in-block coordinate), the coordinates themselves, and carry when some
+// Fast tier for one lane, address form: block offset s with valid = block
+// allocated and inside the grid, the sum pc of the three per-axis in-block
+// address contributions, the in-block coordinates, and carry when some
 // coordinate equals sub_dim (the lane's reference index then leaves the
 // block row, and its address is recomputed from the linear index).  o >= 0
 // on every axis here, so trunc is a plain conversion and the fraction
 // o - trunc(o) is exactly fract(o).
 template <bool kU24, bool kRank>
-__device__ __forceinline__ void fast_lookup(const DevMap& m, const SubEntry* __restrict__ tab,
-                                            const OccWord* __restrict__ occ, const uint32_t* __restrict__ contrib,
-                                            float ox, float oy, float oz, int32_t& s, uint32_t& pc, uint32_t& kx,
-                                            uint32_t& ky, uint32_t& kz, bool& carry)
+__device__ __forceinline__ void fast_lookup(const DevMap& m, const AxisEntry* __restrict__ atab,
+                                            const OccWord* __restrict__ occ, float ox, float oy, float oz,
+                                            uint32_t& s, bool& valid, uint32_t& pc, uint32_t& kx, uint32_t& ky,
+                                            uint32_t& kz, bool& carry)
 {
     const uint32_t ix = (uint32_t)ox, iy = (uint32_t)oy, iz = (uint32_t)oz;
     const float fx = __builtin_amdgcn_fractf(ox), fy = __builtin_amdgcn_fractf(oy), fz = __builtin_amdgcn_fractf(oz);
-    kx = sub_index(m.sub_base, m.sub_shift, tab, fx);
-    ky = sub_index(m.sub_base, m.sub_shift, tab, fy);
-    kz = sub_index(m.sub_base, m.sub_shift, tab, fz);
-    const uint32_t n1 = (uint32_t)m.sub_dim + 1u;
-    pc = contrib[kx] + contrib[n1 + ky] + contrib[2u * n1 + kz];
+    const uint32_t n = m.sub_entries;
+    pc = axis_part(m.sub_base, m.sub_shift, atab, fx, kx) + axis_part(m.sub_base, m.sub_shift, atab + n, fy, ky) +
+         axis_part(m.sub_base, m.sub_shift, atab + 2u * n, fz, kz);
     carry = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz)) >= m.carry_bits;
     const bool grid = (ix < m.gthr_x) & (iy < m.gthr_y) & (iz < m.gthr_z);
     const uint32_t up = ix + mul_u<kU24>(iy, m.up_dim_x) + mul_u<kU24>(iz, m.up_dim_2);
-    s = -1;
-    if (grid & (up < m.occ)) s = block_offset<kRank>(m, occ, up);
+    const bool inside = grid & (up < m.occ);
+    if (kRank) {
+        // branch-free: rejected lanes read word 0 and discard it
+        bool set;
+        s = rank_block(occ, inside ? up : 0u, m.sd3, set);
+        valid = inside & set;
+    } else {
+        int32_t g = -1;
+        if (inside) g = m.grid_occ[up];
+        s = (uint32_t)g;
+        valid = g >= 0;
+    }
 }
 
 // Wrap tier for one lane (|o| < R on every axis): the reference's index,
