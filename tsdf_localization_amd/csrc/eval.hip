@@ -15,6 +15,8 @@
 // order, so results are deterministic.
 #include "tsl_lookup.h"
 
+#include <type_traits>
+
 namespace tsl {
 
 namespace {
@@ -198,13 +200,16 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
     for (uint32_t i = threadIdx.x; i < te - tb; i += blockDim.x) tile[i] = pts[tb + i];
     __syncthreads();
     if (!active) continue;
-    for (uint32_t base = tb; base < te; base += 64 * kP) {
+    // one iteration: 64*kP consecutive points from base; kFull: all of
+    // them lie before te (no per-lane tail checks)
+    auto step = [&](uint32_t base, auto full_tag) {
+        constexpr bool kFull = decltype(full_tag)::value;
         float4 q[kP];
         bool act[kP];
 #pragma unroll
         for (int u = 0; u < kP; ++u) {
             const uint32_t j = base + 64u * u + lane;
-            act[u] = j < te;
+            act[u] = kFull || j < te;
             q[u] = tile[act[u] ? j - tb : 0u];
         }
         float ox[kP][K], oy[kP][K], oz[kP][K];
@@ -261,7 +266,10 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     code[u][k] = map.sentinel_code;
-                    if (ok[u][k]) code[u][k] = map.codes[addr[u][k]];
+                    // 32-bit byte offset from the array base (addresses < 2^31)
+                    if (ok[u][k])
+                        code[u][k] = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(map.codes) +
+                                                                        (addr[u][k] << 1));
                 }
             }
 #pragma unroll
@@ -291,7 +299,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll 1
             for (int u = 0; u < kP; ++u) {
                 const uint32_t j = base + 64u * u + lane;
-                const bool a = j < te;
+                const bool a = kFull || j < te;
                 const float4 qq = tile[a ? j - tb : 0u];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -315,7 +323,10 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
                 }
             }
         }
-    }
+    };
+    uint32_t base = tb;
+    for (; base + 64u * kP <= te; base += 64u * kP) step(base, std::true_type{});
+    if (base < te) step(base, std::false_type{});
     }
     if (kTrace || !active) return;
 #pragma unroll

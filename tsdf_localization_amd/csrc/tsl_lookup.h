@@ -86,8 +86,8 @@ __device__ __forceinline__ uint32_t rank_block(const OccWord* __restrict__ occ, 
 {
     const OccWord w = occ[up >> 5];
     const uint32_t r = up & 31u;
-    set = ((w.bits >> r) & 1u) != 0u;
-    return __umul24((uint32_t)__popc(w.bits & ~(0xffffffffu << r)) + w.prefix, sd3);
+    set = __builtin_amdgcn_ubfe(w.bits, r, 1u) != 0u;
+    return __umul24((uint32_t)__popc(__builtin_amdgcn_ubfe(w.bits, 0u, r)) + w.prefix, sd3);
 }
 
 __device__ __forceinline__ int32_t rank_offset(const OccWord* __restrict__ occ, uint32_t up, uint32_t sd3)
