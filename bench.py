@@ -72,20 +72,41 @@ def _cpu_model():
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_eval_global_localisation_traffic.md")
+VALU_FILE = os.path.join(ROOT, "profiles", "k_eval_global_localisation_valu.md")
+# VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes per cycle (a wave64 instruction
+# takes 4 cycles) at the 2.4 GHz peak engine clock, in lane-operations/s
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 
 def measured_traffic():
     """HBM bytes per evaluation-kernel launch from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes (scaled by the plain-copy calibration),
     or None if that summary is absent."""
+    return _profile_value(TRAFFIC_FILE, "traffic_bytes_per_launch")
+
+
+def _profile_value(path, key):
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(path) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
+
+
+def valu_issue(kern_avg_ms):
+    """Issue-rate view of the evaluation kernel: VALU instructions per
+    launch from the committed rocprofv3 SQ_INSTS_VALU pass (same workload and
+    tile), x 64 lanes, over the measured launch time, against the VALU peak."""
+    instr = _profile_value(VALU_FILE, "valu_instructions_per_launch")
+    if instr is None or kern_avg_ms <= 0:
+        return None
+    achieved = instr * 64 / (kern_avg_ms / 1000.0)
+    return {"achieved": achieved, "peak": VALU_PEAK_LANE_OPS, "unit": "VALU lane-ops/s",
+            "frac": achieved / VALU_PEAK_LANE_OPS, "instructions_per_launch": instr,
+            "source": "profiles/k_eval_global_localisation_valu.md"}
 
 
 def copy_calibration(mib, reps):
@@ -159,7 +180,7 @@ def bench_batched(args):
     filters = make_filters()
     b = tsl.Batch(ev, filters)
     results = {}
-    for k, chunk in ((1, 2048), (2, 2048), (2, 8192), (1, 0), (2, 0)):
+    for k, chunk in ((1, 0), (2, 1024), (2, 4096), (2, 0)):
         b.set_tile(k, chunk, 1)
         ms = timed(lambda i: b.step_async(draws[i], restore=True, stream=stream))
         b.sync(stream)
@@ -297,7 +318,8 @@ def bench_serving(args):
         "dtype": "f32",
         "data": "synthetic (building map, shared ray-cast scans, particles around each robot)",
         "config": {"workload": "serving", "robots": R, "requests": n_req, "arrival_rate_per_s": rate,
-                   "max_batch": args.max_batch, "map": "sparse 1024x1024x128 @0.05 m int16"},
+                   "max_batch": args.max_batch, "chunk_points": args.chunk or 2048,
+                   "map": "sparse 1024x1024x128 @0.05 m int16"},
         "latency_ms": {"p50": p50, "p99": p99, "mean_batch": float(batch.mean()), "failed": int(bad)},
         "plain_value": plain_rate,
         "speedup_vs_plain": served_rate / plain_rate,
@@ -317,6 +339,7 @@ def main():
     ap.add_argument("--hbm-budget-gb", type=int, default=64, help="serving: device bytes the server may hold")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the host-array drop-in timing")
     ap.add_argument("--graph-step", action="store_true", help="also time the step replayed from a HIP graph")
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--cpu-repeats", type=int, default=3)
@@ -366,7 +389,7 @@ def main():
     if args.tile_k is None:
         order, k, pp, chunk, tms = f.autotune(stream, reps=3)
         log(f"autotuned: order={order} K={k} points/lane={pp} chunk={chunk or 'auto'} "
-            f"({tms:.3f} ms per evaluation kernel)")
+            f"({tms:.3f} ms per update for evaluation and particle order)")
     else:
         order, k, pp, chunk = args.order, args.tile_k, args.tile_p, args.chunk
         f.set_tile(k, chunk, pp)
@@ -427,6 +450,22 @@ def main():
         graph_ms = g0.elapsed_ms(g1) / args.steps
         log(f"graph-replayed step: {graph_ms:.3f} ms/step, {evals / graph_ms * 1e3:.3e} evals/s")
 
+    # the drop-in form: host particle and point arrays through tsl_evaluate,
+    # which copies them in and the weights out on every call like the
+    # reference's CudaEvaluator (PCIe-inclusive; reported beside the value)
+    dropin = None
+    if not args.no_dropin:
+        reps = 3
+        copies = [w.particles.copy() for _ in range(reps + 1)]
+        ev.evaluate(copies[0], w.points, w.tf)
+        t0 = time.perf_counter()
+        for i in range(reps):
+            ev.evaluate(copies[i + 1], w.points, w.tf)
+        dropin_ms = (time.perf_counter() - t0) * 1000.0 / reps
+        dropin = {"ms_per_update": dropin_ms, "evals_per_s": evals / dropin_ms * 1e3,
+                  "form": "tsl_evaluate on host arrays: copy in, pose + evaluate + normalise, copy weights out"}
+        log(f"drop-in evaluate on host arrays: {dropin_ms:.3f} ms/update, {evals / dropin_ms * 1e3:.3e} evals/s")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -459,8 +498,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_eval", "kernel_avg_ms": kern_avg_ms,
                      "bytes_per_eval": BYTES_PER_EVAL, "evals_per_launch": pairs // max(launches, 1)},
+        "valu": valu_issue(kern_avg_ms) if w.name == "building" else None,
         "cpu_baseline": cpu,
         "graph_step_ms": graph_ms,
+        "dropin_host_buffers": dropin,
         "resampled_count": n_after,
     }
     print(json.dumps(out), flush=True)

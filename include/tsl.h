@@ -287,11 +287,12 @@ int tsl_filter_set_order(tsl_filter* f, int mode);
  * counterpart in the reference. */
 int tsl_filter_set_block_lookup(tsl_filter* f, int mode);
 
-/* Time the candidate evaluation shapes (particle order x tile) on the
- * filter's current particles and scan (reps launches each, on `stream`,
- * synchronously) and keep the fastest; reports it through the non-NULL out
- * arguments.  No counterpart in the reference (its launch shape is fixed,
- * src/cuda/cuda_evaluator.cu:301). */
+/* Time the candidate evaluation shapes (particle order x block lookup x
+ * tile) on the filter's current particles and scan (reps launches each, on
+ * `stream`, synchronously) and keep the fastest; reports it through the
+ * non-NULL out arguments (best_ms: evaluation kernel plus the per-update
+ * cost of the particle order).  No counterpart in the reference (its launch
+ * shape is fixed, src/cuda/cuda_evaluator.cu:301). */
 int tsl_filter_autotune(tsl_filter* f, void* stream, int reps, int* best_order, int* best_k,
                         int* best_points_per_lane, uint32_t* best_chunk, float* best_ms);
 
@@ -348,10 +349,10 @@ int tsl_batch_create(tsl_evaluator* ev, tsl_filter* const* filters, size_t n, ts
 int tsl_batch_destroy(tsl_batch* b);
 
 /* Tile of the batched evaluation (k particles per wavefront, 1 or 2; points
- * per lane 1, 2 or 4; chunk_len points per chunk, 0 = chosen per member
- * against the whole batch: the whole scan unless the batch is too small to
- * fill the GPU that way).  The persistent grid splits the batch's
- * particle-point pairs evenly over its workgroups.  Re-prepares the batch. */
+ * per lane 1, 2 or 4; chunk_len points per chunk, 0 = automatic: 2048, or
+ * the whole scan if shorter).  The workgroups of the persistent grid claim
+ * work items (4k particles x one chunk) one at a time.  Re-prepares the
+ * batch. */
 int tsl_batch_set_tile(tsl_batch* b, int k, int points_per_lane, uint32_t chunk_len);
 
 /* Rebuild the members' device descriptors and reserve their buffers. */
@@ -422,7 +423,8 @@ typedef struct tsl_result {
 /* hbm_budget_bytes: device bytes the server may hold (map included; 0 = no
  * limit); max_robots resident at once; max_particles per robot;
  * max_batch requests per tick; chunk_len scan points per evaluation chunk
- * (0 = 2048). */
+ * (0 = each request's whole scan).  A request's result depends on its own
+ * chunking only, never on the tick it ran in. */
 int tsl_server_create(tsl_evaluator* ev, uint64_t hbm_budget_bytes, uint32_t max_robots, uint64_t max_particles,
                       uint32_t max_batch, uint32_t chunk_len, tsl_server** out);
 int tsl_server_destroy(tsl_server* s);

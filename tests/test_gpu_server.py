@@ -43,11 +43,12 @@ def _motion(tsl, i):
                                            seed=1000 + i)
 
 
-def test_server_alone_equals_mix_equals_filter(tsl, room):
+@pytest.mark.parametrize("chunk", [CHUNK, 0], ids=["chunk512", "whole-scan"])
+def test_server_alone_equals_mix_equals_filter(tsl, room, chunk):
     from tsdf_localization_amd.server import Server
     ev = tsl.TSDFEvaluator(room.map)
     robots = _robots(room, 8)
-    srv = Server(ev, max_robots=32, max_particles=2000, max_batch=16, chunk_len=CHUNK)
+    srv = Server(ev, max_robots=32, max_particles=2000, max_batch=16, chunk_len=chunk)
     try:
         ids_a = [srv.add_robot(p, 3000, room.tf) for p, _ in robots]
         ids_b = [srv.add_robot(p, 3000, room.tf) for p, _ in robots]  # twins
@@ -73,7 +74,7 @@ def test_server_alone_equals_mix_equals_filter(tsl, room):
             parts, pts = robots[i]
             f = tsl.DeviceFilter(ev, parts.shape[0])
             f.set_particles(parts)
-            f.set_tile(1, CHUNK, 1)
+            f.set_tile(1, chunk or (pts.shape[0] + 63) // 64 * 64, 1)
             f.set_order(0)
             s = tsl.Stream()
             f.motion_async(_motion(tsl, i), s)

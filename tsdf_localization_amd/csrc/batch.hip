@@ -18,7 +18,7 @@ struct tsl_batch {
     std::vector<tsl_filter*> filters;
     int order = 0, K = 1, P = 1;
     bool rank = true;        // block offsets from the LDS occupancy-rank table when the map has one
-    uint32_t chunk_len = 0;  // 0: chosen per member against the whole batch (batch_chunk)
+    uint32_t chunk_len = 0;  // 0: batch_chunk of each member's scan
     DevBuf members, plan, u0_dev, states, motions;
     std::vector<Member> host_members;
     BatchLaunch launch;
@@ -72,24 +72,6 @@ int prepare(tsl_batch* b)
         max_cap = std::max<uint64_t>(max_cap, f->capacity);
     }
     const uint64_t nb = (max_cap + 255) / 256;
-    // automatic chunks: sized against the whole batch on the full persistent
-    // grid (every member's particle bound x its scan)
-    uint64_t batch_cost = 0;
-    uint32_t full_grid = 0;
-    if (b->chunk_len == 0) {
-        const uint64_t per = 4ull * (uint64_t)b->K;
-        for (tsl_filter* f : b->filters) {
-            const uint64_t cnt = std::max<size_t>(f->count_bound, f->snapshot_count);
-            batch_cost += (cnt + per - 1) / per * per * f->num_points;
-        }
-        BatchLaunch probe;
-        probe.map = b->ev->map;
-        probe.K = b->K;
-        probe.P = b->P;
-        probe.rank = b->rank;
-        probe.max_items = ~0ull >> 8;
-        TSL_TRY(batch_grid(probe, &full_grid));
-    }
     b->host_members.assign(n, Member{});
     for (uint32_t i = 0; i < n; ++i) {
         tsl_filter* f = b->filters[i];
@@ -100,7 +82,7 @@ int prepare(tsl_batch* b)
         const uint32_t saved_chunk = f->tile_chunk;
         f->tile_k = b->K;
         f->tile_p = b->P;
-        f->tile_chunk = b->chunk_len ? b->chunk_len : batch_chunk(batch_cost, full_grid, b->K, f->num_points, 0);
+        f->tile_chunk = b->chunk_len ? b->chunk_len : batch_chunk(f->num_points);
         const Tile t = choose_tile(f, f->capacity, f->num_points);
         f->tile_k = saved_k;
         f->tile_p = saved_p;

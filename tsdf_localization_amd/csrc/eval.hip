@@ -501,98 +501,86 @@ __global__ void k_restore_m(const Member* __restrict__ members)
     }
 }
 
-// Work plan of a batched evaluation: member f costs groups * kWaves*K * N
-// particle-point slots (its items are groups x chunks, one item = kWaves*K
-// particles x one chunk); cost[] is the exclusive prefix over members.
+// Work plan of a batched evaluation: member f owns items [prefix[f],
+// prefix[f+1]), groups x chunks of them in (particle group, chunk) order;
+// the item counter starts at zero.
 __global__ __launch_bounds__(1024) void k_batch_plan(const Member* __restrict__ members, uint32_t n, uint32_t K,
                                                      BatchPlan* __restrict__ plan)
 {
-    __shared__ uint64_t wsum[16];
+    __shared__ uint32_t wsum[16];
     const uint32_t f = threadIdx.x;
-    uint64_t cost = 0;
+    uint32_t items = 0;
     if (f < n) {
         const Member& m = members[f];
-        const uint64_t per = (uint64_t)kWaves * K;
-        cost = ((uint64_t)m.st->count + per - 1) / per * per * m.N;
+        const uint32_t per = kWaves * K;
+        items = m.N ? (m.st->count + per - 1) / per * m.chunks : 0u;
     }
-    // block exclusive scan, 64-bit values shuffled as two halves
+    // block exclusive scan
     const uint32_t lane = f & 63, wave = f >> 6;
-    uint64_t incl = cost;
+    uint32_t incl = items;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t olo = __shfl_up((uint32_t)incl, off, 64);
-        const uint32_t ohi = __shfl_up((uint32_t)(incl >> 32), off, 64);
-        if (lane >= (uint32_t)off) incl += ((uint64_t)ohi << 32) | olo;
+        const uint32_t o = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off) incl += o;
     }
     if (lane == 63) wsum[wave] = incl;
     __syncthreads();
-    uint64_t woff = 0;
+    uint32_t woff = 0;
     for (uint32_t w = 0; w < wave; ++w) woff += wsum[w];
-    const uint64_t excl = woff + incl - cost;
-    if (f < n) plan->cost[f] = excl;
+    const uint32_t excl = woff + incl - items;
+    if (f < n) plan->prefix[f] = excl;
     if (f == n - 1) {
-        plan->cost[n] = excl + cost;
+        plan->prefix[n] = excl + items;
+        plan->total = excl + items;
         plan->n = n;
     }
+    if (f < kPlanXcds) plan->next[f * 32] = 0;
 }
 
-// Batched evaluation: a persistent grid splits the batch's total cost
-// evenly; each workgroup walks the items whose first pair falls in its
-// slice, in member, particle group, chunk order, with the same wave body
-// as k_eval.  Workgroups are dealt to the XCDs round-robin by index, so the
-// workgroups of one XCD take neighbouring slices (few members per XCD).
+// Batched evaluation: a persistent grid claims items one at a time, the
+// workgroups of XCD x (dealt round-robin by index) from the items x, x + 8,
+// ... through their XCD's counter, so all workgroups sweep the members
+// together, as plain per-filter launches would, and the tail balances
+// itself; per item the same wave body as k_eval.
 template <int K, int kP, bool kU24, bool kBrick, bool kRank>
 __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member* __restrict__ members,
-                                                       const BatchPlan* __restrict__ plan, float a_hit)
+                                                       BatchPlan* __restrict__ plan, float a_hit)
 {
     const LdsTables t = stage_tables<kRank>(map, true, a_hit);
     __shared__ float4 tile[kTile];
-    constexpr uint32_t per = kWaves * K;
+    __shared__ uint32_t claim[2];
     const uint32_t n = plan->n;
-    const uint64_t total = plan->cost[n];
+    const uint32_t total = plan->total;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t per_xcd = gridDim.x / kXcds;  // gridDim.x is a multiple of the XCD count
-    const uint64_t rank = (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-    const uint64_t x0 = total * rank / gridDim.x, x1 = total * (rank + 1) / gridDim.x;
-    if (x0 >= x1) return;  // uniform across the workgroup
-    // member holding x0: cost[f] <= x0 < cost[f + 1] (so its cost is > 0)
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (plan->cost[mid] <= x0) lo = mid; else hi = mid;
-    }
-    uint32_t f = lo, g, c;
-    uint64_t base = plan->cost[f];
-    {
+    uint32_t* counter = &plan->next[(blockIdx.x % kPlanXcds) * 32];
+    const uint32_t xcd = blockIdx.x % kPlanXcds;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const uint32_t item = atomicAdd(counter, 1u) * kPlanXcds + xcd;
+            uint32_t lo = 0, hi = n;  // member: prefix[lo] <= item < prefix[lo + 1]
+            if (item < total) {
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (plan->prefix[mid] <= item) lo = mid; else hi = mid;
+                }
+            }
+            claim[0] = item;
+            claim[1] = lo;
+        }
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(claim[0]);
+        const uint32_t f = __builtin_amdgcn_readfirstlane(claim[1]);
+        __syncthreads();  // claim[] is read before thread 0 overwrites it
+        if (item >= total) break;  // uniform across the workgroup
         const Member& m = members[f];
-        const uint64_t unit = (uint64_t)per * m.N;  // one particle group x the whole scan
-        const uint64_t local = x0 - base;
-        g = (uint32_t)(local / unit);
-        const uint64_t step = (uint64_t)per * m.chunk_len;
-        c = (uint32_t)((local - (uint64_t)g * unit + step - 1) / step);  // first chunk starting at or after x0
-    }
-    while (f < n) {
-        const Member& m = members[f];
+        const uint32_t local = item - plan->prefix[f];
+        const uint32_t g = local / m.chunks, c = local - g * m.chunks;
         const uint32_t P = m.st->count;
-        if (c >= m.chunks) {
-            c = 0;
-            ++g;
-        }
-        if (m.N == 0 || (uint64_t)g * per >= P) {
-            ++f;
-            g = 0;
-            c = 0;
-            if (f < n) base = plan->cost[f];
-            continue;
-        }
-        const uint64_t start = base + (uint64_t)g * per * m.N + (uint64_t)c * per * m.chunk_len;
-        if (start >= x1) break;
         const uint32_t p0 = (g * kWaves + wave) * K;
         const uint32_t j0 = c * m.chunk_len;
         const uint32_t j1 = min(m.N, j0 + m.chunk_len);
         eval_wave<K, kP, 0, kU24, kBrick, kRank>(map, t, m.L, P, p0, p0 < P, m.points, m.N, j0, j1, a_hit,
                                                  m.partial + (size_t)c * m.stride, nullptr, nullptr, tile);
-        ++c;
     }
 }
 
@@ -616,18 +604,20 @@ __global__ void k_restore(const tsl_particle* __restrict__ src, tsl_particle* __
 }
 
 // Tile shape: K particles per wavefront (template instantiations 1, 2)
-// and the scan chunk a wavefront walks.  Automatic choice: K = 1 (fewest
-// scalar registers, full occupancy on gfx950) and enough point chunks to
-// give the 256 CUs at least four full rounds of wavefronts.
+// and the scan chunk a wavefront walks.  Automatic choice: K = 2 (the
+// point load, tile walk and tail handling are shared by two particles; the
+// kernel is bound by VALU issue), block offsets from the LDS rank table when
+// the map has one, spatial particle order from 8192 particles, and enough
+// point chunks to give the 256 CUs at least four full rounds of wavefronts.
 }  // namespace
 
 Tile choose_tile(const tsl_filter* f, size_t P, size_t N)
 {
     Tile t;
-    t.K = f->tile_k ? f->tile_k : 1;
+    t.K = f->tile_k ? f->tile_k : 2;
     t.P = f->tile_p ? f->tile_p : 1;
     t.order = f->order_mode >= 0 ? f->order_mode : (P >= 8192 ? 1 : 0);
-    t.rank = f->rank_mode >= 0 ? f->rank_mode : 0;
+    t.rank = f->rank_mode >= 0 ? f->rank_mode : 1;
     const size_t groups = (P + t.K - 1) / t.K;
     size_t len;
     if (f->tile_chunk) {
@@ -651,16 +641,13 @@ Tile choose_tile(const tsl_filter* f, size_t P, size_t N)
     return t;
 }
 
-uint32_t batch_chunk(uint64_t batch_cost, uint32_t grid, int K, size_t N, uint32_t min_chunk)
+uint32_t batch_chunk(size_t N)
 {
-    // the whole scan per item while an item (kWaves*K particles x the
-    // scan) is at most half a workgroup's share of the batch; shorter
-    // chunks (multiples of 64) only when the batch is too small for that
-    const uint64_t per = (uint64_t)kWaves * (uint64_t)(K > 0 ? K : 1);
-    const uint64_t share = batch_cost / (2ull * (grid ? grid : 1u));
-    uint64_t len = (share / per + 63) / 64 * 64;
-    len = std::max<uint64_t>(len, std::max<uint32_t>(min_chunk, 256u));
-    len = std::min<uint64_t>(len, (N + 63) / 64 * 64);
+    // 2048-point chunks: items of kWaves*K particles x 2048 points keep the
+    // workgroups of an XCD on neighbouring particle groups and scan chunks
+    // while they are claimed dynamically (measured best on the batched
+    // filters workload against 8192 points and whole scans)
+    const uint64_t len = std::min<uint64_t>(2048, (N + 63) / 64 * 64);
     return (uint32_t)(len ? len : 64);
 }
 
@@ -806,9 +793,11 @@ static int launch_eval_core(tsl_filter* f, hipStream_t stream, Tile& tile)
 }
 
 // Time every candidate tile shape on the current particles and scan; keep
-// the fastest.  Candidates: particle order {caller, spatial} x K in {1, 2}
-// x points per lane in {1, 2, 4} x {automatic chunks, 1/2/4/8 chunks of the
-// scan}.
+// the fastest.  Candidates: particle order {caller, spatial} x block lookup
+// x K in {1, 2} x points per lane in {1, 2, 4} x {automatic chunks, 1/2/4/8
+// chunks of the scan}.  A candidate's time is its evaluation kernel plus the
+// per-update cost of its particle order (pose matrices, and for the spatial
+// order the key computation and sort).
 int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_order, int* best_k, int* best_p,
                   uint32_t* best_chunk, float* best_ms)
 {
@@ -831,7 +820,16 @@ int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_order, 
     for (int order = 0; order < 2 && rc == TSL_OK; ++order) {
         f->order_mode = order;
         Tile pt = choose_tile(f, P, N);
-        rc = launch_pose_for_tile(f, pt, stream);
+        rc = launch_pose_for_tile(f, pt, stream);  // warm-up
+        float pose_ms = 0.0f;
+        if (rc == TSL_OK) {
+            (void)hipEventRecord(a, stream);
+            for (int r = 0; r < reps && rc == TSL_OK; ++r) rc = launch_pose_for_tile(f, pt, stream);
+            (void)hipEventRecord(b, stream);
+            (void)hipEventSynchronize(b);
+            (void)hipEventElapsedTime(&pose_ms, a, b);
+            pose_ms /= (float)(reps > 0 ? reps : 1);
+        }
         for (int rk = 0; rk < ranks && rc == TSL_OK; ++rk) {
         f->rank_mode = rk;
         for (int ki = 0; ki < 2 && rc == TSL_OK; ++ki) {
@@ -852,7 +850,7 @@ int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_order, 
                     (void)hipEventSynchronize(b);
                     float ms = 0.0f;
                     (void)hipEventElapsedTime(&ms, a, b);
-                    ms /= (float)(reps > 0 ? reps : 1);
+                    ms = ms / (float)(reps > 0 ? reps : 1) + pose_ms;
                     if (ms < best) {
                         best = ms;
                         bo = order;

@@ -68,7 +68,7 @@ struct tsl_server {
     int device = 0;
     uint64_t budget = 0, used = 0;
     uint32_t max_batch = 64;
-    uint32_t chunk_len = 2048;
+    uint32_t chunk_len = 0;        // 0: each request's whole scan is one chunk
     int K = 2;                     // particles per wavefront of the batched evaluation
     uint64_t max_particles = 0;    // per robot; sizes the shared reduction grid
     std::vector<Robot> robots;     // fixed size (max_robots): entries never move
@@ -142,7 +142,7 @@ int dispatch(tsl_server* s, Slot* sl)
                                              hipMemcpyHostToDevice, st));
         f->num_points = np;
         TSL_TRY(launch_prep_points(f, st));
-        const uint32_t chunk = s->chunk_len;
+        const uint32_t chunk = s->chunk_len ? s->chunk_len : (uint32_t)std::max<size_t>(64, (np + 63) / 64 * 64);
         const uint32_t chunks = (uint32_t)std::max<size_t>(1, (np + chunk - 1) / chunk);
         Member& h = sl->members_host[i];
         std::memset(&h, 0, sizeof(h));
@@ -321,7 +321,7 @@ int tsl_server_create(tsl_evaluator* ev, uint64_t hbm_budget_bytes, uint32_t max
     s->device = ev->device;
     s->budget = hbm_budget_bytes;
     s->max_batch = max_batch;
-    s->chunk_len = chunk_len ? (chunk_len + 63) / 64 * 64 : 2048;
+    s->chunk_len = chunk_len ? (chunk_len + 63) / 64 * 64 : 0;
     s->max_particles = max_particles;
     s->robots.resize(max_robots);
     s->t0 = std::chrono::steady_clock::now();
@@ -406,7 +406,7 @@ int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n,
     // everything a tick can touch, reserved now: ticks never allocate
     if (rc == TSL_OK) rc = filter_reserve_particles(f, cap);
     if (rc == TSL_OK) rc = filter_reserve_scan(f, max_points);
-    const size_t chunks = ((size_t)max_points + s->chunk_len - 1) / s->chunk_len;
+    const size_t chunks = s->chunk_len ? ((size_t)max_points + s->chunk_len - 1) / s->chunk_len : 1;
     if (rc == TSL_OK) rc = f->partial.reserve(sizeof(float) * std::max<size_t>(chunks, 1) * f->capacity);
     if (rc == TSL_OK) rc = f->ubuf.reserve(sizeof(float) * f->capacity);
     if (rc == TSL_OK) rc = f->plan.reserve(sizeof(RsPlan));

@@ -230,12 +230,15 @@ struct Member {
 constexpr uint32_t kMaxMembers = 1024;  // filters in one batch
 
 // Work plan of a batched evaluation, built on the device each step from the
-// members' device counts: cost[f] is the exclusive prefix over members of
-// groups_f * kWaves*K * N_f (particle slots x scan points), cost[n] the
-// total.  Workgroups split the total cost evenly.
+// members' device counts: member f owns items [prefix[f], prefix[f+1]), one
+// item = kWaves*K particles x one chunk of its scan.  The workgroups of XCD
+// x claim items x, x + 8, x + 16, ... from counter next[x * 32] (counters
+// 128 B apart; reset by the plan kernel).
+constexpr uint32_t kPlanXcds = 8;
 struct BatchPlan {
-    uint32_t n, pad[3];
-    uint64_t cost[kMaxMembers + 1];
+    uint32_t n, total, pad[30];
+    uint32_t next[kPlanXcds * 32];
+    uint32_t prefix[kMaxMembers + 1];
 };
 
 // Upper bound on the resampling output for M particles whose weights sum
@@ -375,9 +378,8 @@ struct BatchLaunch {
 };
 int launch_batch_update(const BatchLaunch& b, hipStream_t stream);
 int batch_grid(const BatchLaunch& b, uint32_t* grid);
-// Chunk length of one member of a batch whose members hold batch_cost
-// particle-point pairs in all, for a persistent grid of `grid` workgroups.
-uint32_t batch_chunk(uint64_t batch_cost, uint32_t grid, int K, size_t N, uint32_t min_chunk);
+// Automatic chunk length of a batch member with an N-point scan.
+uint32_t batch_chunk(size_t N);
 int launch_batch_restore(const BatchLaunch& b, hipStream_t stream);
 Tile choose_tile(const tsl_filter* f, size_t P, size_t N);
 

@@ -378,7 +378,7 @@ class DeviceFilter:
     def autotune(self, stream=None, reps=3):
         """Pick the fastest evaluation shape on the current particles and
         scan; returns (order, K, points per lane, chunk_len (0 = automatic),
-        ms per evaluation)."""
+        ms per update of the evaluation kernel plus the particle order)."""
         o = ctypes.c_int()
         k = ctypes.c_int()
         pp = ctypes.c_int()

@@ -10,7 +10,7 @@ from .evaluator import _particles, _points, _pose_dict, _tf
 
 
 class Server:
-    def __init__(self, evaluator, hbm_budget=0, max_robots=256, max_particles=20_000, max_batch=64, chunk_len=2048):
+    def __init__(self, evaluator, hbm_budget=0, max_robots=256, max_particles=20_000, max_batch=64, chunk_len=0):
         self.evaluator = evaluator
         self.handle = ctypes.c_void_p()
         check(lib.tsl_server_create(evaluator.handle, int(hbm_budget), int(max_robots), int(max_particles),
