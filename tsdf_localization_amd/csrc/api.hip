@@ -226,7 +226,11 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     bool brick = sd >= 4 && sd % 4 == 0 && sd3 < (1ull << 30);
     for (uint64_t b = 0; brick && b < c.grid_occ_size; ++b)
         if (grid_occ[b] >= 0 && (uint64_t)grid_occ[b] % sd3 != 0) brick = false;
-    const std::vector<uint16_t>* upload = &codes;
+    // device codes are byte offsets into the palette (code * 4 < 2^14), so a
+    // lookup addresses the LDS palette without scaling
+    std::vector<uint16_t> scaled(codes.size());
+    for (size_t i = 0; i < codes.size(); ++i) scaled[i] = (uint16_t)(codes[i] * 4u);
+    const std::vector<uint16_t>* upload = &scaled;
     std::vector<uint16_t> phys;
     uint64_t phys_size = c.data_size;
     if (brick && c.data_size) {
@@ -238,7 +242,7 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
             const uint64_t lz = rem / (sd * sd), ly = (rem / sd) % sd, lx = rem % sd;
             const uint64_t off = ((lx >> 2) + (ly >> 2) * B + (lz >> 2) * B * B) * 64 + (lx & 3) + (ly & 3) * 4 +
                                  (lz & 3) * 16;
-            phys[slot * sd3 + off] = codes[idx];
+            phys[slot * sd3 + off] = scaled[idx];
         }
         upload = &phys;
     }
@@ -252,7 +256,9 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     if (phys_size)
         TSL_HIP_CHECK(hipMemcpy(m->codes.ptr, upload->data(), sizeof(uint16_t) * phys_size,
                                 hipMemcpyHostToDevice));
-    TSL_HIP_CHECK(hipMemcpy(m->codes.as<uint16_t>() + phys_size, &init_code, sizeof(uint16_t), hipMemcpyHostToDevice));
+    const uint16_t init_offset = (uint16_t)(init_code * 4u);
+    TSL_HIP_CHECK(hipMemcpy(m->codes.as<uint16_t>() + phys_size, &init_offset, sizeof(uint16_t),
+                            hipMemcpyHostToDevice));
     if (!palette.empty())
         TSL_HIP_CHECK(hipMemcpy(m->palette.ptr, palette.data(), sizeof(float) * palette.size(),
                                 hipMemcpyHostToDevice));
@@ -314,7 +320,7 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     d.brick = brick ? 1u : 0u;
     d.sd3 = (uint32_t)sd3;
     d.sentinel = (uint32_t)phys_size;
-    d.sentinel_code = init_code;
+    d.sentinel_code = init_code * 4u;
     {
         // occupancy-rank form of the block table
         // (rank < 2^17 and sub_dim^3 < 2^24 keep the device product a

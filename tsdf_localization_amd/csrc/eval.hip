@@ -118,17 +118,33 @@ struct LdsTables {
     const uint64_t* neg;
     const SubEntry* tab;
     const float* lut;
+    float4* tile;       // kTile staged scan points
+    uint32_t* scratch;  // 4 words for workgroup-wide values
 };
+
+// Palette value at byte offset off (a voxel code).
+__device__ __forceinline__ float palette_at(const float* lut, uint32_t off)
+{
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(lut) + off);
+}
 
 template <bool kRank>
 __device__ __forceinline__ LdsTables stage_tables(const DevMap& map, bool scaled, float a_hit)
 {
     extern __shared__ uint4 smem_raw[];
-    AxisEntry* atab = reinterpret_cast<AxisEntry*>(smem_raw);  // 16-byte entries first (aligned)
+    // palette first (voxel codes are byte offsets from the start of the
+    // dynamic LDS), padded to 16 bytes for the 16-byte axis entries
+    float* lut = reinterpret_cast<float*>(smem_raw);
+    AxisEntry* atab = reinterpret_cast<AxisEntry*>(smem_raw + (map.palette_size + 3) / 4);
     OccWord* occ = reinterpret_cast<OccWord*>(atab + map.axis_n);
     uint64_t* neg = reinterpret_cast<uint64_t*>(occ + (kRank ? map.occ_nwords : 0u));
     SubEntry* tab = reinterpret_cast<SubEntry*>(neg + 8);
-    float* lut = reinterpret_cast<float*>(tab + map.sub_entries);
+    // the point tile and the workgroup words follow, 16-byte aligned (the
+    // kernels declare no static LDS, so the palette sits at address 0)
+    const uint32_t tables_end = (uint32_t)(reinterpret_cast<char*>(tab + map.sub_entries) -
+                                           reinterpret_cast<char*>(smem_raw));
+    float4* tile = reinterpret_cast<float4*>(reinterpret_cast<char*>(smem_raw) + ((tables_end + 15u) & ~15u));
+    uint32_t* scratch = reinterpret_cast<uint32_t*>(tile + kTile);
     {
         const uint4* src = reinterpret_cast<const uint4*>(map.axis_tab);
         uint4* dst = reinterpret_cast<uint4*>(atab);
@@ -144,14 +160,16 @@ __device__ __forceinline__ LdsTables stage_tables(const DevMap& map, bool scaled
     for (uint32_t i = threadIdx.x; i < map.palette_size; i += blockDim.x)
         lut[i] = scaled ? a_hit * map.palette[i] : map.palette[i];
     __syncthreads();
-    return LdsTables{atab, occ, neg, tab, lut};
+    return LdsTables{atab, occ, neg, tab, lut, tile, scratch};
 }
 
 // LDS bytes stage_tables needs for a map.
 inline size_t lds_bytes(const DevMap& d, bool rank)
 {
-    return sizeof(AxisEntry) * d.axis_n + (rank ? sizeof(OccWord) * d.occ_nwords : 0) + 8 * sizeof(uint64_t) +
-           sizeof(SubEntry) * d.sub_entries + sizeof(float) * d.palette_size;
+    const size_t tables = 16 * ((d.palette_size + 3) / 4) + sizeof(AxisEntry) * d.axis_n +
+                          (rank ? sizeof(OccWord) * d.occ_nwords : 0) + 8 * sizeof(uint64_t) +
+                          sizeof(SubEntry) * d.sub_entries;
+    return (tables + 15) / 16 * 16 + sizeof(float4) * kTile + 16;
 }
 
 // One wavefront: K particles (pose matrices wavefront-uniform) x the points
@@ -275,7 +293,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll
             for (int u = 0; u < kP; ++u) {
 #pragma unroll
-                for (int k = 0; k < K; ++k) v[u][k] = lut[code[u][k]];
+                for (int k = 0; k < K; ++k) v[u][k] = palette_at(lut, code[u][k]);
             }
 #pragma unroll
             for (int u = 0; u < kP; ++u) {
@@ -294,24 +312,19 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
                 }
             }
         } else {
-            // exact tiers, one point group at a time (offsets recomputed from
-            // the point, bit-identical); same per-lane summation order
-#pragma unroll 1
+            // exact tiers on the same offsets; same per-lane summation order
+#pragma unroll
             for (int u = 0; u < kP; ++u) {
                 const uint32_t j = base + 64u * u + lane;
-                const bool a = kFull || j < te;
-                const float4 qq = tile[a ? j - tb : 0u];
+                const bool a = act[u];
+                const float4 qq = q[u];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const float tx = m[k][0] * qq.x + m[k][1] * qq.y + m[k][2] * qq.z + m[k][3];
-                    const float ty = m[k][4] * qq.x + m[k][5] * qq.y + m[k][6] * qq.z + m[k][7];
-                    const float tz = m[k][8] * qq.x + m[k][9] * qq.y + m[k][10] * qq.z + m[k][11];
-                    const uint64_t full =
-                        slow_index<kU24, kRank>(map, tab, neg, occ, tx - map.min_x, ty - map.min_y, tz - map.min_z);
+                    const uint64_t full = slow_index<kU24, kRank>(map, tab, neg, occ, ox[u][k], oy[u][k], oz[u][k]);
                     float v = kTrace ? map.init : a_hit * map.init;
                     if (full < map.data_size) {
                         const uint32_t x = (uint32_t)full;
-                        v = lut[map.codes[kBrick ? phys_from_linear(map, x) : x]];
+                        v = palette_at(lut, map.codes[kBrick ? phys_from_linear(map, x) : x]);
                     }
                     if (kMode == 1) {
                         if (a && p0 + k < P) static_cast<uint64_t*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = full;
@@ -362,10 +375,9 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
     if (g * kWaves * K >= P) return;  // the whole workgroup is idle (uniform across its waves)
     const uint32_t j0 = c * chunk_len;
     const uint32_t j1 = min(N, j0 + chunk_len);
-    __shared__ float4 tile[kTile];
     eval_wave<K, kP, kMode, kU24, kBrick, kRank>(map, t, L, P, p0, p0 < P, pts, N, j0, j1, a_hit,
                                                  kMode == 0 ? partial + (size_t)c * stride : nullptr,
-                                                 trace_out, perm, tile);
+                                                 trace_out, perm, t.tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -547,8 +559,8 @@ __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member*
                                                        BatchPlan* __restrict__ plan, float a_hit)
 {
     const LdsTables t = stage_tables<kRank>(map, true, a_hit);
-    __shared__ float4 tile[kTile];
-    __shared__ uint32_t claim[2];
+    float4* tile = t.tile;
+    uint32_t* claim = t.scratch;
     const uint32_t n = plan->n;
     const uint32_t total = plan->total;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

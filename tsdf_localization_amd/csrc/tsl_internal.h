@@ -124,7 +124,7 @@ struct DevMapFull {
 // Kernel-argument view of a map: what the fast tier reads every iteration.
 struct DevMap {
     const int32_t* grid_occ;   // block table: voxel offset of the block or -1
-    const uint16_t* codes;     // data_size palette codes, reference voxel order
+    const uint16_t* codes;     // per voxel: byte offset (4 * palette code) of its value in the palette
     const float* palette;      // palette_size values
     const SubEntry* sub_tab;   // sub-voxel threshold buckets
     const uint64_t* neg_tab;   // index contribution of the wrap-band axes, by axis bit mask (8)
@@ -157,7 +157,7 @@ struct DevMap {
     const OccWord* occ_words;
     uint32_t occ_nwords;              // 0: use grid_occ
     uint32_t sentinel;                // element of codes holding the code of init_value
-    uint32_t sentinel_code;           // that code
+    uint32_t sentinel_code;           // that byte offset
     uint32_t pad1;
 };
 
