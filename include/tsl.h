@@ -420,8 +420,10 @@ typedef struct tsl_result {
     uint32_t pad;
 } tsl_result;
 
-/* hbm_budget_bytes: device bytes the server may hold (map included; 0 = no
- * limit); max_robots resident at once; max_particles per robot;
+/* hbm_budget_bytes: device bytes the map and the resident robots may hold
+ * (0 = no limit; the tick buffers -- three ticks' scan staging of max_batch
+ * scans of the largest admitted robot -- are not charged); max_robots
+ * resident at once; max_particles per robot;
  * max_batch requests per tick; chunk_len scan points per evaluation chunk
  * (0 = each request's whole scan).  A request's result depends on its own
  * chunking only, never on the tick it ran in. */

@@ -142,3 +142,35 @@ def test_server_many_requests_per_robot_in_order(tsl, room):
             assert [r["id"] for r in sorted(mine, key=lambda r: r["done_ms"])] == [s * 10 + i for s in range(4)]
     finally:
         srv.close()
+
+
+def test_server_tick_scan_preparation_paths_agree(tsl, room):
+    """Ticks of few scans sort each scan on its own; larger ticks sort all of
+    them in one segmented sort.  Both give the same points in the same order:
+    every robot run alone equals its twin run in a large tick, bit for bit."""
+    from tsdf_localization_amd.server import Server
+    ev = tsl.TSDFEvaluator(room.map)
+    robots = _robots(room, 12, seed=11)
+    srv = Server(ev, max_robots=32, max_particles=2000, max_batch=16)
+    try:
+        ids_a = [srv.add_robot(p, 3000, room.tf) for p, _ in robots]
+        ids_b = [srv.add_robot(p, 3000, room.tf) for p, _ in robots]
+        u0 = [np.float32(0.27 / p.shape[0]) for p, _ in robots]
+        alone = {}
+        for i, (p, pts) in enumerate(robots):
+            srv.submit(ids_a[i], _motion(tsl, i), pts, u0[i], request_id=i)
+            r = srv.wait(1)[0]
+            assert r["batch"] == 1 and r["status"] == 0
+            alone[i] = r
+        for i, (p, pts) in enumerate(robots):
+            srv.submit(ids_b[i], _motion(tsl, i), pts, u0[i], request_id=100 + i)
+        mix = {r["id"] - 100: r for r in srv.wait(len(robots))}
+        assert max(r["batch"] for r in mix.values()) >= 4, "no tick large enough for the segmented sort"
+        for i in range(len(robots)):
+            assert mix[i]["status"] == 0
+            assert mix[i]["pose"] == alone[i]["pose"] and mix[i]["count"] == alone[i]["count"], i
+            pa = srv.get_particles(ids_a[i])
+            pb = srv.get_particles(ids_b[i])
+            assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32)), i
+    finally:
+        srv.close()

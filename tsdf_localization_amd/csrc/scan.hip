@@ -325,4 +325,52 @@ int launch_prep_points(tsl_filter* f, hipStream_t stream)
     return TSL_OK;
 }
 
+namespace {
+
+// Sorted position i of scan blockIdx.y: the point and its score constant,
+// as k_prep_sorted writes them for one filter.
+__global__ void k_prep_sorted_seg(const tsl_point* __restrict__ in, const uint32_t* __restrict__ off,
+                                  const uint32_t* __restrict__ sorted_idx, float range_term, float a_max, float mr2,
+                                  const ScanDest* __restrict__ dst)
+{
+    const uint32_t r = blockIdx.y;
+    const uint32_t b = off[r], n = off[r + 1] - b;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t src = sorted_idx[b + i];
+    const tsl_point p = in[src];
+    const float d2 = p.x * p.x + p.y * p.y + p.z * p.z;
+    dst[r].points[i] = make_float4(p.x, p.y, p.z, d2 < mr2 ? range_term : a_max);
+    dst[r].perm[i] = src - b;
+}
+
+}  // namespace
+
+size_t prep_points_batch_temp(uint32_t total, uint32_t n)
+{
+    size_t temp = 0;
+    if (total == 0 || n == 0) return 0;
+    if (hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, temp, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                    (uint32_t*)nullptr, (uint32_t*)nullptr, (int)total, (int)n,
+                                                    (const uint32_t*)nullptr, (const uint32_t*)nullptr, 0, 63,
+                                                    (hipStream_t)0) != hipSuccess)
+        return 0;
+    return temp;
+}
+
+int launch_prep_points_batch(const tsl_evaluator* ev, const tsl_point* pts, const uint32_t* off, const ScanDest* dst,
+                             uint32_t n, uint32_t total, uint32_t max_count, uint64_t* keys, uint32_t* vals,
+                             void* temp, size_t temp_bytes, hipStream_t stream)
+{
+    if (n == 0 || total == 0) return TSL_OK;
+    hipLaunchKernelGGL(k_morton, dim3((total + 255) / 256), dim3(256), 0, stream, pts, total, keys, vals);
+    TSL_HIP_CHECK(hipGetLastError());
+    TSL_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortPairs(temp, temp_bytes, keys, keys + total, vals, vals + total,
+                                                              (int)total, (int)n, off, off + 1, 0, 63, stream));
+    hipLaunchKernelGGL(k_prep_sorted_seg, dim3((max_count + 255) / 256, n), dim3(256), 0, stream, pts, off,
+                       vals + total, ev->range_term, ev->a_max, ev->max_range_squared, dst);
+    TSL_HIP_CHECK(hipGetLastError());
+    return TSL_OK;
+}
+
 }  // namespace tsl

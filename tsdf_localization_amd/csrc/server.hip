@@ -2,8 +2,9 @@
 // requests (odometry step + scan + resampling draw) arrive from any thread;
 // a worker thread keeps the GPU busy with continuous batching -- every
 // "tick" takes the pending requests of robots that are not in flight (at
-// most max_batch, oldest first), enqueues motion update, scan preparation,
-// batched sensor update and resampling for all of them, and completes
+// most max_batch, oldest first), enqueues motion update, scan preparation
+// (one upload and one segmented sort for all scans), batched sensor update
+// and resampling for all of them, and completes
 // earlier ticks as their events fire, so requests that arrive while others
 // run are admitted into the next tick at once.  Robots are admitted against
 // an HBM budget.  Each robot's result is the same whether it runs alone or
@@ -26,6 +27,7 @@ using namespace tsl;
 namespace {
 
 constexpr int kMaxInflight = 3;
+constexpr uint32_t kSegmentedScans = 4;  // ticks with this many scans prepare them with one segmented sort
 
 struct Robot {
     tsl_filter* f = nullptr;
@@ -33,7 +35,6 @@ struct Robot {
     uint64_t bytes = 0;
     bool active = false;
     bool busy = false;
-    tsl_point* staging = nullptr;  // pinned, max_points
 };
 
 struct Pending {
@@ -50,6 +51,13 @@ struct Slot {  // one tick in flight
     float* u0_host = nullptr;            // pinned
     tsl_motion* motions_host = nullptr;  // pinned
     FilterState* states_host = nullptr;  // pinned
+    // the tick's scans, prepared together: concatenated points (pinned
+    // staging and device copy), segment offsets, destinations, sort buffers
+    DevBuf scan_pts, scan_off, scan_dst, scan_keys, scan_vals, scan_temp;
+    tsl_point* scan_host = nullptr;      // pinned, scan_cap points
+    uint32_t* off_host = nullptr;        // pinned, max_batch + 1
+    ScanDest* dst_host = nullptr;        // pinned, max_batch
+    size_t scan_cap = 0;
     hipEvent_t done = nullptr;
     double start_ms = 0.0;
     bool used = false;
@@ -71,6 +79,7 @@ struct tsl_server {
     uint32_t chunk_len = 0;        // 0: each request's whole scan is one chunk
     int K = 2;                     // particles per wavefront of the batched evaluation
     uint64_t max_particles = 0;    // per robot; sizes the shared reduction grid
+    std::atomic<uint32_t> max_points{0};  // largest scan any admitted robot may send
     std::vector<Robot> robots;     // fixed size (max_robots): entries never move
     std::mutex mu;
     std::condition_variable cv_req, cv_done;
@@ -125,6 +134,31 @@ void complete(tsl_server* s, Slot* sl)
     s->cv_done.notify_all();
 }
 
+// Grow a free slot's scan buffers to hold `total` points of up to
+// max_batch scans (worker thread only, so no tick in flight uses them);
+// sized for max_batch full scans of the largest robot, so they grow at most
+// once per admission of a robot with a larger scan.
+int slot_reserve_scans(tsl_server* s, Slot& sl, size_t total)
+{
+    const size_t want = std::max<size_t>(total, (size_t)s->max_batch * s->max_points.load());
+    if (want <= sl.scan_cap && sl.scan_host) return TSL_OK;
+    const size_t cap = std::max<size_t>(want, 1);
+    if (sl.scan_host) (void)hipHostFree(sl.scan_host);
+    sl.scan_host = nullptr;
+    sl.scan_cap = 0;
+    if (hipHostMalloc(reinterpret_cast<void**>(&sl.scan_host), sizeof(tsl_point) * cap, hipHostMallocDefault) !=
+        hipSuccess)
+        return set_error(TSL_ERR_OUT_OF_MEMORY, "pinned scan staging");
+    TSL_TRY(sl.scan_pts.reserve(sizeof(tsl_point) * cap));
+    TSL_TRY(sl.scan_keys.reserve(sizeof(uint64_t) * 2 * cap));
+    TSL_TRY(sl.scan_vals.reserve(sizeof(uint32_t) * 2 * cap));
+    const size_t temp = prep_points_batch_temp((uint32_t)cap, s->max_batch);
+    if (temp == 0) return set_error(TSL_ERR_HIP, "segmented sort temporary size");
+    TSL_TRY(sl.scan_temp.reserve(temp));
+    sl.scan_cap = cap;
+    return TSL_OK;
+}
+
 // Enqueue one tick for the selected requests (robots marked busy already).
 int dispatch(tsl_server* s, Slot* sl)
 {
@@ -132,16 +166,54 @@ int dispatch(tsl_server* s, Slot* sl)
     const uint32_t n = (uint32_t)sl->reqs.size();
     uint64_t max_items = 0;
     uint32_t max_cap = 0;
+    // every scan of the tick: one upload, one segmented sort
+    uint32_t total = 0, max_np = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        sl->off_host[i] = total;
+        const uint32_t np = (uint32_t)sl->reqs[i].points.size();
+        total += np;
+        max_np = std::max(max_np, np);
+    }
+    sl->off_host[n] = total;
+    TSL_TRY(slot_reserve_scans(s, *sl, total));
+    if (total) TSL_TRY(sl->scan_temp.reserve(prep_points_batch_temp(total, n)));
+    for (uint32_t i = 0; i < n; ++i) {
+        const Pending& pd = sl->reqs[i];
+        if (!pd.points.empty())
+            std::memcpy(sl->scan_host + sl->off_host[i], pd.points.data(), sizeof(tsl_point) * pd.points.size());
+        tsl_filter* f = s->robots[sl->robots[i]].f;
+        sl->dst_host[i] = ScanDest{f->points.as<float4>(), f->perm.as<uint32_t>()};
+    }
+    if (n >= kSegmentedScans) {
+        // one upload and one segmented sort (a workgroup per scan)
+        if (total)
+            TSL_HIP_CHECK(hipMemcpyAsync(sl->scan_pts.ptr, sl->scan_host, sizeof(tsl_point) * total,
+                                         hipMemcpyHostToDevice, st));
+        TSL_HIP_CHECK(
+            hipMemcpyAsync(sl->scan_off.ptr, sl->off_host, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, st));
+        TSL_HIP_CHECK(hipMemcpyAsync(sl->scan_dst.ptr, sl->dst_host, sizeof(ScanDest) * n, hipMemcpyHostToDevice, st));
+        TSL_TRY(launch_prep_points_batch(s->ev, sl->scan_pts.as<tsl_point>(), sl->scan_off.as<uint32_t>(),
+                                         sl->scan_dst.as<ScanDest>(), n, total, max_np, sl->scan_keys.as<uint64_t>(),
+                                         sl->scan_vals.as<uint32_t>(), sl->scan_temp.ptr, sl->scan_temp.bytes, st));
+    } else {
+        // few scans: the device-wide sort of each (same keys, same stable
+        // order, so the same points in the same order)
+        for (uint32_t i = 0; i < n; ++i) {
+            tsl_filter* f = s->robots[sl->robots[i]].f;
+            const uint32_t np = sl->off_host[i + 1] - sl->off_host[i];
+            f->num_points = np;
+            if (np)
+                TSL_HIP_CHECK(hipMemcpyAsync(f->points_in.ptr, sl->scan_host + sl->off_host[i], sizeof(tsl_point) * np,
+                                             hipMemcpyHostToDevice, st));
+            TSL_TRY(launch_prep_points(f, st));
+        }
+    }
     for (uint32_t i = 0; i < n; ++i) {
         Robot& rb = s->robots[sl->robots[i]];
         tsl_filter* f = rb.f;
         const Pending& pd = sl->reqs[i];
         const size_t np = pd.points.size();
-        if (np) std::memcpy(rb.staging, pd.points.data(), sizeof(tsl_point) * np);
-        if (np) TSL_HIP_CHECK(hipMemcpyAsync(f->points_in.ptr, rb.staging, sizeof(tsl_point) * np,
-                                             hipMemcpyHostToDevice, st));
         f->num_points = np;
-        TSL_TRY(launch_prep_points(f, st));
         const uint32_t chunk = s->chunk_len ? s->chunk_len : (uint32_t)std::max<size_t>(64, (np + 63) / 64 * 64);
         const uint32_t chunks = (uint32_t)std::max<size_t>(1, (np + chunk - 1) / chunk);
         Member& h = sl->members_host[i];
@@ -283,8 +355,13 @@ int slot_init(tsl_server* s, Slot& sl)
         hipHostMalloc(reinterpret_cast<void**>(&sl.motions_host), sizeof(tsl_motion) * n, hipHostMallocDefault) !=
             hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&sl.states_host), sizeof(FilterState) * n, hipHostMallocDefault) !=
-            hipSuccess)
+            hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&sl.off_host), sizeof(uint32_t) * (n + 1), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&sl.dst_host), sizeof(ScanDest) * n, hipHostMallocDefault) != hipSuccess)
         return set_error(TSL_ERR_OUT_OF_MEMORY, "pinned host buffers");
+    TSL_TRY(sl.scan_off.reserve(sizeof(uint32_t) * (n + 1)));
+    TSL_TRY(sl.scan_dst.reserve(sizeof(ScanDest) * n));
     TSL_HIP_CHECK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     return TSL_OK;
 }
@@ -372,15 +449,16 @@ int tsl_server_destroy(tsl_server* s)
     }
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (Robot& r : s->robots) {
+    for (Robot& r : s->robots)
         if (r.f) tsl_filter_destroy(r.f);
-        if (r.staging) (void)hipHostFree(r.staging);
-    }
     for (Slot& sl : s->slots) {
         if (sl.members_host) (void)hipHostFree(sl.members_host);
         if (sl.u0_host) (void)hipHostFree(sl.u0_host);
         if (sl.motions_host) (void)hipHostFree(sl.motions_host);
         if (sl.states_host) (void)hipHostFree(sl.states_host);
+        if (sl.off_host) (void)hipHostFree(sl.off_host);
+        if (sl.dst_host) (void)hipHostFree(sl.dst_host);
+        if (sl.scan_host) (void)hipHostFree(sl.scan_host);
         if (sl.done) (void)hipEventDestroy(sl.done);
     }
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -413,13 +491,8 @@ int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n,
     // the reduction grid of a tick spans its largest member
     const uint64_t nb_max = (resample_bound(s->max_particles) + 255) / 256;
     if (rc == TSL_OK) rc = f->blocks.reserve(sizeof(double) * 9 * nb_max);
-    tsl_point* staging = nullptr;
-    if (rc == TSL_OK && hipHostMalloc(reinterpret_cast<void**>(&staging), sizeof(tsl_point) * max_points,
-                                      hipHostMallocDefault) != hipSuccess)
-        rc = set_error(TSL_ERR_OUT_OF_MEMORY, "pinned scan staging");
     if (rc == TSL_OK) std::memcpy(f->tf, tf, sizeof(float) * 16);
     if (rc != TSL_OK) {
-        if (staging) (void)hipHostFree(staging);
         tsl_filter_destroy(f);
         return rc;
     }
@@ -428,7 +501,6 @@ int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n,
     uint32_t slot = 0;
     while (slot < s->robots.size() && (s->robots[slot].active || s->robots[slot].busy)) ++slot;
     if (slot == s->robots.size() || (s->budget && s->used + bytes > s->budget)) {
-        (void)hipHostFree(staging);
         tsl_filter_destroy(f);
         if (slot == s->robots.size())
             return set_error(TSL_ERR_ADMISSION, "server holds its maximum of %zu robots", s->robots.size());
@@ -442,7 +514,7 @@ int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n,
     r.bytes = bytes;
     r.active = true;
     r.busy = false;
-    r.staging = staging;
+    if (max_points > s->max_points.load()) s->max_points.store(max_points);
     f->count_bound = f->capacity;
     f->count_known = false;
     *robot = slot;
@@ -462,9 +534,7 @@ int tsl_server_remove_robot(tsl_server* s, uint32_t robot)
     r.active = false;
     s->used -= r.bytes;
     tsl_filter_destroy(r.f);
-    (void)hipHostFree(r.staging);
     r.f = nullptr;
-    r.staging = nullptr;
     return TSL_OK;
 }
 

@@ -347,6 +347,19 @@ struct Tile {
 int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const float tf[16],
                          float* d_out, hipStream_t stream);
 int launch_prep_points(tsl_filter* f, hipStream_t stream);
+
+// Scan preparation of many scans at once (a serving tick): scan r is
+// points [off[r], off[r+1]) of pts; each is Morton-sorted within its own
+// segment exactly as launch_prep_points sorts a filter's scan (same keys,
+// stable order) and written to dst[r].
+struct ScanDest {
+    float4* points;  // sorted points with their score constant
+    uint32_t* perm;  // sorted position -> index in the scan
+};
+size_t prep_points_batch_temp(uint32_t total, uint32_t n);
+int launch_prep_points_batch(const tsl_evaluator* ev, const tsl_point* pts, const uint32_t* off, const ScanDest* dst,
+                             uint32_t n, uint32_t total, uint32_t max_count, uint64_t* keys, uint32_t* vals,
+                             void* temp, size_t temp_bytes, hipStream_t stream);
 int launch_particle_order(tsl_filter* f, hipStream_t stream);
 int filter_reserve_scan(tsl_filter* f, size_t n);
 int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring, uint32_t n, float cell,
