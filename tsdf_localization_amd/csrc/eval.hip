@@ -56,6 +56,25 @@ __device__ double block_sum_d(double v, double* sh)
     return r;
 }
 
+// Nine block sums in one pass: thread k < 9 returns the total of v[k] over
+// the workgroup, each formed exactly as block_sum_d forms it (wave sums,
+// then the waves in order).
+__device__ double block_sum9(const double (&v)[9], double* sh /* kWaves * 9 */)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const double w = wave_sum_d(v[k]);
+        if (lane == 0) sh[wave * 9 + k] = w;
+    }
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x < 9)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w * 9 + threadIdx.x];
+    __syncthreads();
+    return r;
+}
+
 struct Tf12 {
     float m[16];
 };
@@ -453,7 +472,7 @@ __global__ __launch_bounds__(kBlock) void k_normalize_m(const Member* __restrict
 {
     const Member& m = members[blockIdx.y];
     FilterState* st = m.st;
-    __shared__ double sh[kWaves];
+    __shared__ double sh[kWaves * 9];
     __shared__ bool last;
     const uint32_t P = st->count;
     const float sum = (float)st->stats[0];
@@ -477,26 +496,28 @@ __global__ __launch_bounds__(kBlock) void k_normalize_m(const Member* __restrict
             q[4 + 2 * a] = (double)c * wd;
         }
     }
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const double b = block_sum_d(q[k], sh);
-        if (threadIdx.x == 0) m.blocks[(size_t)k * gridDim.x + blockIdx.x] = b;
+    {
+        const double b = block_sum9(q, sh);
+        if (threadIdx.x < 9) m.blocks[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = b;
     }
+    __threadfence();
+    __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
         const uint32_t done = atomicAdd(&st->counter1, 1u);
         last = (done == gridDim.x - 1);
     }
     __syncthreads();
     if (last) {
         __threadfence();
+        double t[9];
+#pragma unroll
         for (int qq = 0; qq < 9; ++qq) {
-            double t = 0.0;
+            t[qq] = 0.0;
             for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x)
-                t += *(volatile const double*)&m.blocks[(size_t)qq * gridDim.x + k];
-            const double total = block_sum_d(t, sh);
-            if (threadIdx.x == 0) st->stats[1 + qq] = total;
+                t[qq] += *(volatile const double*)&m.blocks[(size_t)qq * gridDim.x + k];
         }
+        const double total = block_sum9(t, sh);
+        if (threadIdx.x < 9) st->stats[1 + threadIdx.x] = total;
         if (threadIdx.x == 0) st->counter1 = 0;
     }
 }

@@ -38,11 +38,19 @@ __host__ __device__ __forceinline__ float rec_step(float u, double inv_m) { retu
 
 // Candidates needed for weights summing to at most 1 + 1e-5: the number of
 // U_j below 1 + 1e-5, plus one; 0 if the draw does not advance.  Walks the
-// runs of constant float step without materialising them.
-__host__ __device__ uint64_t cap_walk(uint64_t M, float u0, double inv_m)
+// runs of the U recurrence without materialising them: inside a binade the
+// float step is constant (rounding of U + 1/M depends only on the binade
+// once U is on the float grid), so U_j = U_{j0} + (j - j0)*step exactly.
+// Runs end before the next power of two; a step that differs from its
+// successor (round-to-even ties, U = 0, subnormals) becomes a run of length
+// 1.  With seg, the first kMaxSeg runs are also recorded as the run table
+// the kernels rebuild U from; they re-check every element, so the table
+// only needs to be right to be fast, not to be exact.
+__host__ __device__ uint64_t cap_walk(uint64_t M, float u0, double inv_m, USeg* seg = nullptr, uint32_t* nseg = nullptr)
 {
     const double stop = 1.0 + 1e-5;
     uint64_t j = 0;
+    uint32_t n = 0;
     float u = u0;
     while ((double)u < stop) {
         const float n1 = rec_step(u, inv_m);
@@ -62,49 +70,18 @@ __host__ __device__ uint64_t cap_walk(uint64_t M, float u0, double inv_m)
             if ((double)rec_step((float)last, inv_m) != last + d || (double)rec_step(n1, inv_m) - (double)n1 != d)
                 L = 1;
         }
+        if (seg && n < (uint32_t)kMaxSeg) {
+            seg[n].j0 = (uint32_t)j;
+            seg[n].u = u;
+            seg[n].step = d;
+            ++n;
+        }
         j += L;
         u = (float)((double)u + (double)L * d);
         if (j > 4 * M + 1024) break;
     }
+    if (nseg) *nseg = n;
     return j + 1;
-}
-
-// Run table of the U recurrence: inside a binade the float step is constant
-// (rounding of U + 1/M depends only on the binade once U is on the float
-// grid), so U_j = U_{j0} + (j - j0)*step exactly.  Runs end before the next
-// power of two; a step that differs from its successor (round-to-even ties,
-// U = 0, subnormals) becomes a run of length 1.  The kernels re-check every
-// element, so the table only needs to be right to be fast, not to be exact.
-__device__ uint32_t build_segments(double inv_m, float u0, uint32_t cap, USeg* seg)
-{
-    uint32_t n = 0, j = 0;
-    float u = u0;
-    while (j < cap && n < (uint32_t)kMaxSeg) {
-        const float n1 = rec_step(u, inv_m);
-        const double d = (double)n1 - (double)u;
-        uint64_t L = 1;
-        if (u >= 1.1754943508222875e-38f && d > 0.0) {
-            int e;
-            (void)frexp((double)u, &e);
-            const double top = ldexp(1.0, e);
-            const double n2 = (double)rec_step(n1, inv_m) - (double)n1;
-            if (n2 == d) {
-                double steps = ceil((top - (double)u) / d) - 1.0;
-                if (steps < 1.0) steps = 1.0;
-                if (steps > (double)(cap - j)) steps = (double)(cap - j);
-                L = (uint64_t)steps;
-                const double last = (double)u + (double)(L - 1) * d;
-                if ((double)rec_step((float)last, inv_m) != last + d) L = 1;
-            }
-        }
-        seg[n].j0 = j;
-        seg[n].u = u;
-        seg[n].step = d;
-        ++n;
-        j += (uint32_t)L;
-        u = (float)((double)u + (double)L * d);
-    }
-    return n;  // candidates beyond the table are repaired serially
 }
 
 // Plan (one thread per member): M from the device count, cap, run table.
@@ -129,7 +106,8 @@ __global__ void k_rs_plan(Member* __restrict__ members, float u0_arg)
         return;
     }
     const double inv_m = 1.0 / (double)M;
-    const uint64_t cap = cap_walk(M, u0, inv_m);
+    uint32_t nseg = 0;
+    const uint64_t cap = cap_walk(M, u0, inv_m, p->seg, &nseg);
     if (cap == 0) {
         st->status = kStNoAdvance;
         return;
@@ -139,7 +117,7 @@ __global__ void k_rs_plan(Member* __restrict__ members, float u0_arg)
         return;
     }
     p->inv_m = inv_m;
-    p->nseg = build_segments(inv_m, u0, (uint32_t)cap, p->seg);
+    p->nseg = nseg;
     p->cap = (uint32_t)cap;
 }
 
