@@ -325,7 +325,22 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
         // occupancy-rank form of the block table
         // (rank < 2^17 and sub_dim^3 < 2^24 keep the device product a
         // 24-bit multiply; rank * sub_dim^3 < 2^31 as a block offset)
-        const uint64_t nwords = (c.grid_occ_size + 31) / 32;
+        // (the word count is padded to a power of two, so any block index's
+        // word bits address a word inside the table)
+        // The table also covers every block index a lane inside the grid
+        // check can form (gthr may exceed a block dim by rounding), so the
+        // lookup needs no separate up < grid_occ_size test: such indices
+        // read zero padding (unallocated).
+        const uint64_t max_up = (uint64_t)(fm.gthr_x ? fm.gthr_x - 1 : 0) +
+                                (uint64_t)(fm.gthr_y ? fm.gthr_y - 1 : 0) * c.up_dim_x +
+                                (uint64_t)(fm.gthr_z ? fm.gthr_z - 1 : 0) * c.up_dim_2;
+        const uint64_t span = std::max<uint64_t>(c.grid_occ_size, max_up + 1);
+        uint64_t nwords = 1;
+        uint32_t log2w = 0;
+        while (nwords * 32 < span) {
+            nwords *= 2;
+            ++log2w;
+        }
         bool rank = nwords <= kMaxOccWords && sd3 < (1ull << 24);
         std::vector<OccWord> words;
         if (rank) {
@@ -346,6 +361,7 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
             TSL_HIP_CHECK(hipMemcpy(m->occ_words.ptr, words.data(), sizeof(OccWord) * nwords, hipMemcpyHostToDevice));
             d.occ_words = m->occ_words.as<OccWord>();
             d.occ_nwords = (uint32_t)nwords;
+            d.occ_log2 = log2w;
         }
     }
     {

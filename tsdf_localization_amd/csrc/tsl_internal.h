@@ -155,7 +155,8 @@ struct DevMap {
     // occupancy-rank form of the block table (set when the offsets follow
     // setData order and the bitmap fits the LDS budget)
     const OccWord* occ_words;
-    uint32_t occ_nwords;              // 0: use grid_occ
+    uint32_t occ_nwords;              // 0: use grid_occ; else a power of two (zero words past the blocks)
+    uint32_t occ_log2;                // log2(occ_nwords): block up reads word bits [5, 5 + occ_log2) of up
     uint32_t sentinel;                // element of codes holding the code of init_value
     uint32_t sentinel_code;           // that byte offset
     uint32_t pad1;

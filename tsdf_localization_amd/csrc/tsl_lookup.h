@@ -77,30 +77,34 @@ __device__ __forceinline__ int32_t mul_i(int32_t a, int32_t b)
     return a * b;
 }
 
-// Occupancy-rank form of the block table (LDS copy): for block up
-// (< grid_occ_size), set = allocated and the return value = rank *
+// Occupancy-rank form of the block table (LDS copy, 2^log2w words): for
+// block up (< grid_occ_size), set = allocated and the return value = rank *
 // sub_dim^3, which equals grid_occ[up] for maps whose offsets follow setData
-// order.  The rank is below 2^17 and sub_dim^3 below 2^24, so the product is
-// a 24-bit multiply.
-__device__ __forceinline__ uint32_t rank_block(const OccWord* __restrict__ occ, uint32_t up, uint32_t sd3, bool& set)
+// order.  Any up reads a word inside the table (word index = bits
+// [5, 5 + log2w) of up), so lanes outside the grid need no clamp.  The bit
+// field extracts use the low five bits of their offset and width operands,
+// i.e. up mod 32.  The rank is below 2^17 and sub_dim^3 below 2^24, so the
+// product is a 24-bit multiply.
+__device__ __forceinline__ uint32_t rank_block(const OccWord* __restrict__ occ, uint32_t log2w, uint32_t up,
+                                               uint32_t sd3, bool& set)
 {
-    const OccWord w = occ[up >> 5];
-    const uint32_t r = up & 31u;
-    set = __builtin_amdgcn_ubfe(w.bits, r, 1u) != 0u;
-    return __umul24((uint32_t)__popc(__builtin_amdgcn_ubfe(w.bits, 0u, r)) + w.prefix, sd3);
+    const OccWord w = occ[__builtin_amdgcn_ubfe(up, 5u, log2w)];
+    set = __builtin_amdgcn_ubfe(w.bits, up, 1u) != 0u;
+    return __umul24((uint32_t)__popc(__builtin_amdgcn_ubfe(w.bits, 0u, up)) + w.prefix, sd3);
 }
 
-__device__ __forceinline__ int32_t rank_offset(const OccWord* __restrict__ occ, uint32_t up, uint32_t sd3)
+__device__ __forceinline__ int32_t rank_offset(const OccWord* __restrict__ occ, uint32_t log2w, uint32_t up,
+                                               uint32_t sd3)
 {
     bool set;
-    const uint32_t off = rank_block(occ, up, sd3, set);
+    const uint32_t off = rank_block(occ, log2w, up, sd3, set);
     return set ? (int32_t)off : -1;
 }
 
 template <bool kRank>
 __device__ __forceinline__ int32_t block_offset(const DevMap& m, const OccWord* __restrict__ occ, uint32_t up)
 {
-    if (kRank) return rank_offset(occ, up, m.sd3);
+    if (kRank) return rank_offset(occ, m.occ_log2, up, m.sd3);
     return m.grid_occ[up];
 }
 
@@ -171,15 +175,16 @@ __device__ __forceinline__ void fast_lookup(const DevMap& m, const AxisEntry* __
     carry = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz)) >= m.carry_bits;
     const bool grid = (ix < m.gthr_x) & (iy < m.gthr_y) & (iz < m.gthr_z);
     const uint32_t up = ix + mul_u<kU24>(iy, m.up_dim_x) + mul_u<kU24>(iz, m.up_dim_2);
-    const bool inside = grid & (up < m.occ);
     if (kRank) {
-        // branch-free: rejected lanes read word 0 and discard it
+        // branch-free: lanes outside the grid read some word and discard
+        // it; inside the grid every up lies in the table (padded past
+        // grid_occ_size with unallocated blocks)
         bool set;
-        s = rank_block(occ, inside ? up : 0u, m.sd3, set);
-        valid = inside & set;
+        s = rank_block(occ, m.occ_log2, up, m.sd3, set);
+        valid = grid & set;
     } else {
         int32_t g = -1;
-        if (inside) g = m.grid_occ[up];
+        if (grid & (up < m.occ)) g = m.grid_occ[up];
         s = (uint32_t)g;
         valid = g >= 0;
     }
