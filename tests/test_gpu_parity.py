@@ -91,7 +91,8 @@ def test_room_evaluate_normalised(tsl, room, room_oracle, params):
 
 def test_negative_band_and_outside(tsl):
     """Particles around and outside the map's low faces: offsets below zero
-    take the x86-semantics path; indices must still match bit for bit."""
+    take the x86-semantics path; indices and values must match bit for
+    bit."""
     from tsdf_localization_amd import synth
     sc = synth.Scene()
     sc.add_box((-5, -5, -5), (20, 20, 0.3))
@@ -118,6 +119,8 @@ def test_negative_band_and_outside(tsl):
     assert np.count_nonzero(ref != ds) > 1000  # the case exercises real lookups
     mism = np.count_nonzero(gpu != ref)
     assert mism == 0, f"{mism} mismatches"
+    got = ev.trace_values(parts, pts, tf)
+    assert np.array_equal(got.view(np.uint32), oracle_values(om, ref).view(np.uint32))
     raw = ev.evaluate_raw(parts, pts, tf)
     _, w64 = om.evaluate(parts, pts, tf)
     assert np.max(rel_err(raw, w64)) <= WEIGHT_RTOL
