@@ -236,6 +236,7 @@ def bench_serving(args):
         f"(generated in {time.time() - t0:.1f}s)")
     ev = tsl.TSDFEvaluator(w.map)
     rng = np.random.default_rng(4242)
+    rng_h = np.random.default_rng(4343)
     n_req = args.requests
     rate = args.rate
     robots_of = rng.integers(0, R, n_req)
@@ -271,6 +272,26 @@ def bench_serving(args):
     p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
     log(f"poisson {rate}/s: {n_req} requests in {wall:.2f} s, latency p50 {p50:.2f} ms p99 {p99:.2f} ms, "
         f"mean batch {batch.mean():.1f}, failed {bad}")
+
+    # 1b. the same requests under heavier Poisson loads (latency headroom)
+    by_rate = {str(rate): {"p50": p50, "p99": p99, "mean_batch": float(batch.mean())}}
+    for hr in args.extra_rates:
+        if hr <= 0:
+            continue
+        arr = np.cumsum(rng_h.exponential(1.0 / hr, n_req))
+        t_h = time.perf_counter()
+        for i in range(n_req):
+            delay = arr[i] - (time.perf_counter() - t_h)
+            if delay > 0:
+                time.sleep(delay)
+            srv.submit(ids[robots_of[i]], motions[i], w.robots[robots_of[i]].points, u0[i], request_id=i)
+        rh = srv.wait(n_req, timeout_ms=600_000)
+        lh = np.array([r["done_ms"] - r["submit_ms"] for r in rh])
+        bh = np.array([r["batch"] for r in rh])
+        by_rate[str(hr)] = {"p50": float(np.percentile(lh, 50)), "p99": float(np.percentile(lh, 99)),
+                            "mean_batch": float(bh.mean())}
+        log(f"poisson {hr}/s: latency p50 {by_rate[str(hr)]['p50']:.2f} ms p99 {by_rate[str(hr)]['p99']:.2f} ms, "
+            f"mean batch {bh.mean():.1f}")
     srv.close()
 
     # 2. saturation: every request at once
@@ -321,6 +342,7 @@ def bench_serving(args):
                    "max_batch": args.max_batch, "chunk_points": args.chunk or 2048,
                    "map": "sparse 1024x1024x128 @0.05 m int16"},
         "latency_ms": {"p50": p50, "p99": p99, "mean_batch": float(batch.mean()), "failed": int(bad)},
+        "latency_ms_by_rate": by_rate,
         "plain_value": plain_rate,
         "speedup_vs_plain": served_rate / plain_rate,
     }
@@ -335,6 +357,8 @@ def main():
     ap.add_argument("--workload", default="building", choices=["building", "room", "batched", "serving"])
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--extra-rates", type=float, nargs="*", default=[1000.0, 2000.0],
+                    help="serving: further Poisson rates to report latency at")
     ap.add_argument("--max-batch", type=int, default=64, help="serving: requests per tick")
     ap.add_argument("--hbm-budget-gb", type=int, default=64, help="serving: device bytes the server may hold")
     ap.add_argument("--particles", type=int, default=None)
