@@ -511,10 +511,14 @@ __global__ __launch_bounds__(kBlock) void k_normalize_m(const Member* __restrict
         __threadfence();
         double t[9];
 #pragma unroll
-        for (int qq = 0; qq < 9; ++qq) {
-            t[qq] = 0.0;
-            for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x)
-                t[qq] += *(volatile const double*)&m.blocks[(size_t)qq * gridDim.x + k];
+        for (int qq = 0; qq < 9; ++qq) t[qq] = 0.0;
+        for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
+            // the nine loads of one block are independent: issue them together
+            double v[9];
+#pragma unroll
+            for (int qq = 0; qq < 9; ++qq) v[qq] = *(volatile const double*)&m.blocks[(size_t)qq * gridDim.x + k];
+#pragma unroll
+            for (int qq = 0; qq < 9; ++qq) t[qq] += v[qq];
         }
         const double total = block_sum9(t, sh);
         if (threadIdx.x < 9) st->stats[1 + threadIdx.x] = total;

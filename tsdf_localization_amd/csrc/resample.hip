@@ -10,8 +10,9 @@
 // Equivalently output j is particle m(j) = min{m : s_m > U_j} and the number
 // emitted is J = #{j : U_j < s_{M-1}}.  Both sequences are sequential
 // recurrences with rounding, so the kernels
-//   1. compute a candidate for each in parallel (a block scan of the
-//      weights in double; U_j from a per-binade table of constant float
+//   1. compute a candidate for each in parallel (a scan of the weights in
+//      double, split over workgroups that start from the sum of the earlier
+//      workgroups' totals; U_j from a per-binade table of constant float
 //      steps),
 //   2. check every element against one step of the recurrence from its
 //      predecessor (s_m == s_{m-1} + w_m, U_j == float(U_{j-1} + 1/M)); if all
@@ -96,6 +97,7 @@ __global__ void k_rs_plan(Member* __restrict__ members, float u0_arg)
     const uint32_t M = st->count;
     const float u0 = m.u0_dev ? *m.u0_dev : u0_arg;
     st->first_bad = 0xffffffffu;
+    p->s_bad = 0xffffffffu;
     p->M = M;
     p->u0 = u0;
     p->cap = 0;
@@ -131,36 +133,84 @@ __device__ __forceinline__ double wave_incl_scan(double v, int lane)
     return v;
 }
 
-// Phase A (one workgroup per member): s_m, the double running sum,
-// candidate by block scan, checked element by element, repaired serially
-// from the first mismatch.
-__global__ __launch_bounds__(kRsThreads) void k_rs_scan(const Member* __restrict__ members)
+// Span of scan workgroup b of B for M particles: whole chunks, the last
+// workgroups possibly empty.
+constexpr int kRsItems = 4;
+constexpr uint32_t kRsChunk = kRsThreads * kRsItems;
+
+__device__ __forceinline__ uint32_t rs_span(uint32_t M, uint32_t B)
 {
-    const Member& mb = members[blockIdx.x];
-    const RsPlan* p = mb.plan;
+    const uint32_t chunks = (M + kRsChunk - 1) / kRsChunk;
+    return (chunks + B - 1) / B * kRsChunk;
+}
+
+__device__ __forceinline__ double block_sum_rs(double v, double* sh)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kRsThreads / 64; ++w) t += sh[w];
+    __syncthreads();
+    return t;  // valid in thread 0
+}
+
+// Phase A1 (grid x scan workgroups, y members; launched when a member's
+// scan spans several workgroups): the weight total of each workgroup's
+// span, in any order (it only seeds the candidate offsets of phase A2).
+__global__ __launch_bounds__(kRsThreads) void k_rs_blk(const Member* __restrict__ members)
+{
+    const Member& mb = members[blockIdx.y];
+    RsPlan* p = mb.plan;
     if (p->cap == 0) return;
     const uint32_t M = p->M;
+    const uint32_t span = rs_span(M, gridDim.x);
+    const uint32_t b0 = blockIdx.x * span, b1 = min(M, b0 + span);
+    __shared__ double sh[kRsThreads / 64];
+    double acc = 0.0;
+    for (uint32_t m = b0 + threadIdx.x; m < b1; m += kRsThreads) acc += (double)mb.particles[m].second;
+    const double t = block_sum_rs(acc, sh);
+    if (threadIdx.x == 0) p->blk[blockIdx.x] = t;
+}
+
+// Phase A2 (grid x scan workgroups, y members): s_m, the double running
+// sum, as a candidate: workgroup b scans its span from the sum of the
+// earlier spans' totals, then checks s_m == s_{m-1} + w_m for every
+// element of its span but the first (whose predecessor another workgroup
+// writes; k_rs_count checks those and repairs serially from the first
+// mismatch).
+__global__ __launch_bounds__(kRsThreads) void k_rs_scan(const Member* __restrict__ members)
+{
+    const Member& mb = members[blockIdx.y];
+    RsPlan* p = mb.plan;
+    if (p->cap == 0) return;
+    const uint32_t M = p->M;
+    const uint32_t span = rs_span(M, gridDim.x);
+    const uint32_t lo = blockIdx.x * span;
+    if (lo >= M) return;
+    const uint32_t hi = min(M, lo + span);
     const tsl_particle* __restrict__ in = mb.particles;
     double* __restrict__ s = mb.cumsum;
     __shared__ double wave_tot[kRsThreads / 64];
     __shared__ double carry_sh;
-    __shared__ uint32_t first_bad;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    constexpr int kItems = 4;
-    constexpr uint32_t kChunk = kRsThreads * kItems;
     if (tid == 0) {
-        carry_sh = 0.0;
-        first_bad = 0xffffffffu;
+        double off = 0.0;
+        for (uint32_t b = 0; b < blockIdx.x; ++b) off += p->blk[b];
+        carry_sh = off;
     }
     __syncthreads();
-    for (uint32_t base = 0; base < M; base += kChunk) {
-        double loc[kItems];
+    for (uint32_t base = lo; base < hi; base += kRsChunk) {
+        double loc[kRsItems];
         double run = 0.0;
 #pragma unroll
-        for (int e = 0; e < kItems; ++e) {
-            const uint32_t m = base + tid * kItems + e;
-            const double w = m < M ? (double)in[m].second : 0.0;
+        for (int e = 0; e < kRsItems; ++e) {
+            const uint32_t m = base + tid * kRsItems + e;
+            const double w = m < hi ? (double)in[m].second : 0.0;
             run += w;
             loc[e] = run;
         }
@@ -172,9 +222,9 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scan(const Member* __restrict
         const double carry = carry_sh;
         const double excl = carry + (woff + (incl - run));
 #pragma unroll
-        for (int e = 0; e < kItems; ++e) {
-            const uint32_t m = base + tid * kItems + e;
-            if (m < M) s[m] = excl + loc[e];
+        for (int e = 0; e < kRsItems; ++e) {
+            const uint32_t m = base + tid * kRsItems + e;
+            if (m < hi) s[m] = excl + loc[e];
         }
         __syncthreads();
         if (tid == kRsThreads - 1) {
@@ -184,18 +234,11 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scan(const Member* __restrict
         }
         __syncthreads();
     }
-    // check s_m == s_{m-1} + w_m
-    for (uint32_t m = tid; m < M; m += kRsThreads) {
+    // check s_m == s_{m-1} + w_m inside the span (element 0: s_0 == 0 + w_0)
+    for (uint32_t m = lo + tid; m < hi; m += kRsThreads) {
+        if (m == lo && m != 0) continue;
         const double prev = m == 0 ? 0.0 : s[m - 1];
-        if (s[m] != prev + (double)in[m].second) atomicMin(&first_bad, m);
-    }
-    __syncthreads();
-    if (first_bad != 0xffffffffu && tid == 0) {
-        double acc = first_bad == 0 ? 0.0 : s[first_bad - 1];
-        for (uint32_t m = first_bad; m < M; ++m) {
-            acc += (double)in[m].second;
-            s[m] = acc;
-        }
+        if (s[m] != prev + (double)in[m].second) atomicMin(&p->s_bad, m);
     }
 }
 
@@ -225,10 +268,10 @@ __global__ void k_rs_u(const Member* __restrict__ members)
     if (u != expect) atomicMin(&mb.st->first_bad, j);
 }
 
-// Phase C (one thread per member): serial repair of U from the first
-// mismatch, then the number emitted J = #{j : U_j < s_{M-1}} (U is
-// non-decreasing).
-__global__ void k_rs_count(const Member* __restrict__ members)
+// Phase C (one thread per member): the running-sum checks left to it and
+// serial repairs of s and U from their first mismatches, then the number
+// emitted J = #{j : U_j < s_{M-1}} (U is non-decreasing).
+__global__ void k_rs_count(const Member* __restrict__ members, uint32_t scan_blocks)
 {
     if (threadIdx.x != 0) return;
     const Member& mb = members[blockIdx.x];
@@ -236,6 +279,24 @@ __global__ void k_rs_count(const Member* __restrict__ members)
     const uint32_t cap = p->cap;
     if (cap == 0) return;
     FilterState* st = mb.st;
+    {
+        // running sum: the first element of each scan workgroup's span, then
+        // a serial repair from the first mismatch
+        const uint32_t M = p->M;
+        const tsl_particle* __restrict__ in = mb.particles;
+        double* __restrict__ sum = mb.cumsum;
+        const uint32_t span = rs_span(M, scan_blocks);
+        uint32_t bad = p->s_bad;
+        for (uint32_t m = span; m < M && m < bad; m += span)
+            if (sum[m] != sum[m - 1] + (double)in[m].second) bad = m;
+        if (bad < M) {
+            double acc = bad == 0 ? 0.0 : sum[bad - 1];
+            for (uint32_t m = bad; m < M; ++m) {
+                acc += (double)in[m].second;
+                sum[m] = acc;
+            }
+        }
+    }
     float* __restrict__ U = mb.ubuf;
     const double inv_m = p->inv_m;
     const uint32_t fb = st->first_bad;
@@ -315,11 +376,19 @@ int launch_resample_members(Member* d_members, uint32_t n, uint32_t grid_cap, fl
     const unsigned gx = (grid_cap + 255) / 256 ? (grid_cap + 255) / 256 : 1;
     hipLaunchKernelGGL(k_rs_plan, dim3(n), dim3(64), 0, stream, d_members, u0_arg);
     TSL_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_rs_scan, dim3(n), dim3(kRsThreads), 0, stream, d_members);
+    // running-sum workgroups per member: one per 4096-particle chunk (the
+    // largest member's), at most kMaxRsBlocks
+    const uint32_t chunks = (grid_cap + kRsChunk - 1) / kRsChunk;
+    const uint32_t nblk = chunks < 1 ? 1u : (chunks > (uint32_t)kMaxRsBlocks ? (uint32_t)kMaxRsBlocks : chunks);
+    if (nblk > 1) {
+        hipLaunchKernelGGL(k_rs_blk, dim3(nblk, n), dim3(kRsThreads), 0, stream, d_members);
+        TSL_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_rs_scan, dim3(nblk, n), dim3(kRsThreads), 0, stream, d_members);
     TSL_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_rs_u, dim3(gx, n), dim3(256), 0, stream, d_members);
     TSL_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_rs_count, dim3(n), dim3(64), 0, stream, d_members);
+    hipLaunchKernelGGL(k_rs_count, dim3(n), dim3(64), 0, stream, d_members, nblk);
     TSL_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_rs_assign, dim3(gx, n), dim3(256), 0, stream, d_members);
     TSL_HIP_CHECK(hipGetLastError());

@@ -19,6 +19,10 @@ namespace tsl {
 
 namespace {
 
+// particle-order buckets: kBucketBits bits per axis (x, y, yaw)
+constexpr uint32_t kBucketBits = 5;
+constexpr uint32_t kBuckets = 1u << (3 * kBucketBits);
+
 __device__ __forceinline__ uint64_t spread3(uint64_t v)
 {
     v &= 0x1fffffull;
@@ -61,32 +65,93 @@ __global__ void k_prep_sorted(const tsl_point* __restrict__ in, const uint32_t* 
     out[j] = make_float4(p.x, p.y, p.z, d2 < mr2 ? range_term : a_max);
 }
 
-// Evaluation order of the particles: Morton code of (x, y) in 0.5 m cells
-// and yaw in 0.1 rad cells.  Particles close in position and heading
-// transform the scan onto nearly the same voxels, so evaluating them
-// together on one XCD turns most of their voxel gathers into L2 hits.
-// Slots past the device count get the largest key and sort last.
-__device__ __forceinline__ uint64_t quant(float v, float inv_cell)
+// Evaluation order of the particles: a counting sort into Morton-ordered
+// buckets of (x, y) cells over the map and yaw sectors.  Particles close in
+// position and heading transform the scan onto nearly the same voxels, so
+// evaluating them together on one XCD turns most of their voxel gathers
+// into L2 hits; the thousands of particles an XCD has in flight at once
+// span many buckets, so the order inside a bucket (arrival order of the
+// scatter's atomics) does not matter, and no particle's weight depends on
+// its slot.
+struct BucketGrid {
+    float min_x, min_y, inv_x, inv_y, inv_yaw;
+};
+
+__device__ __forceinline__ uint32_t bucket_coord(float v, float inv_cell, uint32_t cells)
 {
-    float c = floorf(v * inv_cell) + 1048576.0f;
+    float c = floorf(v * inv_cell);
     c = c > 0.0f ? c : 0.0f;  // NaN -> 0
-    c = c < 2097151.0f ? c : 2097151.0f;
-    return (uint64_t)c;
+    const float top = (float)(cells - 1);
+    c = c < top ? c : top;
+    return (uint32_t)c;
 }
 
-__global__ void k_particle_keys(const tsl_particle* __restrict__ in, const uint32_t* __restrict__ count,
-                                uint32_t n_bound, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+// bits 3k, 3k+1, 3k+2 of the bucket = bit k of x, y, yaw
+__device__ __forceinline__ uint32_t bucket_of(const tsl_particle& p, const BucketGrid& g)
+{
+    const uint32_t x = bucket_coord(p.first[0] - g.min_x, g.inv_x, 1u << kBucketBits);
+    const uint32_t y = bucket_coord(p.first[1] - g.min_y, g.inv_y, 1u << kBucketBits);
+    const uint32_t a = bucket_coord(p.first[5] + 3.14159265f, g.inv_yaw, 1u << kBucketBits);
+    uint32_t key = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kBucketBits; ++k)
+        key |= (((x >> k) & 1u) << (3 * k)) | (((y >> k) & 1u) << (3 * k + 1)) | (((a >> k) & 1u) << (3 * k + 2));
+    return key;
+}
+
+__global__ void k_bucket_zero(uint32_t* __restrict__ hist)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_bound) return;
-    uint64_t key = ~0ull;
-    if (i < *count) {
-        const tsl_particle p = in[i];
-        key = spread3(quant(p.first[0], 2.0f)) | (spread3(quant(p.first[1], 2.0f)) << 1) |
-              (spread3(quant(p.first[5], 10.0f)) << 2);
+    if (i < kBuckets) hist[i] = 0u;
+}
+
+__global__ void k_bucket_count(const tsl_particle* __restrict__ in, const uint32_t* __restrict__ count,
+                               BucketGrid g, uint32_t* __restrict__ hist, uint32_t* __restrict__ bucket)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    const uint32_t k = bucket_of(in[i], g);
+    bucket[i] = k;
+    atomicAdd(&hist[k], 1u);
+}
+
+// One workgroup: hist becomes the exclusive prefix sum of the counts.
+constexpr int kBucketThreads = 1024;
+constexpr uint32_t kPerThread = kBuckets / kBucketThreads;
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_scan(uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t wsum[kBucketThreads / 64];
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    uint32_t v[kPerThread];
+    uint32_t run = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < kPerThread; ++e) {
+        v[e] = hist[t * kPerThread + e];
+        run += v[e];
     }
-    keys[i] = key;
-    vals[i] = i;
+    uint32_t incl = run;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off, 64);
+        if ((int)lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t base = incl - run;
+    for (uint32_t w = 0; w < wave; ++w) base += wsum[w];
+#pragma unroll
+    for (uint32_t e = 0; e < kPerThread; ++e) {
+        hist[t * kPerThread + e] = base;
+        base += v[e];
+    }
+}
+
+__global__ void k_bucket_scatter(const uint32_t* __restrict__ count, const uint32_t* __restrict__ bucket,
+                                 uint32_t* __restrict__ hist, uint32_t* __restrict__ order)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    order[atomicAdd(&hist[bucket[i]], 1u)] = i;
 }
 
 // ---- scan reduction (TSDFEvaluator::evaluateParticles,
@@ -265,20 +330,32 @@ int launch_particle_order(tsl_filter* f, hipStream_t stream)
 {
     const uint32_t n = (uint32_t)f->count_bound;
     if (n == 0) return TSL_OK;
-    TSL_TRY(f->pkeys.reserve(sizeof(uint64_t) * 2 * n));
-    TSL_TRY(f->pvals.reserve(sizeof(uint32_t) * n));
+    TSL_TRY(f->pkeys.reserve(sizeof(uint32_t) * ((size_t)kBuckets + n)));
     TSL_TRY(f->porder.reserve(sizeof(uint32_t) * n));
     TSL_TRY(f->pinv.reserve(sizeof(uint32_t) * n));
-    uint64_t* keys = f->pkeys.as<uint64_t>();
-    size_t temp = 0;
-    TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, keys, keys + n, f->pvals.as<uint32_t>(),
-                                                     f->porder.as<uint32_t>(), (int)n, 0, 64, stream));
-    TSL_TRY(f->ptemp.reserve(temp));
-    hipLaunchKernelGGL(k_particle_keys, dim3((n + 255) / 256), dim3(256), 0, stream, f->particles.as<tsl_particle>(),
-                       &f->state.as<FilterState>()->count, n, keys, f->pvals.as<uint32_t>());
+    uint32_t* hist = f->pkeys.as<uint32_t>();
+    uint32_t* bucket = hist + kBuckets;
+    const uint32_t* count = &f->state.as<FilterState>()->count;
+    // x and y cells spanning the map's extent (at least 0.5 m), yaw sectors
+    // spanning [-pi, pi)
+    const tsl_map_coef& c = f->ev->map->coef;
+    const float cells = (float)(1u << kBucketBits);
+    BucketGrid g;
+    g.min_x = c.min_x;
+    g.min_y = c.min_y;
+    g.inv_x = 1.0f / std::max(0.5f, (c.max_x - c.min_x) / cells);
+    g.inv_y = 1.0f / std::max(0.5f, (c.max_y - c.min_y) / cells);
+    g.inv_yaw = cells / 6.2831853f;
+    hipLaunchKernelGGL(k_bucket_zero, dim3(kBuckets / 256), dim3(256), 0, stream, hist);
     TSL_HIP_CHECK(hipGetLastError());
-    TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(f->ptemp.ptr, temp, keys, keys + n, f->pvals.as<uint32_t>(),
-                                                     f->porder.as<uint32_t>(), (int)n, 0, 64, stream));
+    hipLaunchKernelGGL(k_bucket_count, dim3((n + 255) / 256), dim3(256), 0, stream, f->particles.as<tsl_particle>(),
+                       count, g, hist, bucket);
+    TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(kBucketThreads), 0, stream, hist);
+    TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_bucket_scatter, dim3((n + 255) / 256), dim3(256), 0, stream, count, bucket, hist,
+                       f->porder.as<uint32_t>());
+    TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
 }
 

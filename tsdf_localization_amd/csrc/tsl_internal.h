@@ -194,13 +194,17 @@ struct USeg {
     float u;       // U_{j0}
     double step;   // constant difference inside the run (exact)
 };
+constexpr int kMaxRsBlocks = 256;  // workgroups of one member's running-sum scan
 struct RsPlan {
     uint32_t M;     // particles resampled
     uint32_t cap;   // candidates U_0 .. U_{cap-1}; 0 = skip this resampling
     uint32_t nseg;
     float u0;
     double inv_m;
+    uint32_t s_bad;  // first running-sum element that failed its check
+    uint32_t pad;
     USeg seg[kMaxSeg];
+    double blk[kMaxRsBlocks];  // weight total of each scan workgroup's span
 };
 
 // Device descriptor of one filter for the member-indexed kernels (a single
@@ -304,7 +308,7 @@ struct tsl_filter {
     tsl::DevBuf points_in;   // tsl_point staging (caller order)
     tsl::DevBuf perm;        // uint32[num_points]: caller index of sorted point j
     tsl::DevBuf sort_keys, sort_vals, sort_temp;
-    tsl::DevBuf pkeys, pvals, ptemp;  // particle-order sort
+    tsl::DevBuf pkeys;       // particle-order buckets: counts/offsets, then bucket of particle i
     tsl::DevBuf red_scratch;          // scan reduction
     tsl::DevBuf raw_in, raw_ring;     // raw scan staging
     tsl::DevBuf porder;      // uint32: particle evaluated in slot s
