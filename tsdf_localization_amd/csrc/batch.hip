@@ -77,7 +77,7 @@ int prepare(tsl_batch* b)
         tsl_filter* f = b->filters[i];
         TSL_TRY(f->ubuf.reserve(sizeof(float) * f->capacity));
         TSL_TRY(f->plan.reserve(sizeof(RsPlan)));
-        TSL_TRY(f->blocks.reserve(sizeof(double) * 9 * nb));
+        TSL_TRY(f->blocks.reserve(sizeof(double) * kReducePartials * nb));
         const int saved_k = f->tile_k, saved_p = f->tile_p;
         const uint32_t saved_chunk = f->tile_chunk;
         f->tile_k = b->K;

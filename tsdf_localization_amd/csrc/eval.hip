@@ -424,15 +424,20 @@ __global__ void k_pose_m(const Member* __restrict__ members)
     for (int k = 0; k < 12; ++k) m.L[(size_t)i * 12 + k] = out[k];
 }
 
+// The reductions below avoid a "last workgroup finishes" step: on this GPU
+// the device-scope fence it needs writes back and invalidates the L2 of
+// the XCD, once per wavefront.  Instead every workgroup that needs a total
+// re-forms it from the workgroup partials, in one fixed order, and a small
+// final kernel forms the pose sums.  Partials of a member: blocks[0, nb) the
+// weight sums, blocks[nb + q * nb + b] the nine pose sums of workgroup b.
+
 // raw[p] = sum over chunks (ascending) of partial[c][p], in double, stored as
-// float; block partial sums of raw in double; the last block to finish adds
-// the block partials in a fixed order into stats[0].
+// float; workgroup partial sums of raw in double.
 __global__ __launch_bounds__(kBlock) void k_reduce_m(const Member* __restrict__ members)
 {
     const Member& m = members[blockIdx.y];
     FilterState* st = m.st;
     __shared__ double sh[kWaves];
-    __shared__ bool last;
     const uint32_t P = st->count;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double w = 0.0;
@@ -445,37 +450,43 @@ __global__ __launch_bounds__(kBlock) void k_reduce_m(const Member* __restrict__ 
         w = (double)wf;
     }
     const double b = block_sum_d(w, sh);
-    if (threadIdx.x == 0) {
-        m.blocks[blockIdx.x] = b;
-        __threadfence();
-        const uint32_t done = atomicAdd(&st->counter0, 1u);
-        last = (done == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        double t = 0.0;
-        for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) t += *(volatile const double*)&m.blocks[k];
-        const double total = block_sum_d(t, sh);
-        if (threadIdx.x == 0) {
-            st->stats[0] = total;
-            st->status = ((float)total == 0.0f) ? kStNoValid : kStOk;
-            st->counter0 = 0;
-        }
-    }
+    if (threadIdx.x == 0) m.blocks[blockIdx.x] = b;
 }
 
-// w_i = raw_i / (float)sum written to particles[i].second (the reference's
-// particles[index].second /= weight_sum, float division), plus the nine
+// The total of nb workgroup partials (thread k adds partials k, k + kBlock,
+// ... in order, then block_sum_d): the same value in every workgroup.
+__device__ __forceinline__ double total_of(const double* __restrict__ part, uint32_t nb, double* sh)
+{
+    double t = 0.0;
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) t += part[k];
+    return block_sum_d(t, sh);  // thread 0
+}
+
+// sum = total of the raw weights (stats[0]; status "no valid particle" when
+// it is 0 as a float); w_i = raw_i / (float)sum written to
+// particles[i].second (the reference's particles[index].second /=
+// weight_sum, float division), plus workgroup partials of the nine
 // weighted pose sums (tsdf_evaluator.cpp:198-217) in double.
 __global__ __launch_bounds__(kBlock) void k_normalize_m(const Member* __restrict__ members)
 {
     const Member& m = members[blockIdx.y];
     FilterState* st = m.st;
     __shared__ double sh[kWaves * 9];
-    __shared__ bool last;
+    __shared__ double total_sh;
+    const uint32_t nb = gridDim.x;
     const uint32_t P = st->count;
-    const float sum = (float)st->stats[0];
+    {
+        const double total = total_of(m.blocks, nb, sh);
+        if (threadIdx.x == 0) {
+            total_sh = total;
+            if (blockIdx.x == 0) {
+                st->stats[0] = total;
+                st->status = ((float)total == 0.0f) ? kStNoValid : kStOk;
+            }
+        }
+        __syncthreads();
+    }
+    const float sum = (float)total_sh;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double q[9];
 #pragma unroll
@@ -496,34 +507,29 @@ __global__ __launch_bounds__(kBlock) void k_normalize_m(const Member* __restrict
             q[4 + 2 * a] = (double)c * wd;
         }
     }
-    {
-        const double b = block_sum9(q, sh);
-        if (threadIdx.x < 9) m.blocks[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = b;
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t done = atomicAdd(&st->counter1, 1u);
-        last = (done == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        double t[9];
+    const double b = block_sum9(q, sh);
+    if (threadIdx.x < 9) m.blocks[nb + (size_t)threadIdx.x * nb + blockIdx.x] = b;
+}
+
+// One workgroup per member: the nine pose sums from nb workgroup partials
+// into stats[1..9].
+__global__ __launch_bounds__(kBlock) void k_pose_sums_m(const Member* __restrict__ members, uint32_t nb)
+{
+    const Member& m = members[blockIdx.y];
+    __shared__ double sh[kWaves * 9];
+    double t[9];
 #pragma unroll
-        for (int qq = 0; qq < 9; ++qq) t[qq] = 0.0;
-        for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
-            // the nine loads of one block are independent: issue them together
-            double v[9];
+    for (int qq = 0; qq < 9; ++qq) t[qq] = 0.0;
+    const double* part = m.blocks + nb;
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+        double v[9];
 #pragma unroll
-            for (int qq = 0; qq < 9; ++qq) v[qq] = *(volatile const double*)&m.blocks[(size_t)qq * gridDim.x + k];
+        for (int qq = 0; qq < 9; ++qq) v[qq] = part[(size_t)qq * nb + k];
 #pragma unroll
-            for (int qq = 0; qq < 9; ++qq) t[qq] += v[qq];
-        }
-        const double total = block_sum9(t, sh);
-        if (threadIdx.x < 9) st->stats[1 + threadIdx.x] = total;
-        if (threadIdx.x == 0) st->counter1 = 0;
+        for (int qq = 0; qq < 9; ++qq) t[qq] += v[qq];
     }
+    const double total = block_sum9(t, sh);
+    if (threadIdx.x < 9) m.st->stats[1 + threadIdx.x] = total;
 }
 
 // Restore every member from its snapshot.
@@ -931,12 +937,14 @@ int launch_update(tsl_filter* f, hipStream_t stream)
     Tile tile;
     TSL_TRY(launch_eval_core(f, stream, tile));
     const unsigned nb = (unsigned)((P + kBlock - 1) / kBlock);
-    TSL_TRY(f->blocks.reserve(sizeof(double) * 9 * (size_t)nb));
+    TSL_TRY(f->blocks.reserve(sizeof(double) * kReducePartials * (size_t)nb));
     Member* dm;
     TSL_TRY(filter_member(f, &dm, tile.chunk_len, tile.chunks));
     hipLaunchKernelGGL(k_reduce_m, dim3(nb), dim3(kBlock), 0, stream, dm);
     TSL_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_normalize_m, dim3(nb), dim3(kBlock), 0, stream, dm);
+    TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_pose_sums_m, dim3(1), dim3(kBlock), 0, stream, dm, nb);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
 }
@@ -948,7 +956,7 @@ int launch_raw(tsl_filter* f, hipStream_t stream)
     Tile tile;
     TSL_TRY(launch_eval_core(f, stream, tile));
     const unsigned nb = (unsigned)((P + kBlock - 1) / kBlock);
-    TSL_TRY(f->blocks.reserve(sizeof(double) * 9 * (size_t)nb));
+    TSL_TRY(f->blocks.reserve(sizeof(double) * kReducePartials * (size_t)nb));
     Member* dm;
     TSL_TRY(filter_member(f, &dm, tile.chunk_len, tile.chunks));
     hipLaunchKernelGGL(k_reduce_m, dim3(nb), dim3(kBlock), 0, stream, dm);
@@ -1075,6 +1083,8 @@ int launch_batch_update(const BatchLaunch& b, hipStream_t stream)
     hipLaunchKernelGGL(k_reduce_m, dim3(gx, b.n), dim3(kBlock), 0, stream, b.members);
     TSL_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_normalize_m, dim3(gx, b.n), dim3(kBlock), 0, stream, b.members);
+    TSL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_pose_sums_m, dim3(1, b.n), dim3(kBlock), 0, stream, b.members, gx);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
 }

@@ -273,22 +273,29 @@ __global__ void k_rs_u(const Member* __restrict__ members)
 // emitted J = #{j : U_j < s_{M-1}} (U is non-decreasing).
 __global__ void k_rs_count(const Member* __restrict__ members, uint32_t scan_blocks)
 {
-    if (threadIdx.x != 0) return;
     const Member& mb = members[blockIdx.x];
     RsPlan* p = mb.plan;
     const uint32_t cap = p->cap;
     if (cap == 0) return;
     FilterState* st = mb.st;
+    const uint32_t M = p->M;
+    const tsl_particle* __restrict__ in = mb.particles;
+    double* __restrict__ sum = mb.cumsum;
+    // running sum: the first element of each scan workgroup's span (one
+    // wavefront, a span per lane), then a serial repair from the first
+    // mismatch
+    uint32_t bad = p->s_bad;
     {
-        // running sum: the first element of each scan workgroup's span, then
-        // a serial repair from the first mismatch
-        const uint32_t M = p->M;
-        const tsl_particle* __restrict__ in = mb.particles;
-        double* __restrict__ sum = mb.cumsum;
         const uint32_t span = rs_span(M, scan_blocks);
-        uint32_t bad = p->s_bad;
-        for (uint32_t m = span; m < M && m < bad; m += span)
-            if (sum[m] != sum[m - 1] + (double)in[m].second) bad = m;
+        for (uint32_t b = 1 + threadIdx.x; b < scan_blocks; b += blockDim.x) {
+            const uint32_t m = b * span;
+            if (m < M && sum[m] != sum[m - 1] + (double)in[m].second) bad = min(bad, m);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) bad = min(bad, (uint32_t)__shfl_xor((int)bad, off, 64));
+    }
+    if (threadIdx.x != 0) return;
+    {
         if (bad < M) {
             double acc = bad == 0 ? 0.0 : sum[bad - 1];
             for (uint32_t m = bad; m < M; ++m) {

@@ -490,7 +490,7 @@ int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n,
     if (rc == TSL_OK) rc = f->plan.reserve(sizeof(RsPlan));
     // the reduction grid of a tick spans its largest member
     const uint64_t nb_max = (resample_bound(s->max_particles) + 255) / 256;
-    if (rc == TSL_OK) rc = f->blocks.reserve(sizeof(double) * 9 * nb_max);
+    if (rc == TSL_OK) rc = f->blocks.reserve(sizeof(double) * kReducePartials * nb_max);
     if (rc == TSL_OK) std::memcpy(f->tf, tf, sizeof(float) * 16);
     if (rc != TSL_OK) {
         tsl_filter_destroy(f);

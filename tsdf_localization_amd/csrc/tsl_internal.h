@@ -178,11 +178,9 @@ constexpr uint32_t kStOk = 0, kStNoValid = 1, kStCapacity = 2, kStEmpty = 3, kSt
 struct FilterState {
     uint32_t count;        // particles in the resident set
     uint32_t status;       // kSt*
-    uint32_t counter0;     // last-block-done counters, reset by the last block
-    uint32_t counter1;
     uint32_t resample_count;
     uint32_t first_bad;    // resampling: first U candidate that failed its check
-    uint32_t pad[2];
+    uint32_t pad[4];
     double stats[10];      // sum, Σwx, Σwy, Σwz, Σw sin/cos roll, pitch, yaw
 };
 
@@ -233,6 +231,10 @@ struct Member {
 };
 
 constexpr uint32_t kMaxMembers = 1024;  // filters in one batch
+
+// Doubles of reduction partials per evaluation workgroup of a member: the
+// weight sum and the nine pose sums.
+constexpr size_t kReducePartials = 10;
 
 // Work plan of a batched evaluation, built on the device each step from the
 // members' device counts: member f owns items [prefix[f], prefix[f+1]), one
