@@ -319,12 +319,14 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     d.data_size = c.data_size;
     d.brick = brick ? 1u : 0u;
     d.sd3 = (uint32_t)sd3;
+    d.sd3b = (uint32_t)(2 * sd3);
     d.sentinel = (uint32_t)phys_size;
     d.sentinel_code = init_code * 4u;
     {
         // occupancy-rank form of the block table
-        // (rank < 2^17 and sub_dim^3 < 2^24 keep the device product a
-        // 24-bit multiply; rank * sub_dim^3 < 2^31 as a block offset)
+        // (rank < 2^17 and 2 sub_dim^3 < 2^24 keep the device product, a
+        // byte offset, a 24-bit multiply; rank * sub_dim^3 < 2^31 as a
+        // block offset)
         // (the word count is padded to a power of two, so any block index's
         // word bits address a word inside the table)
         // The table also covers every block index a lane inside the grid
@@ -341,7 +343,7 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
             nwords *= 2;
             ++log2w;
         }
-        bool rank = nwords <= kMaxOccWords && sd3 < (1ull << 24);
+        bool rank = nwords <= kMaxOccWords && 2 * sd3 < (1ull << 24);
         std::vector<OccWord> words;
         if (rank) {
             words.assign(nwords, OccWord{0u, 0u});
@@ -373,19 +375,19 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
             magic_divisor((uint32_t)(sd * sd), &d.mag_sd2, &d.sh_sd2);
             magic_divisor((uint32_t)sd3, &d.mag_sd3, &d.sh_sd3);
         }
-        // in-block address contribution of each axis coordinate k in [0, sd]
-        // (k = sd only on carry lanes, which recompute their address)
-        std::vector<uint32_t> ctab(3 * (sd + 1));
-        for (uint64_t k = 0; k <= sd; ++k) {
-            const uint64_t kk = k < sd ? k : 0;
+        // in-block byte-offset contribution of each axis coordinate k in
+        // [0, sd]; k = sd (a lane whose index leaves the block row, which
+        // recomputes its address) contributes kCarryPoison
+        std::vector<uint32_t> ctab(3 * (sd + 1), kCarryPoison);
+        for (uint64_t k = 0; k < sd; ++k) {
             if (brick) {
-                ctab[k] = (uint32_t)((kk & 3) + 64 * (kk >> 2));
-                ctab[(sd + 1) + k] = (uint32_t)(4 * (kk & 3) + 64 * B * (kk >> 2));
-                ctab[2 * (sd + 1) + k] = (uint32_t)(16 * (kk & 3) + 64 * B * B * (kk >> 2));
+                ctab[k] = (uint32_t)(2 * ((k & 3) + 64 * (k >> 2)));
+                ctab[(sd + 1) + k] = (uint32_t)(2 * (4 * (k & 3) + 64 * B * (k >> 2)));
+                ctab[2 * (sd + 1) + k] = (uint32_t)(2 * (16 * (k & 3) + 64 * B * B * (k >> 2)));
             } else {
-                ctab[k] = (uint32_t)kk;
-                ctab[(sd + 1) + k] = (uint32_t)(kk * sd);
-                ctab[2 * (sd + 1) + k] = (uint32_t)(kk * sd * sd);
+                ctab[k] = (uint32_t)(2 * k);
+                ctab[(sd + 1) + k] = (uint32_t)(2 * k * sd);
+                ctab[2 * (sd + 1) + k] = (uint32_t)(2 * k * sd * sd);
             }
         }
         const size_t ne = st.entries.size();
@@ -400,15 +402,11 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
         TSL_HIP_CHECK(hipMemcpy(m->axis_tab.ptr, atab.data(), sizeof(AxisEntry) * atab.size(), hipMemcpyHostToDevice));
         d.axis_tab = m->axis_tab.as<AxisEntry>();
         d.axis_n = (uint32_t)atab.size();
-        // smallest fraction in [0, 1) whose in-block coordinate reaches sd
-        d.carry_bits = 0x7f800000u;
-        for (uint32_t bits = 0x3f7fffffu;; --bits) {
-            float f;
-            std::memcpy(&f, &bits, 4);
-            if (cvt_f32_u64(f / c.resolution) < sd) break;
-            d.carry_bits = bits;
-            if (bits == 0) break;
-        }
+        // a lane without a poisoned axis has s + pc < 2 (phys_size + sd^3)
+        // bytes (s of an unallocated or outside lane included); one with
+        // one to three poisoned axes stays below 2^32.  Larger maps send
+        // every lane through the recomputation.
+        d.carry_limit = 2 * (phys_size + sd3) <= kCarryPoison ? kCarryPoison : 0u;
     }
     return TSL_OK;
 }

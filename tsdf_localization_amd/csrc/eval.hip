@@ -141,10 +141,13 @@ struct LdsTables {
     uint32_t* scratch;  // 4 words for workgroup-wide values
 };
 
-// Palette value at byte offset off (a voxel code).
+// Palette value at byte offset off (a voxel code).  The palette starts at
+// LDS address 0 (the kernels declare no static LDS and stage it first), so
+// the code is the LDS address itself.
 __device__ __forceinline__ float palette_at(const float* lut, uint32_t off)
 {
-    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(lut) + off);
+    (void)lut;
+    return *reinterpret_cast<const __attribute__((address_space(3))) float*>((uintptr_t)off);
 }
 
 template <bool kRank>
@@ -275,22 +278,26 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     uint32_t s, pc, kx, ky, kz;
-                    bool blk, carry;
-                    fast_lookup<kU24, kRank>(map, t.atab, occ, ox[u][k], oy[u][k], oz[u][k], s, blk, pc, kx, ky, kz,
-                                             carry);
-                    // a whole block row lies inside the array, so only a lane
-                    // whose index leaves its row (carry) can fall past data_size
+                    bool blk;
+                    fast_lookup<kU24, kRank>(map, t.atab, occ, ox[u][k], oy[u][k], oz[u][k], s, blk, pc, kx, ky, kz);
+                    // byte address of the code; a lane whose reference index
+                    // leaves its block row (carry: an axis poisoned its part)
+                    // recomputes it from the linear index, and only such a
+                    // lane can fall past data_size (a whole row lies inside
+                    // the array)
                     uint32_t a = s + pc;
                     bool valid = blk;
+                    const bool carry = a >= map.carry_limit;
                     if (kTrace || __builtin_expect(carry, 0)) {
+                        const uint32_t s_el = s >> 1;
                         const uint32_t small = kx + mul_u<kU24>(ky, (uint32_t)map.sub_dim) +
                                                mul_u<kU24>(kz, (uint32_t)map.sub_dim_2);
-                        const uint32_t sum = s + small;
+                        const uint32_t sum = s_el + small;
                         if (carry) {
                             valid = valid & (sum < map.ds);
-                            a = kBrick ? phys_from_linear(map, sum) : sum;
+                            a = (kBrick ? phys_from_linear(map, sum) : sum) << 1;
                         }
-                        if (kTrace) full[u][k] = blk ? (uint64_t)s + small : map.data_size;
+                        if (kTrace) full[u][k] = blk ? (uint64_t)s_el + small : map.data_size;
                     }
                     addr[u][k] = a;
                     ok[u][k] = valid;
@@ -303,10 +310,10 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     code[u][k] = map.sentinel_code;
-                    // 32-bit byte offset from the array base (addresses < 2^31)
+                    // 32-bit byte offset from the array base (data_size < 2^31)
                     if (ok[u][k])
                         code[u][k] = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(map.codes) +
-                                                                        (addr[u][k] << 1));
+                                                                        addr[u][k]);
                 }
             }
 #pragma unroll
@@ -696,10 +703,26 @@ uint32_t batch_chunk(size_t N)
 
 namespace {
 
+// The scoring kernels read the palette at LDS address 0 (palette_at): they
+// must declare no static LDS, which would sit before the dynamic block.
+// Checked once per kernel.
+bool palette_at_zero(const void* kernel)
+{
+    hipFuncAttributes a{};
+    return hipFuncGetAttributes(&a, kernel) == hipSuccess && a.sharedSizeBytes == 0;
+}
+
+#define TSL_CHECK_PALETTE_AT_ZERO(kernel)                                                              \
+    do {                                                                                               \
+        static const bool tsl_lds0_ = palette_at_zero(reinterpret_cast<const void*>(kernel));          \
+        if (!tsl_lds0_) return set_error(TSL_ERR_UNSUPPORTED, "evaluation kernel declares static LDS"); \
+    } while (0)
+
 template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank>
 int launch_eval_t(dim3 grid, size_t lds, hipStream_t stream, const tsl_filter* f, const Tile& tile, size_t N,
                   float* partial, void* trace_out)
 {
+    TSL_CHECK_PALETTE_AT_ZERO((&k_eval<K, kP, kMode, kU24, kBrick, kRank>));
     const tsl_map* map = f->ev->map;
     FilterState* st = f->state.as<FilterState>();
     hipLaunchKernelGGL((k_eval<K, kP, kMode, kU24, kBrick, kRank>), grid, dim3(kBlock), lds, stream, map->dev,
@@ -1004,6 +1027,7 @@ namespace {
 template <int K, int kP, bool kU24, bool kBrick, bool kRank>
 int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid_out)
 {
+    TSL_CHECK_PALETTE_AT_ZERO((&k_eval_batch<K, kP, kU24, kBrick, kRank>));
     const size_t lds = lds_bytes(b.map->dev, kRank);
     if (grid_out) {
         int per_cu = 0;

@@ -94,12 +94,13 @@ struct SubTable {
 // allocated blocks, and rank(up) = prefix + popcount(bits below up).
 // Sub-voxel bucket b of one axis: the in-block coordinate is count, or
 // count + 1 when the fraction's bit pattern is >= thr; c_lo / c_hi are that
-// axis's contribution to the voxel's physical offset inside its block for
-// those two coordinates (a coordinate equal to sub_dim contributes as 0:
-// such lanes recompute their address).
+// axis's contribution to the voxel's physical byte offset inside its block
+// for those two coordinates (a coordinate equal to sub_dim contributes
+// kCarryPoison: such lanes recompute their address).
 struct AxisEntry {
     uint32_t thr, c_lo, c_hi, count;
 };
+constexpr uint32_t kCarryPoison = 1u << 30;
 
 struct OccWord {
     uint32_t bits;     // bit j: block 32w + j allocated
@@ -145,12 +146,13 @@ struct DevMap {
     // slot*sd3 + ((lx>>2) + (ly>>2)*B + (lz>>2)*B^2)*64 + (lx&3) + (ly&3)*4 + (lz&3)*16.
     uint32_t brick;                   // 1: bricked codes
     uint32_t sd3;                     // sub_dim^3
+    uint32_t sd3b;                    // 2 sub_dim^3: bytes of one block's codes
     uint32_t brick_cy, brick_cz;      // 64B - 16, 64B^2 - 64
     // per-axis sub-voxel buckets with the physical-address contribution of
     // both in-block coordinates a bucket can give: axis_tab[a * sub_entries + b]
     const AxisEntry* axis_tab;
     uint32_t axis_n;                  // 3 * sub_entries
-    uint32_t carry_bits;              // bits of the smallest fraction whose in-block coordinate is sub_dim (+inf if none)
+    uint32_t carry_limit;             // fast-tier byte address from which a lane recomputes (kCarryPoison; 0 = every lane)
     uint32_t mag_sd, sh_sd, mag_sd2, sh_sd2, mag_sd3, sh_sd3;  // x / d = umulhi(x, mag) >> sh, x < 2^31
     // occupancy-rank form of the block table (set when the offsets follow
     // setData order and the bitmap fits the LDS budget)

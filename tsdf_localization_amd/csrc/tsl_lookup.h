@@ -154,25 +154,25 @@ __device__ __forceinline__ uint32_t axis_part(uint32_t sub_base, uint32_t sub_sh
     return hi ? e.c_hi : e.c_lo;
 }
 
-// Fast tier for one lane, address form: block offset s with valid = block
-// allocated and inside the grid, the sum pc of the three per-axis in-block
-// address contributions, the in-block coordinates, and carry when some
-// coordinate equals sub_dim (the lane's reference index then leaves the
-// block row, and its address is recomputed from the linear index).  o >= 0
-// on every axis here, so trunc is a plain conversion and the fraction
-// o - trunc(o) is exactly fract(o).
+// Fast tier for one lane, address form, in bytes of the code array: block
+// offset s with valid = block allocated and inside the grid, the sum pc of
+// the three per-axis in-block address contributions, and the in-block
+// coordinates.  An axis whose coordinate equals sub_dim contributes
+// kCarryPoison, so s + pc >= carry_limit flags a lane whose reference index
+// leaves the block row (its address is then recomputed from the linear
+// index).  o >= 0 on every axis here, so trunc is a plain conversion and the
+// fraction o - trunc(o) is exactly fract(o).
 template <bool kU24, bool kRank>
 __device__ __forceinline__ void fast_lookup(const DevMap& m, const AxisEntry* __restrict__ atab,
                                             const OccWord* __restrict__ occ, float ox, float oy, float oz,
                                             uint32_t& s, bool& valid, uint32_t& pc, uint32_t& kx, uint32_t& ky,
-                                            uint32_t& kz, bool& carry)
+                                            uint32_t& kz)
 {
     const uint32_t ix = (uint32_t)ox, iy = (uint32_t)oy, iz = (uint32_t)oz;
     const float fx = __builtin_amdgcn_fractf(ox), fy = __builtin_amdgcn_fractf(oy), fz = __builtin_amdgcn_fractf(oz);
     const uint32_t n = m.sub_entries;
     pc = axis_part(m.sub_base, m.sub_shift, atab, fx, kx) + axis_part(m.sub_base, m.sub_shift, atab + n, fy, ky) +
          axis_part(m.sub_base, m.sub_shift, atab + 2u * n, fz, kz);
-    carry = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz)) >= m.carry_bits;
     const bool grid = (ix < m.gthr_x) & (iy < m.gthr_y) & (iz < m.gthr_z);
     const uint32_t up = ix + mul_u<kU24>(iy, m.up_dim_x) + mul_u<kU24>(iz, m.up_dim_2);
     if (kRank) {
@@ -180,13 +180,13 @@ __device__ __forceinline__ void fast_lookup(const DevMap& m, const AxisEntry* __
         // it; inside the grid every up lies in the table (padded past
         // grid_occ_size with unallocated blocks)
         bool set;
-        s = rank_block(occ, m.occ_log2, up, m.sd3, set);
+        s = rank_block(occ, m.occ_log2, up, m.sd3b, set);
         valid = grid & set;
     } else {
         int32_t g = -1;
         if (grid & (up < m.occ)) g = m.grid_occ[up];
-        s = (uint32_t)g;
         valid = g >= 0;
+        s = valid ? (uint32_t)g << 1 : 0u;
     }
 }
 
