@@ -25,6 +25,7 @@ constexpr int kBlock = 256;           // threads per workgroup (4 wavefronts)
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kXcds = 8;         // accelerator complex dies of the MI355X (one L2 each)
 constexpr uint32_t kTile = 512;       // scan points staged in LDS per step (8 KB), a multiple of 64 * 4
+constexpr uint32_t kFinishSmallSlices = 16;  // filters of up to 16 * kBlock particles finish in one workgroup
 
 __device__ __forceinline__ float wave_sum(float v)
 {
@@ -38,41 +39,6 @@ __device__ __forceinline__ double wave_sum_d(double v)
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
-}
-
-// Block-wide double sum with a fixed combination order; result valid in
-// thread 0.
-__device__ double block_sum_d(double v, double* sh)
-{
-    v = wave_sum_d(v);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) sh[wave] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0) {
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
-    }
-    __syncthreads();
-    return r;
-}
-
-// Nine block sums in one pass: thread k < 9 returns the total of v[k] over
-// the workgroup, each formed exactly as block_sum_d forms it (wave sums,
-// then the waves in order).
-__device__ double block_sum9(const double (&v)[9], double* sh /* kWaves * 9 */)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const double w = wave_sum_d(v[k]);
-        if (lane == 0) sh[wave * 9 + k] = w;
-    }
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x < 9)
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w * 9 + threadIdx.x];
-    __syncthreads();
-    return r;
 }
 
 struct Tf12 {
@@ -431,24 +397,60 @@ __global__ void k_pose_m(const Member* __restrict__ members)
     for (int k = 0; k < 12; ++k) m.L[(size_t)i * 12 + k] = out[k];
 }
 
-// The reductions below avoid a "last workgroup finishes" step: on this GPU
-// the device-scope fence it needs writes back and invalidates the L2 of
-// the XCD, once per wavefront.  Instead every workgroup that needs a total
-// re-forms it from the workgroup partials, in one fixed order, and a small
-// final kernel forms the pose sums.  Partials of a member: blocks[0, nb) the
-// weight sums, blocks[nb + q * nb + b] the nine pose sums of workgroup b.
+// Reduction partials of a member: blocks[0, nb) the weight sums of the nb
+// workgroup-sized slices, blocks[nb + q * nb + b] the nine pose sums of
+// slice b.
+
+// The slice functions below run as groups of kBlock threads: kG groups per
+// workgroup (1 in the member kernels, 4 in k_finish_small), group g =
+// threadIdx.x / kBlock working on its own slice; every thread of the
+// workgroup calls them (their barriers are workgroup-wide).  A group sum is
+// formed in one fixed order whatever kG is (the fixed butterfly of each
+// wave, then the group's waves in order from 0.0), so a slice gives the
+// same bits in either kernel.
+template <int kG>
+__device__ __forceinline__ double group_sum_d(double v, double* sh /* kWaves * kG */)
+{
+    v = wave_sum_d(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    if ((threadIdx.x & (kBlock - 1)) == 0)
+        for (int w = 0; w < kWaves; ++w) r += sh[wave + w];
+    __syncthreads();
+    return r;  // in the group's thread 0
+}
+
+template <int kG>
+__device__ __forceinline__ double group_sum9(const double (&v)[9], double* sh /* kWaves * kG * 9 */)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const double w = wave_sum_d(v[k]);
+        if (lane == 0) sh[wave * 9 + k] = w;
+    }
+    __syncthreads();
+    double r = 0.0;
+    const uint32_t lt = threadIdx.x & (kBlock - 1);
+    if (lt < 9)
+        for (int w = 0; w < kWaves; ++w) r += sh[(wave + w) * 9 + lt];
+    __syncthreads();
+    return r;  // sum k in the group's thread k < 9
+}
 
 // raw[p] = sum over chunks (ascending) of partial[c][p], in double, stored as
-// float; workgroup partial sums of raw in double.
-__global__ __launch_bounds__(kBlock) void k_reduce_m(const Member* __restrict__ members)
+// float; the partial sum of raw in double over slice b (particles
+// [b kBlock, (b+1) kBlock)) into blocks[b] (a slice past nb contributes and
+// writes nothing).
+template <int kG>
+__device__ __forceinline__ void reduce_slice(const Member& m, uint32_t P, uint32_t nb, uint32_t b, double* sh)
 {
-    const Member& m = members[blockIdx.y];
-    FilterState* st = m.st;
-    __shared__ double sh[kWaves];
-    const uint32_t P = st->count;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lt = threadIdx.x & (kBlock - 1);
+    const uint32_t i = b * kBlock + lt;
     double w = 0.0;
-    if (i < P) {
+    if (b < nb && i < P) {
         const uint32_t slot = m.inv ? m.inv[i] : i;
         double acc = 0.0;
         for (uint32_t c = 0; c < m.chunks; ++c) acc += (double)m.partial[(size_t)c * m.stride + slot];
@@ -456,49 +458,53 @@ __global__ __launch_bounds__(kBlock) void k_reduce_m(const Member* __restrict__ 
         m.raw[i] = wf;
         w = (double)wf;
     }
-    const double b = block_sum_d(w, sh);
-    if (threadIdx.x == 0) m.blocks[blockIdx.x] = b;
+    const double t = group_sum_d<kG>(w, sh);
+    if (lt == 0 && b < nb) m.blocks[b] = t;
 }
 
-// The total of nb workgroup partials (thread k adds partials k, k + kBlock,
-// ... in order, then block_sum_d): the same value in every workgroup.
-__device__ __forceinline__ double total_of(const double* __restrict__ part, uint32_t nb, double* sh)
-{
-    double t = 0.0;
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) t += part[k];
-    return block_sum_d(t, sh);  // thread 0
-}
-
-// sum = total of the raw weights (stats[0]; status "no valid particle" when
-// it is 0 as a float); w_i = raw_i / (float)sum written to
-// particles[i].second (the reference's particles[index].second /=
-// weight_sum, float division), plus workgroup partials of the nine
-// weighted pose sums (tsdf_evaluator.cpp:198-217) in double.
-__global__ __launch_bounds__(kBlock) void k_normalize_m(const Member* __restrict__ members)
+__global__ __launch_bounds__(kBlock) void k_reduce_m(const Member* __restrict__ members)
 {
     const Member& m = members[blockIdx.y];
-    FilterState* st = m.st;
-    __shared__ double sh[kWaves * 9];
-    __shared__ double total_sh;
-    const uint32_t nb = gridDim.x;
-    const uint32_t P = st->count;
-    {
-        const double total = total_of(m.blocks, nb, sh);
-        if (threadIdx.x == 0) {
-            total_sh = total;
-            if (blockIdx.x == 0) {
-                st->stats[0] = total;
-                st->status = ((float)total == 0.0f) ? kStNoValid : kStOk;
-            }
-        }
-        __syncthreads();
+    __shared__ double sh[kWaves];
+    reduce_slice<1>(m, m.st->count, gridDim.x, blockIdx.x, sh);
+}
+
+// sum = total of the nb raw-weight partials (thread k of a group adds
+// partials k, k + kBlock, ... in order, then the group sum; every group and
+// every workgroup forms the same value), returned to every thread;
+// published to stats[0] (and the status "no valid particle" when it is 0
+// as a float) by thread 0 of a publishing workgroup.
+template <int kG>
+__device__ __forceinline__ float weight_total(const Member& m, uint32_t nb, bool publish, double* sh,
+                                              double* total_sh /* kG */)
+{
+    const uint32_t lt = threadIdx.x & (kBlock - 1);
+    double t = 0.0;
+    for (uint32_t k = lt; k < nb; k += kBlock) t += m.blocks[k];
+    const double total = group_sum_d<kG>(t, sh);
+    if (lt == 0) total_sh[threadIdx.x / kBlock] = total;
+    if (threadIdx.x == 0 && publish) {
+        m.st->stats[0] = total;
+        m.st->status = ((float)total == 0.0f) ? kStNoValid : kStOk;
     }
-    const float sum = (float)total_sh;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __syncthreads();
+    return (float)total_sh[threadIdx.x / kBlock];
+}
+
+// Slice b: w_i = raw_i / sum written to particles[i].second (the
+// reference's particles[index].second /= weight_sum, float division), and
+// the slice's partials of the nine weighted pose sums
+// (tsdf_evaluator.cpp:198-217) in double into blocks[nb + q nb + b].
+template <int kG>
+__device__ __forceinline__ void normalize_slice(const Member& m, uint32_t P, uint32_t nb, uint32_t b, float sum,
+                                                double* sh)
+{
+    const uint32_t lt = threadIdx.x & (kBlock - 1);
+    const uint32_t i = b * kBlock + lt;
     double q[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) q[k] = 0.0;
-    if (i < P && sum != 0.0f) {
+    if (b < nb && i < P && sum != 0.0f) {
         tsl_particle* particles = m.particles;
         const float w = m.raw[i] / sum;
         particles[i].second = w;
@@ -514,29 +520,69 @@ __global__ __launch_bounds__(kBlock) void k_normalize_m(const Member* __restrict
             q[4 + 2 * a] = (double)c * wd;
         }
     }
-    const double b = block_sum9(q, sh);
-    if (threadIdx.x < 9) m.blocks[nb + (size_t)threadIdx.x * nb + blockIdx.x] = b;
+    const double t = group_sum9<kG>(q, sh);
+    if (lt < 9 && b < nb) m.blocks[nb + (size_t)lt * nb + b] = t;
 }
 
-// One workgroup per member: the nine pose sums from nb workgroup partials
-// into stats[1..9].
-__global__ __launch_bounds__(kBlock) void k_pose_sums_m(const Member* __restrict__ members, uint32_t nb)
+// The nine pose sums from nb slice partials into stats[1..9] (every group
+// forms them; group 0 stores).
+template <int kG>
+__device__ __forceinline__ void pose_sums(const Member& m, uint32_t nb, double* sh)
 {
-    const Member& m = members[blockIdx.y];
-    __shared__ double sh[kWaves * 9];
+    const uint32_t lt = threadIdx.x & (kBlock - 1);
     double t[9];
 #pragma unroll
     for (int qq = 0; qq < 9; ++qq) t[qq] = 0.0;
     const double* part = m.blocks + nb;
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+    for (uint32_t k = lt; k < nb; k += kBlock) {
         double v[9];
 #pragma unroll
         for (int qq = 0; qq < 9; ++qq) v[qq] = part[(size_t)qq * nb + k];
 #pragma unroll
         for (int qq = 0; qq < 9; ++qq) t[qq] += v[qq];
     }
-    const double total = block_sum9(t, sh);
+    const double total = group_sum9<kG>(t, sh);
     if (threadIdx.x < 9) m.st->stats[1 + threadIdx.x] = total;
+}
+
+// The reductions avoid a "last workgroup finishes" step: on this GPU the
+// device-scope fence it needs writes back and invalidates the L2 of the
+// XCD, once per wavefront.  Instead every workgroup that needs a total
+// re-forms it from the slice partials in one fixed order, and a small final
+// kernel forms the pose sums.  A lone filter of at most
+// kFinishSmallSlices slices runs all three steps in k_finish_small, four
+// slices at a time, with the same functions and therefore the same bits.
+__global__ __launch_bounds__(kBlock) void k_normalize_m(const Member* __restrict__ members)
+{
+    const Member& m = members[blockIdx.y];
+    __shared__ double sh[kWaves * 9];
+    __shared__ double total_sh[1];
+    const uint32_t nb = gridDim.x;
+    const float sum = weight_total<1>(m, nb, blockIdx.x == 0, sh, total_sh);
+    normalize_slice<1>(m, m.st->count, nb, blockIdx.x, sum, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pose_sums_m(const Member* __restrict__ members, uint32_t nb)
+{
+    __shared__ double sh[kWaves * 9];
+    pose_sums<1>(members[blockIdx.y], nb, sh);
+}
+
+constexpr int kFinishGroups = 4;
+__global__ __launch_bounds__(kBlock * kFinishGroups) void k_finish_small(const Member* __restrict__ members,
+                                                                           uint32_t nb)
+{
+    const Member& m = members[blockIdx.y];
+    __shared__ double sh[kWaves * kFinishGroups * 9];
+    __shared__ double total_sh[kFinishGroups];
+    const uint32_t P = m.st->count;
+    const uint32_t g = threadIdx.x / kBlock;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kFinishGroups) reduce_slice<kFinishGroups>(m, P, nb, b0 + g, sh);
+    __syncthreads();  // the slice partials are written
+    const float sum = weight_total<kFinishGroups>(m, nb, true, sh, total_sh);
+    for (uint32_t b0 = 0; b0 < nb; b0 += kFinishGroups) normalize_slice<kFinishGroups>(m, P, nb, b0 + g, sum, sh);
+    __syncthreads();  // the pose partials are written
+    pose_sums<kFinishGroups>(m, nb, sh);
 }
 
 // Restore every member from its snapshot.
@@ -963,6 +1009,11 @@ int launch_update(tsl_filter* f, hipStream_t stream)
     TSL_TRY(f->blocks.reserve(sizeof(double) * kReducePartials * (size_t)nb));
     Member* dm;
     TSL_TRY(filter_member(f, &dm, tile.chunk_len, tile.chunks));
+    if (nb <= kFinishSmallSlices) {
+        hipLaunchKernelGGL(k_finish_small, dim3(1), dim3(kBlock * kFinishGroups), 0, stream, dm, nb);
+        TSL_HIP_CHECK(hipGetLastError());
+        return TSL_OK;
+    }
     hipLaunchKernelGGL(k_reduce_m, dim3(nb), dim3(kBlock), 0, stream, dm);
     TSL_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_normalize_m, dim3(nb), dim3(kBlock), 0, stream, dm);

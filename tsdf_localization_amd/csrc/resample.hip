@@ -34,6 +34,7 @@ namespace tsl {
 namespace {
 
 constexpr int kRsThreads = 1024;
+constexpr uint32_t kRsSmall = 8192;  // a lone filter with at most this many candidates resamples in one launch
 
 __host__ __device__ __forceinline__ float rec_step(float u, double inv_m) { return (float)((double)u + inv_m); }
 
@@ -88,10 +89,8 @@ __host__ __device__ uint64_t cap_walk(uint64_t M, float u0, double inv_m, USeg* 
 // Plan (one thread per member): M from the device count, cap, run table.
 // A member whose update failed (status set) or that holds no particles is
 // skipped (cap = 0) and keeps its particles.
-__global__ void k_rs_plan(Member* __restrict__ members, float u0_arg)
+__device__ void plan_member(const Member& m, float u0_arg)
 {
-    if (threadIdx.x != 0) return;
-    Member& m = members[blockIdx.x];
     FilterState* st = m.st;
     RsPlan* p = m.plan;
     const uint32_t M = st->count;
@@ -121,6 +120,12 @@ __global__ void k_rs_plan(Member* __restrict__ members, float u0_arg)
     p->inv_m = inv_m;
     p->nseg = nseg;
     p->cap = (uint32_t)cap;
+}
+
+__global__ void k_rs_plan(Member* __restrict__ members, float u0_arg)
+{
+    if (threadIdx.x != 0) return;
+    plan_member(members[blockIdx.x], u0_arg);
 }
 
 __device__ __forceinline__ double wave_incl_scan(double v, int lane)
@@ -182,14 +187,13 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_blk(const Member* __restrict_
 // element of its span but the first (whose predecessor another workgroup
 // writes; k_rs_count checks those and repairs serially from the first
 // mismatch).
-__global__ __launch_bounds__(kRsThreads) void k_rs_scan(const Member* __restrict__ members)
+// Scan workgroup b of B (blockDim.x == kRsThreads).
+__device__ void scan_span(const Member& mb, uint32_t b, uint32_t B)
 {
-    const Member& mb = members[blockIdx.y];
     RsPlan* p = mb.plan;
-    if (p->cap == 0) return;
     const uint32_t M = p->M;
-    const uint32_t span = rs_span(M, gridDim.x);
-    const uint32_t lo = blockIdx.x * span;
+    const uint32_t span = rs_span(M, B);
+    const uint32_t lo = b * span;
     if (lo >= M) return;
     const uint32_t hi = min(M, lo + span);
     const tsl_particle* __restrict__ in = mb.particles;
@@ -200,7 +204,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scan(const Member* __restrict
     const int lane = tid & 63, wave = tid >> 6;
     if (tid == 0) {
         double off = 0.0;
-        for (uint32_t b = 0; b < blockIdx.x; ++b) off += p->blk[b];
+        for (uint32_t k = 0; k < b; ++k) off += p->blk[k];
         carry_sh = off;
     }
     __syncthreads();
@@ -242,6 +246,13 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scan(const Member* __restrict
     }
 }
 
+__global__ __launch_bounds__(kRsThreads) void k_rs_scan(const Member* __restrict__ members)
+{
+    const Member& mb = members[blockIdx.y];
+    if (mb.plan->cap == 0) return;
+    scan_span(mb, blockIdx.x, gridDim.x);
+}
+
 __device__ __forceinline__ float u_candidate(const USeg* __restrict__ seg, uint32_t nseg, uint32_t j)
 {
     uint32_t lo = 0, hi = nseg;  // last run with j0 <= j
@@ -256,24 +267,29 @@ __device__ __forceinline__ float u_candidate(const USeg* __restrict__ seg, uint3
 // Phase B (grid x candidates, y members): U_j candidates from the run
 // table, each checked against one step of the recurrence from its
 // predecessor.
-__global__ void k_rs_u(const Member* __restrict__ members)
+__device__ __forceinline__ void u_step(const Member& mb, const RsPlan* p, uint32_t j)
 {
-    const Member& mb = members[blockIdx.y];
-    const RsPlan* p = mb.plan;
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= p->cap) return;
     const float u = u_candidate(p->seg, p->nseg, j);
     mb.ubuf[j] = u;
     const float expect = j == 0 ? p->u0 : (float)((double)u_candidate(p->seg, p->nseg, j - 1) + p->inv_m);
     if (u != expect) atomicMin(&mb.st->first_bad, j);
 }
 
+__global__ void k_rs_u(const Member* __restrict__ members)
+{
+    const Member& mb = members[blockIdx.y];
+    const RsPlan* p = mb.plan;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= p->cap) return;
+    u_step(mb, p, j);
+}
+
 // Phase C (one thread per member): the running-sum checks left to it and
 // serial repairs of s and U from their first mismatches, then the number
 // emitted J = #{j : U_j < s_{M-1}} (U is non-decreasing).
-__global__ void k_rs_count(const Member* __restrict__ members, uint32_t scan_blocks)
+// Run by the first wavefront (threads 0..63) of a workgroup.
+__device__ void count_member(const Member& mb, uint32_t scan_blocks)
 {
-    const Member& mb = members[blockIdx.x];
     RsPlan* p = mb.plan;
     const uint32_t cap = p->cap;
     if (cap == 0) return;
@@ -284,10 +300,12 @@ __global__ void k_rs_count(const Member* __restrict__ members, uint32_t scan_blo
     // running sum: the first element of each scan workgroup's span (one
     // wavefront, a span per lane), then a serial repair from the first
     // mismatch
-    uint32_t bad = p->s_bad;
+    // (both first-mismatch words are read as atomics: a fused launch sets
+    // them with atomics earlier in the same kernel)
+    uint32_t bad = __hip_atomic_load(&p->s_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     {
         const uint32_t span = rs_span(M, scan_blocks);
-        for (uint32_t b = 1 + threadIdx.x; b < scan_blocks; b += blockDim.x) {
+        for (uint32_t b = 1 + threadIdx.x; b < scan_blocks; b += 64) {
             const uint32_t m = b * span;
             if (m < M && sum[m] != sum[m - 1] + (double)in[m].second) bad = min(bad, m);
         }
@@ -306,7 +324,7 @@ __global__ void k_rs_count(const Member* __restrict__ members, uint32_t scan_blo
     }
     float* __restrict__ U = mb.ubuf;
     const double inv_m = p->inv_m;
-    const uint32_t fb = st->first_bad;
+    const uint32_t fb = __hip_atomic_load(&st->first_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (fb != 0xffffffffu) {
         float u = fb == 0 ? p->u0 : (float)((double)U[fb - 1] + inv_m);
         for (uint32_t j = fb; j < cap; ++j) {
@@ -329,14 +347,15 @@ __global__ void k_rs_count(const Member* __restrict__ members, uint32_t scan_blo
     st->resample_count = lo;
 }
 
+__global__ void k_rs_count(const Member* __restrict__ members, uint32_t scan_blocks)
+{
+    count_member(members[blockIdx.x], scan_blocks);
+}
+
 // Phase D (grid x outputs, y members): output j is particle
 // min{m : s_m > U_j}, written to alt (and its index to idx_out).
-__global__ void k_rs_assign(const Member* __restrict__ members)
+__device__ __forceinline__ void assign_step(const Member& mb, const RsPlan* p, uint32_t j)
 {
-    const Member& mb = members[blockIdx.y];
-    const RsPlan* p = mb.plan;
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p->cap == 0 || j >= mb.st->resample_count) return;
     const uint32_t M = p->M;
     const double* __restrict__ s = mb.cumsum;
     const double u = (double)mb.ubuf[j];
@@ -350,6 +369,15 @@ __global__ void k_rs_assign(const Member* __restrict__ members)
     mb.alt[j] = mb.particles[m];
 }
 
+__global__ void k_rs_assign(const Member* __restrict__ members)
+{
+    const Member& mb = members[blockIdx.y];
+    const RsPlan* p = mb.plan;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p->cap == 0 || j >= mb.st->resample_count) return;
+    assign_step(mb, p, j);
+}
+
 // Phase E (grid x outputs, y members): the emitted particles replace the
 // resident set and its count.
 __global__ void k_rs_copy_back(const Member* __restrict__ members)
@@ -360,6 +388,33 @@ __global__ void k_rs_copy_back(const Member* __restrict__ members)
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < J) mb.particles[j] = mb.alt[j];
     if (blockIdx.x == 0 && threadIdx.x == 0) mb.st->count = J;
+}
+
+// The whole resampling of one member in one workgroup (kRsThreads
+// threads), for sets up to kRsSmall candidates: the phases above in
+// order, separated by workgroup barriers instead of launches, with the
+// same functions (a single scan span) and therefore the same bits.
+__global__ __launch_bounds__(kRsThreads) void k_rs_small(Member* __restrict__ members, float u0_arg, int copy_back)
+{
+    const Member& mb = members[blockIdx.y];
+    const RsPlan* p = mb.plan;
+    if (threadIdx.x == 0) plan_member(mb, u0_arg);
+    __syncthreads();
+    const uint32_t cap = p->cap;
+    if (cap == 0) return;
+    scan_span(mb, 0, 1);
+    for (uint32_t j = threadIdx.x; j < cap; j += kRsThreads) u_step(mb, p, j);
+    __syncthreads();
+    if (threadIdx.x < 64) count_member(mb, 1);
+    __syncthreads();
+    // re-read past this workgroup's cached copy: thread 0 may have cleared it
+    if (__hip_atomic_load(&mb.plan->cap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    const uint32_t J = mb.st->resample_count;
+    for (uint32_t j = threadIdx.x; j < J; j += kRsThreads) assign_step(mb, p, j);
+    if (!copy_back) return;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < J; j += kRsThreads) mb.particles[j] = mb.alt[j];
+    if (threadIdx.x == 0) mb.st->count = J;
 }
 
 }  // namespace
@@ -380,6 +435,11 @@ int launch_resample_members(Member* d_members, uint32_t n, uint32_t grid_cap, fl
                             hipStream_t stream)
 {
     if (n == 0) return TSL_OK;
+    if (n == 1 && grid_cap <= kRsSmall) {
+        hipLaunchKernelGGL(k_rs_small, dim3(1, 1), dim3(kRsThreads), 0, stream, d_members, u0_arg, copy_back ? 1 : 0);
+        TSL_HIP_CHECK(hipGetLastError());
+        return TSL_OK;
+    }
     const unsigned gx = (grid_cap + 255) / 256 ? (grid_cap + 255) / 256 : 1;
     hipLaunchKernelGGL(k_rs_plan, dim3(n), dim3(64), 0, stream, d_members, u0_arg);
     TSL_HIP_CHECK(hipGetLastError());
