@@ -55,7 +55,10 @@ def _alone(tsl, ev, parts, pts, tf, u0, k, pp):
     return weighted, f.get_particles(), pose
 
 
-SIZES = [(37, 300), (1000, 1500), (257, 64), (2500, 900), (64, 1)]
+# includes the edges of the lone filter's one-workgroup finishing kernels
+# (4096 particles: reduction in one workgroup, 4097: not; 8000: resampling
+# in one workgroup, reduction not), which must match the member kernels
+SIZES = [(37, 300), (1000, 1500), (257, 64), (2500, 900), (64, 1), (4096, 700), (4097, 500), (8000, 200)]
 
 
 @pytest.mark.parametrize("k,pp", [(1, 1), (2, 1), (1, 4)])
