@@ -239,20 +239,45 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
             uint64_t full[kP][K];
             uint32_t addr[kP][K];
             bool ok[kP][K];
+            uint32_t sb[kP][K];
+            bool blk[kP][K];
+            bool any = false;
 #pragma unroll
             for (int u = 0; u < kP; ++u) {
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    uint32_t s, pc, kx, ky, kz;
-                    bool blk;
-                    fast_lookup<kU24, kRank>(map, t.atab, occ, ox[u][k], oy[u][k], oz[u][k], s, blk, pc, kx, ky, kz);
+                    fast_block<kU24, kRank>(map, occ, ox[u][k], oy[u][k], oz[u][k], sb[u][k], blk[u][k]);
+                    any = any | blk[u][k];
+                }
+            }
+            // no pair of the iteration in an allocated block inside the grid:
+            // every pair reads init_value (the sentinel code's value), and
+            // the in-block part is skipped
+            const bool none = !kTrace && __builtin_amdgcn_ballot_w64(any) == 0;
+#pragma unroll
+            for (int u = 0; u < kP; ++u) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    ok[u][k] = false;
+                    addr[u][k] = 0u;
+                    if (kTrace) full[u][k] = map.data_size;
+                }
+            }
+            if (!none) {
+#pragma unroll
+            for (int u = 0; u < kP; ++u) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    uint32_t pc, kx, ky, kz;
+                    fast_inblock(map, t.atab, ox[u][k], oy[u][k], oz[u][k], pc, kx, ky, kz);
+                    const uint32_t s = sb[u][k];
                     // byte address of the code; a lane whose reference index
                     // leaves its block row (carry: an axis poisoned its part)
                     // recomputes it from the linear index, and only such a
                     // lane can fall past data_size (a whole row lies inside
                     // the array)
                     uint32_t a = s + pc;
-                    bool valid = blk;
+                    bool valid = blk[u][k];
                     const bool carry = a >= map.carry_limit;
                     if (kTrace || __builtin_expect(carry, 0)) {
                         const uint32_t s_el = s >> 1;
@@ -263,11 +288,12 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
                             valid = valid & (sum < map.ds);
                             a = (kBrick ? phys_from_linear(map, sum) : sum) << 1;
                         }
-                        if (kTrace) full[u][k] = blk ? (uint64_t)s_el + small : map.data_size;
+                        if (kTrace) full[u][k] = blk[u][k] ? (uint64_t)s_el + small : map.data_size;
                     }
                     addr[u][k] = a;
                     ok[u][k] = valid;
                 }
+            }
             }
             // rejected pairs take the code of init_value without a load
             uint32_t code[kP][K];

@@ -154,25 +154,17 @@ __device__ __forceinline__ uint32_t axis_part(uint32_t sub_base, uint32_t sub_sh
     return hi ? e.c_hi : e.c_lo;
 }
 
-// Fast tier for one lane, address form, in bytes of the code array: block
-// offset s with valid = block allocated and inside the grid, the sum pc of
-// the three per-axis in-block address contributions, and the in-block
-// coordinates.  An axis whose coordinate equals sub_dim contributes
-// kCarryPoison, so s + pc >= carry_limit flags a lane whose reference index
-// leaves the block row (its address is then recomputed from the linear
-// index).  o >= 0 on every axis here, so trunc is a plain conversion and the
-// fraction o - trunc(o) is exactly fract(o).
+// Fast tier for one lane, in two parts.  o >= 0 on every axis here, so
+// trunc is a plain conversion and the fraction o - trunc(o) is exactly
+// fract(o).
+//
+// Block part: the block's offset s in bytes of the code array, with valid
+// = block allocated and inside the grid.
 template <bool kU24, bool kRank>
-__device__ __forceinline__ void fast_lookup(const DevMap& m, const AxisEntry* __restrict__ atab,
-                                            const OccWord* __restrict__ occ, float ox, float oy, float oz,
-                                            uint32_t& s, bool& valid, uint32_t& pc, uint32_t& kx, uint32_t& ky,
-                                            uint32_t& kz)
+__device__ __forceinline__ void fast_block(const DevMap& m, const OccWord* __restrict__ occ, float ox, float oy,
+                                           float oz, uint32_t& s, bool& valid)
 {
     const uint32_t ix = (uint32_t)ox, iy = (uint32_t)oy, iz = (uint32_t)oz;
-    const float fx = __builtin_amdgcn_fractf(ox), fy = __builtin_amdgcn_fractf(oy), fz = __builtin_amdgcn_fractf(oz);
-    const uint32_t n = m.sub_entries;
-    pc = axis_part(m.sub_base, m.sub_shift, atab, fx, kx) + axis_part(m.sub_base, m.sub_shift, atab + n, fy, ky) +
-         axis_part(m.sub_base, m.sub_shift, atab + 2u * n, fz, kz);
     const bool grid = (ix < m.gthr_x) & (iy < m.gthr_y) & (iz < m.gthr_z);
     const uint32_t up = ix + mul_u<kU24>(iy, m.up_dim_x) + mul_u<kU24>(iz, m.up_dim_2);
     if (kRank) {
@@ -188,6 +180,20 @@ __device__ __forceinline__ void fast_lookup(const DevMap& m, const AxisEntry* __
         valid = g >= 0;
         s = valid ? (uint32_t)g << 1 : 0u;
     }
+}
+
+// In-block part: the sum pc of the three per-axis in-block byte-offset
+// contributions and the in-block coordinates.  An axis whose coordinate
+// equals sub_dim contributes kCarryPoison, so s + pc >= carry_limit flags a
+// lane whose reference index leaves the block row (its address is then
+// recomputed from the linear index).
+__device__ __forceinline__ void fast_inblock(const DevMap& m, const AxisEntry* __restrict__ atab, float ox, float oy,
+                                             float oz, uint32_t& pc, uint32_t& kx, uint32_t& ky, uint32_t& kz)
+{
+    const float fx = __builtin_amdgcn_fractf(ox), fy = __builtin_amdgcn_fractf(oy), fz = __builtin_amdgcn_fractf(oz);
+    const uint32_t n = m.sub_entries;
+    pc = axis_part(m.sub_base, m.sub_shift, atab, fx, kx) + axis_part(m.sub_base, m.sub_shift, atab + n, fy, ky) +
+         axis_part(m.sub_base, m.sub_shift, atab + 2u * n, fz, kz);
 }
 
 // Wrap tier for one lane (|o| < R on every axis): the reference's index,
