@@ -131,7 +131,8 @@ def bench_batched(args):
     in 500-50k, scans of 2k-30k points): one step = every filter restored,
     updated and resampled.  Timed three ways on one stream: plain per-filter
     launches (the drop-in path, one filter after the other), the batched
-    launch sequence, and the batched step replayed from a HIP graph."""
+    launch sequence with the tile the batch autotuner picks, and the batched
+    step replayed from a HIP graph."""
     import tsdf_localization_amd as tsl
     from tsdf_localization_amd import synth
 
@@ -183,21 +184,17 @@ def bench_batched(args):
 
     filters = make_filters()
     b = tsl.Batch(ev, filters)
-    results = {}
-    for k, chunk in ((1, 0), (2, 1024), (2, 4096), (2, 0)):
-        b.set_tile(k, chunk, 1)
-        ms = timed(lambda i: b.step_async(draws[i], restore=True, stream=stream))
-        b.sync(stream)
-        results[(k, chunk)] = ms
-        log(f"batched eager K={k} chunk={chunk or 'auto'}: {ms:.3f} ms/step, {evals / ms * 1e3:.3e} evals/s")
-    k, chunk = min(results, key=results.get)
-    b.set_tile(k, chunk, 1)
+    k, pp, chunk, tune_ms = b.autotune(stream)
+    log(f"batch autotune: K={k} points/lane={pp} chunk={chunk or 'auto'} ({tune_ms:.3f} ms per batched update)")
     eager_ms = timed(lambda i: b.step_async(draws[i], restore=True, stream=stream))
+    b.sync(stream)
+    log(f"batched eager: {eager_ms:.3f} ms/step, {evals / eager_ms * 1e3:.3e} evals/s")
     b.capture(stream, restore=True)
     graph_ms = timed(lambda i: b.launch(draws[i], stream))
     _, counts, status = b.sync(stream)
-    log(f"batched graph K={k} chunk={chunk or 'auto'}: {graph_ms:.3f} ms/step, {evals / graph_ms * 1e3:.3e} evals/s "
-        f"({plain_ms / graph_ms:.2f}x plain); member status {sorted(set(status.tolist()))}")
+    log(f"batched graph K={k} points/lane={pp} chunk={chunk or 'auto'}: {graph_ms:.3f} ms/step, "
+        f"{evals / graph_ms * 1e3:.3e} evals/s ({plain_ms / graph_ms:.2f}x plain); "
+        f"member status {sorted(set(status.tolist()))}")
     out = {
         "metric": "particle*point evals/s",
         "value": evals / graph_ms * 1e3,
@@ -214,7 +211,8 @@ def bench_batched(args):
         "config": {"workload": "batched filters", "filters": n, "particles_total": w.meta["particles_total"],
                    "evals_per_step": evals, "map": "sparse 512x512x96 @0.05 m int16",
                    "step": "restore + pose + evaluate + normalise + resample, every filter",
-                   "tile": {"particles_per_wavefront": k, "chunk_points": chunk or "auto"}},
+                   "tile": {"particles_per_wavefront": k, "points_per_lane": pp, "chunk_points": chunk or "auto",
+                            "picked_by": "tsl_batch_autotune"}},
         "plain_ms_per_step": plain_ms,
         "plain_value": evals / plain_ms * 1e3,
         "batched_eager_ms_per_step": eager_ms,

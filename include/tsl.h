@@ -355,6 +355,17 @@ int tsl_batch_destroy(tsl_batch* b);
  * batch. */
 int tsl_batch_set_tile(tsl_batch* b, int k, int points_per_lane, uint32_t chunk_len);
 
+/* Time the batched update (every member, reps launches per candidate on
+ * `stream`, synchronously) for every tile k in {1, 2} x points per lane in
+ * {1, 2, 4} x chunk_len in {automatic, 1024, 4096, 8192} and keep the
+ * fastest (tsl_batch_set_tile); reports it through the non-NULL out
+ * arguments.  Leaves every member as one update would (weights
+ * normalised, particles and counts unchanged).  No counterpart in the
+ * reference (one filter per process, fixed launch shape,
+ * src/cuda/cuda_evaluator.cu:301). */
+int tsl_batch_autotune(tsl_batch* b, void* stream, int reps, int* best_k, int* best_points_per_lane,
+                       uint32_t* best_chunk, float* best_ms);
+
 /* Rebuild the members' device descriptors and reserve their buffers. */
 int tsl_batch_prepare(tsl_batch* b);
 

@@ -96,6 +96,40 @@ def test_batch_member_equals_alone(tsl, room, k, pp):
         assert np.array_equal(got, weighted[i][idx])
 
 
+def test_batch_autotune_keeps_results(tsl, room):
+    """The batch tile autotuner returns one of its candidates, leaves the
+    particles untouched (weights as one update gives them), and the update
+    with the tile it picked equals each member updated alone with that tile,
+    bit for bit."""
+    ev = tsl.TSDFEvaluator(room.map)
+    mem = _members(room, SIZES, seed=13)
+    filters = []
+    for parts, pts in mem:
+        f = tsl.DeviceFilter(ev, parts.shape[0])
+        f.set_particles(parts)
+        f.set_scan(pts, room.tf)
+        filters.append(f)
+    b = tsl.Batch(ev, filters)
+    s = tsl.Stream()
+    k, pp, chunk, ms = b.autotune(s, reps=1)
+    assert k in (1, 2) and pp in (1, 2, 4) and chunk in (0, 1024, 4096, 8192) and ms > 0.0
+    for f, (parts, _) in zip(filters, mem):
+        assert np.array_equal(f.get_particles()[:, :6], parts[:, :6])
+    b.update_async(s)
+    _, _, status = b.sync(s)
+    assert np.all(status == 0)
+    for i, (parts, pts) in enumerate(mem):
+        alone = tsl.DeviceFilter(ev, parts.shape[0])
+        alone.set_particles(parts)
+        alone.set_scan(pts, room.tf)
+        # the batch's automatic chunk: 2048 points, or the whole scan if shorter
+        alone.set_tile(k, chunk or min(2048, (pts.shape[0] + 63) // 64 * 64), pp)
+        alone.set_order(0)
+        alone.update_async(s)
+        alone.sync(s)
+        assert np.array_equal(filters[i].get_particles().view(np.uint32), alone.get_particles().view(np.uint32)), i
+
+
 def test_batch_graph_replays_match_eager(tsl, room):
     ev = tsl.TSDFEvaluator(room.map)
     mem = _members(room, SIZES, seed=9)

@@ -113,6 +113,8 @@ _SIGS = {
                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_F)]),
     "tsl_filter_set_order": (_I, [_P, _I]),
     "tsl_filter_set_block_lookup": (_I, [_P, _I]),
+    "tsl_batch_autotune": (_I, [_P, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_uint32),
+                                ctypes.POINTER(_F)]),
     "tsl_server_create": (_I, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                ctypes.POINTER(_P)]),
     "tsl_server_destroy": (_I, [_P]),

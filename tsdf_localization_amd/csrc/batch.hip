@@ -232,6 +232,66 @@ int tsl_batch_prepare(tsl_batch* b)
     return prepare(b);
 }
 
+int tsl_batch_autotune(tsl_batch* b, void* stream, int reps, int* best_k, int* best_points_per_lane,
+                       uint32_t* best_chunk, float* best_ms)
+{
+    if (!b || !b->prepared) return set_error(TSL_ERR_INVALID_ARGUMENT, "batch not prepared");
+    TSL_HIP_CHECK(hipSetDevice(b->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (reps < 1) reps = 1;
+    const int ks[2] = {1, 2};
+    const int ps[3] = {1, 2, 4};
+    const uint32_t chunks[4] = {0u, 1024u, 4096u, 8192u};
+    const int saved_k = b->K, saved_p = b->P;
+    const uint32_t saved_chunk = b->chunk_len;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TSL_HIP_CHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_error(TSL_ERR_HIP, "hipEventCreate failed");
+    }
+    float best = 1e30f;
+    int bk = saved_k, bp = saved_p;
+    uint32_t bc = saved_chunk;
+    int rc = TSL_OK;
+    for (int ki = 0; ki < 2 && rc == TSL_OK; ++ki)
+        for (int pi = 0; pi < 3 && rc == TSL_OK; ++pi)
+            for (int ci = 0; ci < 4 && rc == TSL_OK; ++ci) {
+                b->K = ks[ki];
+                b->P = ps[pi];
+                b->chunk_len = chunks[ci];
+                rc = prepare(b);
+                if (rc == TSL_OK) rc = enqueue_step(b, false, false, s);  // warm-up
+                if (rc != TSL_OK) break;
+                (void)hipEventRecord(e0, s);
+                for (int r = 0; r < reps && rc == TSL_OK; ++r) rc = enqueue_step(b, false, false, s);
+                (void)hipEventRecord(e1, s);
+                if (hipEventSynchronize(e1) != hipSuccess) rc = set_error(TSL_ERR_HIP, "autotune launch failed");
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                ms /= (float)reps;
+                if (rc == TSL_OK && ms < best) {
+                    best = ms;
+                    bk = b->K;
+                    bp = b->P;
+                    bc = b->chunk_len;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->K = rc == TSL_OK ? bk : saved_k;
+    b->P = rc == TSL_OK ? bp : saved_p;
+    b->chunk_len = rc == TSL_OK ? bc : saved_chunk;
+    const int rc2 = prepare(b);
+    if (rc != TSL_OK) return rc;
+    TSL_TRY(rc2);
+    if (best_k) *best_k = bk;
+    if (best_points_per_lane) *best_points_per_lane = bp;
+    if (best_chunk) *best_chunk = bc;
+    if (best_ms) *best_ms = best;
+    return TSL_OK;
+}
+
 int tsl_batch_restore_async(tsl_batch* b, void* stream)
 {
     if (!b || !b->prepared) return set_error(TSL_ERR_INVALID_ARGUMENT, "batch not prepared");

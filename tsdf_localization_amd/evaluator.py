@@ -472,6 +472,14 @@ class Batch:
     def set_tile(self, k=1, chunk_len=0, points_per_lane=1):
         check(lib.tsl_batch_set_tile(self.handle, int(k), int(points_per_lane), int(chunk_len)))
 
+    def autotune(self, stream=None, reps=2):
+        """Time the batched update for every tile shape and keep the fastest;
+        returns (K, points per lane, chunk_len (0 = automatic), ms per update)."""
+        k, p, c, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint32(), ctypes.c_float()
+        check(lib.tsl_batch_autotune(self.handle, _stream(stream), int(reps), ctypes.byref(k), ctypes.byref(p),
+                                     ctypes.byref(c), ctypes.byref(ms)))
+        return k.value, p.value, c.value, ms.value
+
     def prepare(self):
         check(lib.tsl_batch_prepare(self.handle))
 
