@@ -887,9 +887,10 @@ int tsl_evaluate(tsl_evaluator* ev, tsl_particle* particles, size_t P, const tsl
     TSL_TRY(tsl_filter_update_async(f, nullptr));
     int rc = tsl_filter_sync(f, nullptr, out_pose, nullptr);
     if (rc != TSL_OK) return rc;
-    std::vector<tsl_particle> back(P);
-    TSL_HIP_CHECK(hipMemcpy(back.data(), f->particles.ptr, sizeof(tsl_particle) * P, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < P; ++i) particles[i].second = back[i].second;  // cuda_evaluator.cu:405-408
+    // the normalised weights, also kept contiguous by the normalisation
+    std::vector<float> w(P);
+    TSL_HIP_CHECK(hipMemcpy(w.data(), f->raw.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < P; ++i) particles[i].second = w[i];  // cuda_evaluator.cu:405-408
     return TSL_OK;
 }
 

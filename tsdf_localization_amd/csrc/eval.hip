@@ -518,7 +518,8 @@ __device__ __forceinline__ float weight_total(const Member& m, uint32_t nb, bool
 }
 
 // Slice b: w_i = raw_i / sum written to particles[i].second (the
-// reference's particles[index].second /= weight_sum, float division), and
+// reference's particles[index].second /= weight_sum, float division) and
+// over raw_i, and
 // the slice's partials of the nine weighted pose sums
 // (tsdf_evaluator.cpp:198-217) in double into blocks[nb + q nb + b].
 template <int kG>
@@ -534,6 +535,7 @@ __device__ __forceinline__ void normalize_slice(const Member& m, uint32_t P, uin
         tsl_particle* particles = m.particles;
         const float w = m.raw[i] / sum;
         particles[i].second = w;
+        m.raw[i] = w;  // a contiguous copy of the normalised weights for host readers
         const double wd = (double)w;
         q[0] = (double)particles[i].first[0] * wd;
         q[1] = (double)particles[i].first[1] * wd;
