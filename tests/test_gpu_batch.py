@@ -56,9 +56,10 @@ def _alone(tsl, ev, parts, pts, tf, u0, k, pp):
 
 
 # includes the edges of the lone filter's one-workgroup finishing kernels
-# (4096 particles: reduction in one workgroup, 4097: not; 8000: resampling
-# in one workgroup, reduction not), which must match the member kernels
-SIZES = [(37, 300), (1000, 1500), (257, 64), (2500, 900), (64, 1), (4096, 700), (4097, 500), (8000, 200)]
+# (2048 particles: reduction in one workgroup, resampling not; 2049:
+# neither; 2040: both), which must match the member kernels
+SIZES = [(37, 300), (1000, 1500), (257, 64), (2500, 900), (64, 1), (2048, 700), (2049, 500), (2040, 200),
+         (8000, 200)]
 
 
 @pytest.mark.parametrize("k,pp", [(1, 1), (2, 1), (1, 4)])

@@ -145,9 +145,10 @@ def test_server_many_requests_per_robot_in_order(tsl, room):
 
 
 def test_server_tick_scan_preparation_paths_agree(tsl, room):
-    """Ticks of few scans sort each scan on its own; larger ticks sort all of
-    them in one segmented sort.  Both give the same points in the same order:
-    every robot run alone equals its twin run in a large tick, bit for bit."""
+    """A tick of one scan sorts it on its own; larger ticks sort all their
+    points together (by key, then by scan).  Both give the same points in the
+    same order: every robot run alone equals its twin run in a large tick,
+    bit for bit."""
     from tsdf_localization_amd.server import Server
     ev = tsl.TSDFEvaluator(room.map)
     robots = _robots(room, 12, seed=11)
@@ -165,7 +166,7 @@ def test_server_tick_scan_preparation_paths_agree(tsl, room):
         for i, (p, pts) in enumerate(robots):
             srv.submit(ids_b[i], _motion(tsl, i), pts, u0[i], request_id=100 + i)
         mix = {r["id"] - 100: r for r in srv.wait(len(robots))}
-        assert max(r["batch"] for r in mix.values()) >= 4, "no tick large enough for the segmented sort"
+        assert max(r["batch"] for r in mix.values()) >= 4, "no tick with several scans"
         for i in range(len(robots)):
             assert mix[i]["status"] == 0
             assert mix[i]["pose"] == alone[i]["pose"] and mix[i]["count"] == alone[i]["count"], i

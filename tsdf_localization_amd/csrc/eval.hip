@@ -25,7 +25,7 @@ constexpr int kBlock = 256;           // threads per workgroup (4 wavefronts)
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kXcds = 8;         // accelerator complex dies of the MI355X (one L2 each)
 constexpr uint32_t kTile = 512;       // scan points staged in LDS per step (8 KB), a multiple of 64 * 4
-constexpr uint32_t kFinishSmallSlices = 16;  // filters of up to 16 * kBlock particles finish in one workgroup
+constexpr uint32_t kFinishSmallSlices = 8;  // filters of up to 8 * kBlock particles finish in one workgroup
 
 __device__ __forceinline__ float wave_sum(float v)
 {

@@ -34,7 +34,7 @@ namespace tsl {
 namespace {
 
 constexpr int kRsThreads = 1024;
-constexpr uint32_t kRsSmall = 8192;  // a lone filter with at most this many candidates resamples in one launch
+constexpr uint32_t kRsSmall = 2048;  // a lone filter with at most this many candidates resamples in one launch
 
 __host__ __device__ __forceinline__ float rec_step(float u, double inv_m) { return (float)((double)u + inv_m); }
 

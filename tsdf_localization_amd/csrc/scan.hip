@@ -105,14 +105,40 @@ __global__ void k_bucket_zero(uint32_t* __restrict__ hist)
     if (i < kBuckets) hist[i] = 0u;
 }
 
+// hist[key] += the number of active lanes with that key, one atomic per
+// distinct key of the wavefront (the particles of a tracking filter share a
+// few buckets, and per-lane atomics on one counter serialise); returns the
+// counter's old value plus the lane's rank among the lanes with its key.
+// Every lane of the wavefront calls it.
+__device__ __forceinline__ uint32_t wave_bucket_add(uint32_t* __restrict__ hist, uint32_t key, bool active)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(active);
+    uint32_t pos = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lk = __shfl(key, leader, 64);
+        const uint64_t same = __ballot(active && key == lk);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&hist[lk], (uint32_t)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if (active && key == lk) pos = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return pos;
+}
+
 __global__ void k_bucket_count(const tsl_particle* __restrict__ in, const uint32_t* __restrict__ count,
                                BucketGrid g, uint32_t* __restrict__ hist, uint32_t* __restrict__ bucket)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *count) return;
-    const uint32_t k = bucket_of(in[i], g);
-    bucket[i] = k;
-    atomicAdd(&hist[k], 1u);
+    const bool active = i < *count;
+    uint32_t k = 0;
+    if (active) {
+        k = bucket_of(in[i], g);
+        bucket[i] = k;
+    }
+    (void)wave_bucket_add(hist, k, active);
 }
 
 // One workgroup: hist becomes the exclusive prefix sum of the counts.
@@ -150,8 +176,10 @@ __global__ void k_bucket_scatter(const uint32_t* __restrict__ count, const uint3
                                  uint32_t* __restrict__ hist, uint32_t* __restrict__ order)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *count) return;
-    order[atomicAdd(&hist[bucket[i]], 1u)] = i;
+    const bool active = i < *count;
+    const uint32_t k = active ? bucket[i] : 0u;
+    const uint32_t pos = wave_bucket_add(hist, k, active);
+    if (active) order[pos] = i;
 }
 
 // ---- scan reduction (TSDFEvaluator::evaluateParticles,
@@ -425,27 +453,55 @@ __global__ void k_prep_sorted_seg(const tsl_point* __restrict__ in, const uint32
 
 size_t prep_points_batch_temp(uint32_t total, uint32_t n)
 {
-    size_t temp = 0;
+    size_t t1 = 0, t2 = 0;
     if (total == 0 || n == 0) return 0;
-    if (hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, temp, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                                    (uint32_t*)nullptr, (uint32_t*)nullptr, (int)total, (int)n,
-                                                    (const uint32_t*)nullptr, (const uint32_t*)nullptr, 0, 63,
-                                                    (hipStream_t)0) != hipSuccess)
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, t1, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr,
+                                           (uint32_t*)nullptr, (int)total, 0, 63, (hipStream_t)0) != hipSuccess ||
+        hipcub::DeviceRadixSort::SortPairs(nullptr, t2, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (uint32_t*)nullptr, (int)total, 0, 32, (hipStream_t)0) != hipSuccess)
         return 0;
-    return temp;
+    return std::max(t1, t2);
 }
 
+// Sorted position t: the scan (segment of off) that point order[t] belongs
+// to (the last r with off[r] <= order[t]; empty scans own no point).
+__global__ void k_seg_keys(const uint32_t* __restrict__ off, uint32_t n, const uint32_t* __restrict__ order,
+                           uint32_t total, uint32_t* __restrict__ seg)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const uint32_t idx = order[t];
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= idx) lo = mid; else hi = mid;
+    }
+    seg[t] = lo;
+}
+
+// Every scan of a tick prepared at once: one device-wide stable sort of all
+// points by Morton key, then a stable sort by scan.  Within a scan the order
+// is by key, ties in point order: the order the per-scan sort gives.
 int launch_prep_points_batch(const tsl_evaluator* ev, const tsl_point* pts, const uint32_t* off, const ScanDest* dst,
                              uint32_t n, uint32_t total, uint32_t max_count, uint64_t* keys, uint32_t* vals,
                              void* temp, size_t temp_bytes, hipStream_t stream)
 {
     if (n == 0 || total == 0) return TSL_OK;
-    hipLaunchKernelGGL(k_morton, dim3((total + 255) / 256), dim3(256), 0, stream, pts, total, keys, vals);
+    const unsigned g = (total + 255) / 256;
+    hipLaunchKernelGGL(k_morton, dim3(g), dim3(256), 0, stream, pts, total, keys, vals);
     TSL_HIP_CHECK(hipGetLastError());
-    TSL_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortPairs(temp, temp_bytes, keys, keys + total, vals, vals + total,
-                                                              (int)total, (int)n, off, off + 1, 0, 63, stream));
-    hipLaunchKernelGGL(k_prep_sorted_seg, dim3((max_count + 255) / 256, n), dim3(256), 0, stream, pts, off,
-                       vals + total, ev->range_term, ev->a_max, ev->max_range_squared, dst);
+    TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys + total, vals, vals + total,
+                                                     (int)total, 0, 63, stream));
+    // the keys' first half is free again: scan numbers, in and out
+    uint32_t* seg = reinterpret_cast<uint32_t*>(keys);
+    hipLaunchKernelGGL(k_seg_keys, dim3(g), dim3(256), 0, stream, off, n, vals + total, total, seg);
+    TSL_HIP_CHECK(hipGetLastError());
+    int bits = 1;
+    while ((1u << bits) < n) ++bits;
+    TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, seg, seg + total, vals + total, vals,
+                                                     (int)total, 0, bits, stream));
+    hipLaunchKernelGGL(k_prep_sorted_seg, dim3((max_count + 255) / 256, n), dim3(256), 0, stream, pts, off, vals,
+                       ev->range_term, ev->a_max, ev->max_range_squared, dst);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
 }

@@ -3,7 +3,7 @@
 // a worker thread keeps the GPU busy with continuous batching -- every
 // "tick" takes the pending requests of robots that are not in flight (at
 // most max_batch, oldest first), enqueues motion update, scan preparation
-// (one upload and one segmented sort for all scans), batched sensor update
+// (one upload and one key-then-scan sort of all scans), batched sensor update
 // and resampling for all of them, and completes
 // earlier ticks as their events fire, so requests that arrive while others
 // run are admitted into the next tick at once.  Robots are admitted against
@@ -27,7 +27,7 @@ using namespace tsl;
 namespace {
 
 constexpr int kMaxInflight = 3;
-constexpr uint32_t kSegmentedScans = 4;  // ticks with this many scans prepare them with one segmented sort
+constexpr uint32_t kSegmentedScans = 2;  // ticks with this many scans prepare them together (two device-wide sorts)
 
 struct Robot {
     tsl_filter* f = nullptr;
@@ -153,7 +153,7 @@ int slot_reserve_scans(tsl_server* s, Slot& sl, size_t total)
     TSL_TRY(sl.scan_keys.reserve(sizeof(uint64_t) * 2 * cap));
     TSL_TRY(sl.scan_vals.reserve(sizeof(uint32_t) * 2 * cap));
     const size_t temp = prep_points_batch_temp((uint32_t)cap, s->max_batch);
-    if (temp == 0) return set_error(TSL_ERR_HIP, "segmented sort temporary size");
+    if (temp == 0) return set_error(TSL_ERR_HIP, "scan sort temporary size");
     TSL_TRY(sl.scan_temp.reserve(temp));
     sl.scan_cap = cap;
     return TSL_OK;
@@ -166,7 +166,7 @@ int dispatch(tsl_server* s, Slot* sl)
     const uint32_t n = (uint32_t)sl->reqs.size();
     uint64_t max_items = 0;
     uint32_t max_cap = 0;
-    // every scan of the tick: one upload, one segmented sort
+    // every scan of the tick: one upload, one preparation
     uint32_t total = 0, max_np = 0;
     for (uint32_t i = 0; i < n; ++i) {
         sl->off_host[i] = total;
@@ -185,7 +185,7 @@ int dispatch(tsl_server* s, Slot* sl)
         sl->dst_host[i] = ScanDest{f->points.as<float4>(), f->perm.as<uint32_t>()};
     }
     if (n >= kSegmentedScans) {
-        // one upload and one segmented sort (a workgroup per scan)
+        // one upload; all points sorted by key, then by scan (both stable)
         if (total)
             TSL_HIP_CHECK(hipMemcpyAsync(sl->scan_pts.ptr, sl->scan_host, sizeof(tsl_point) * total,
                                          hipMemcpyHostToDevice, st));
