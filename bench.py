@@ -521,7 +521,8 @@ def main():
                    "tile": {"particle_order": {0: "caller", 1: "spatial", -1: "auto"}[order],
                             "particles_per_wavefront": k, "points_per_lane": pp, "chunk_points": chunk or "auto"}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if w.name == "building" else None,
                      "kernel": "k_eval", "kernel_avg_ms": kern_avg_ms,
                      "bytes_per_eval": BYTES_PER_EVAL, "evals_per_launch": pairs // max(launches, 1)},
         "valu": valu_issue(kern_avg_ms) if w.name == "building" else None,

@@ -127,6 +127,23 @@ int tsl_map_create_tsdf(const tsl_map_coef* coef, const int32_t* grid_occ,
 int tsl_map_set_data(tsl_map_coef* coef, const float* cell_xyz, const float* cell_value,
                      size_t num_cells, int32_t* grid_occ, float* data, size_t data_capacity);
 
+/* The cell list createTSDFMap builds from a map file's chunks
+ * (include/tsdf_localization/map/map_util.h:17-154), for tsl_map_set_data.
+ * chunk_pos holds (x, y, z) chunk coordinates per chunk (the /map dataset
+ * name "x_y_z"), chunk_raw CHUNK_SIZE^3 = 262144 raw TSDFValue words per chunk
+ * (x-major; low 16 bits the TSDF value in mm, high 16 bits the weight), in
+ * the order the reference lists them (the group's name order).  coef receives
+ * the map geometry (bounding box of the chunks and the origin, resolution
+ * 0.064 m, init value from sigma).  A voxel with weight != 0 and |tsdf| < 600
+ * mm becomes a cell (cell_xyz, cell_value = its likelihood); weight != 0 and
+ * |tsdf| >= 600 mm a free-space point (free_xyz, the reference's free_map).
+ * Sizes are reported through num_cells / num_free; TSL_ERR_CAPACITY if a
+ * buffer is too small, so a first call with zero capacities sizes them.
+ * The HDF5 read itself is the caller's.  Host only. */
+int tsl_map_chunks_to_cells(const int32_t* chunk_pos, const uint32_t* chunk_raw, size_t num_chunks, float sigma,
+                            tsl_map_coef* coef, float* cell_xyz, float* cell_value, size_t cell_capacity,
+                            size_t* num_cells, float* free_xyz, size_t free_capacity, size_t* num_free);
+
 /* Likelihood value of a valid TSDF cell and the map's init value
  * (map_util.h:68-73,124-126).  Host only. */
 float tsl_likelihood_value(int16_t tsdf_mm, float sigma);

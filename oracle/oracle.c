@@ -13,6 +13,7 @@
  *   - CudaSubVoxelMap ctor / getIndex / getEntry / setData
  *                                              include/tsdf_localization/cuda/cuda_sub_voxel_map.tcc:5-231
  *   - likelihood value of a TSDF cell          include/tsdf_localization/map/map_util.h:68-73,116-129
+ *   - map chunks -> cells and free points      include/tsdf_localization/map/map_util.h:17-154
  *   - SystematicResampler::resample            include/tsdf_localization/resampling/novel_resampling.h:38-73
  *
  * Conventions binding on every function (the reference is built by gcc for
@@ -317,6 +318,66 @@ ORC_EXPORT float orc_likelihood_value(int16_t tsdf_mm, float sigma)
     value = (double)(e / d);
     value = value * value * value;
     return (float)value;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Map chunks -> cells and free points (map_util.h:17-154)                  */
+/* ------------------------------------------------------------------------ */
+/* createTSDFMap after the file read: chunk k sits at chunk coordinates
+ * pos[3k..3k+2] and holds 64^3 raw TSDFValue words w[v] at flat index
+ * v = 4096 i + 64 j + l (voxel x = 64 cx + i, y = 64 cy + j, z = 64 cz + l).
+ * A word is {int16 value (low half), int16 weight (high half)} (tsdf.h:11-45).
+ * Bounds (:23-64): per axis the min / max of 0 and every chunk coordinate, as
+ * floats, then min * 64 * 64 * 0.001 and (max * 64 + 64) * 64 * 0.001 (float
+ * products, the last one in double).  Cell position (:114-116):
+ * (float)x * 64 (float) * 0.001 (double), stored as float.  Cells keep the
+ * chunk order given and x, y, z nesting; the outputs must hold 64^3 entries
+ * per chunk.  Returns the counts through *nc / *nf.
+ * Parity: pinned by hand-computed bounds and positions, a crafted chunk for
+ * the value / weight split and the truncation edge, and a single-voxel chunk
+ * for the x-major layout (tests/test_map_chunks.py). */
+ORC_EXPORT void orc_chunks_to_cells(const int32_t* pos, const uint32_t* words, size_t num_chunks, float sigma,
+                                    orc_coef* c, float* cell_xyz, float* cell_value, size_t* nc,
+                                    float* free_xyz, size_t* nf)
+{
+    float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+    for (size_t k = 0; k < num_chunks; ++k)
+        for (int a = 0; a < 3; ++a) {
+            float v = (float)pos[3 * k + a];
+            lo[a] = v < lo[a] ? v : lo[a];
+            hi[a] = v > hi[a] ? v : hi[a];
+        }
+    for (int a = 0; a < 3; ++a) {
+        float l = lo[a] * 64.0f;
+        l = l * 64.0f;
+        lo[a] = (float)((double)l * 0.001);
+        float h = hi[a] * 64.0f + 64.0f;
+        h = h * 64.0f;
+        hi[a] = (float)((double)h * 0.001);
+    }
+    orc_coef_init(lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], (float)(64 * 0.001), orc_likelihood_init(sigma), c);
+    size_t n_cell = 0, n_free = 0;
+    for (size_t k = 0; k < num_chunks; ++k) {
+        for (size_t v = 0; v < 64 * 64 * 64; ++v) {
+            uint32_t w = words[k * 64 * 64 * 64 + v];
+            int16_t value = (int16_t)(w & 0xffffu), weight = (int16_t)(w >> 16);
+            if (weight == 0)
+                continue;
+            int xyz[3] = {64 * pos[3 * k] + (int)(v >> 12), 64 * pos[3 * k + 1] + (int)((v >> 6) & 63),
+                          64 * pos[3 * k + 2] + (int)(v & 63)};
+            float* dst;
+            if (fabsf((float)value) < 600.0f) {
+                dst = cell_xyz + 3 * n_cell;
+                cell_value[n_cell++] = orc_likelihood_value(value, sigma);
+            } else {
+                dst = free_xyz + 3 * n_free++;
+            }
+            for (int a = 0; a < 3; ++a)
+                dst[a] = (float)((double)((float)xyz[a] * 64.0f) * 0.001);
+        }
+    }
+    *nc = n_cell;
+    *nf = n_free;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -50,6 +50,8 @@ _sigs = {
     "orc_free": (None, [_P]),
     "orc_likelihood_init": (_F, [_F]),
     "orc_likelihood_value": (_F, [ctypes.c_int16, _F]),
+    "orc_chunks_to_cells": (None, [_P, _P, _S, _F, ctypes.POINTER(OrcCoef), _P, _P, ctypes.POINTER(_S), _P,
+                                   ctypes.POINTER(_S)]),
     "orc_pose_matrix": (None, [_P, _P, ctypes.c_int, _P]),
     "orc_evaluate": (None, [ctypes.POINTER(OrcCoef), _P, _P, _P, _S, _P, _S, _P, ctypes.POINTER(OrcParams),
                             ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]),
@@ -160,6 +162,21 @@ class OracleMap:
         olib.orc_trace_indices(ctypes.byref(self.coef), _ptr(self.grid), _ptr(p), p.shape[0], _ptr(pts),
                                pts.shape[0], _ptr(tf), trig_mode, _ptr(out))
         return out
+
+
+def chunks_to_cells(positions, raw, sigma=0.1):
+    """createTSDFMap's cells: (coef, cells (C,3), values (C,), free (F,3))."""
+    pos = np.ascontiguousarray(positions, np.int32).reshape(-1, 3)
+    words = np.ascontiguousarray(raw, np.uint32).reshape(pos.shape[0], 64 ** 3)
+    cap = words.size
+    cells = np.empty((cap, 3), np.float32)
+    vals = np.empty(cap, np.float32)
+    free = np.empty((cap, 3), np.float32)
+    c = OrcCoef()
+    nc, nf = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    olib.orc_chunks_to_cells(_ptr(pos), _ptr(words), pos.shape[0], float(sigma), ctypes.byref(c), _ptr(cells),
+                             _ptr(vals), ctypes.byref(nc), _ptr(free), ctypes.byref(nf))
+    return c, cells[: nc.value].copy(), vals[: nc.value].copy(), free[: nf.value].copy()
 
 
 def set_data(min_xyz, max_xyz, res, init, cells_xyz, values):

@@ -84,6 +84,8 @@ _SIGS = {
     "tsl_map_create": (_I, [ctypes.POINTER(MapCoef), _P, _P, ctypes.POINTER(_P)]),
     "tsl_map_create_tsdf": (_I, [ctypes.POINTER(MapCoef), _P, _P, _I, _F, ctypes.POINTER(_P)]),
     "tsl_map_set_data": (_I, [ctypes.POINTER(MapCoef), _P, _P, _S, _P, _P, _S]),
+    "tsl_map_chunks_to_cells": (_I, [_P, _P, _S, _F, ctypes.POINTER(MapCoef), _P, _P, _S, ctypes.POINTER(_S),
+                                     _P, _S, ctypes.POINTER(_S)]),
     "tsl_likelihood_value": (_F, [ctypes.c_int16, _F]),
     "tsl_likelihood_init": (_F, [_F]),
     "tsl_map_device_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_uint64)]),
