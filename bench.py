@@ -351,12 +351,157 @@ def bench_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_mixed(args):
+    """Four building maps served side by side on one GPU (one server per
+    map), requests from a Poisson process split 9:1 between tracking robots
+    (5,000 particles x 10,000 points) and global-localisation robots
+    (1,000,000 particles x 4,000 points): p50/p99 latency per class under
+    that load, the saturation throughput of the served form (every request
+    submitted at once) and plain per-request launches of the same requests."""
+    import tsdf_localization_amd as tsl
+    from tsdf_localization_amd import synth
+    from tsdf_localization_amd.server import Server
+
+    t0 = time.time()
+    w = synth.mixed()
+    M = len(w.maps)
+    T = len(w.tracking[0])
+    log(f"workload mixed: {M} maps of {w.meta['map_bytes'][0] / 1e6:.0f} MB, {T} tracking + 1 global robot per map "
+        f"(generated in {time.time() - t0:.1f}s)")
+    evs = [tsl.TSDFEvaluator(m) for m in w.maps]
+    rng = np.random.default_rng(5005)
+    n_req = args.mixed_requests
+    rate = args.mixed_rate
+    is_global = rng.random(n_req) < 0.1
+    map_of = rng.integers(0, M, n_req)
+    trk_of = rng.integers(0, T, n_req)
+    robot = [w.global_robots[map_of[i]] if is_global[i] else w.tracking[map_of[i]][trk_of[i]] for i in range(n_req)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    motions = [tsl.evaluator.motion_from_delta(0.05, 0.0, 0.0, 0.0, 0.0, 0.01, 0.01, 0.005, seed=i)
+               for i in range(n_req)]
+    u0 = [np.float32(rng.uniform(0, 0.9 / robot[i].particles.shape[0])) for i in range(n_req)]
+    evals = [robot[i].particles.shape[0] * robot[i].points.shape[0] for i in range(n_req)]
+    gp = w.global_robots[0].particles.shape[0]
+
+    def fresh_servers():
+        srvs, ids = [], []
+        for k in range(M):
+            srv = Server(evs[k], hbm_budget=args.hbm_budget_gb * (1 << 30), max_robots=T + 1, max_particles=gp,
+                         max_batch=8, chunk_len=2048)
+            trk = [srv.add_robot(r.particles, r.points.shape[0], w.tf) for r in w.tracking[k]]
+            glb = srv.add_robot(w.global_robots[k].particles, w.global_robots[k].points.shape[0], w.tf)
+            srvs.append(srv)
+            ids.append((trk, glb))
+        return srvs, ids
+
+    def submit(srvs, ids, i):
+        k = map_of[i]
+        rid = ids[k][1] if is_global[i] else ids[k][0][trk_of[i]]
+        srvs[k].submit(rid, motions[i], robot[i].points, u0[i], request_id=i)
+
+    def collect(srvs):
+        out = []
+        for k in range(M):
+            want = int(np.sum(map_of == k))
+            out.extend(srvs[k].wait(want, timeout_ms=600_000))
+        return out
+
+    # 1. latency under Poisson arrivals
+    srvs, ids = fresh_servers()
+    used = sum(s.stats()["used_bytes"] for s in srvs)
+    log(f"servers: {used / 2**30:.2f} GiB resident for {M * (T + 1)} robots")
+    for k in range(M):  # warm-up: one request of each class per map
+        srvs[k].submit(ids[k][0][0], motions[0], w.tracking[k][0].points, np.float32(0.1 / 5000),
+                       request_id=n_req + 2 * k)
+        srvs[k].submit(ids[k][1], motions[0], w.global_robots[k].points, np.float32(0.1 / gp),
+                       request_id=n_req + 2 * k + 1)
+        srvs[k].wait(2, timeout_ms=600_000)
+    start = time.perf_counter()
+    for i in range(n_req):
+        delay = arrivals[i] - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        submit(srvs, ids, i)
+    res = collect(srvs)
+    wall = time.perf_counter() - start
+    bad = sum(r["status"] != 0 for r in res)
+    classes = {}
+    for name, flag in (("tracking", False), ("global", True)):
+        lat = np.array([r["done_ms"] - r["submit_ms"] for r in res if bool(is_global[r["id"]]) == flag])
+        classes[name] = {"requests": int(lat.size), "p50": float(np.percentile(lat, 50)),
+                         "p99": float(np.percentile(lat, 99))}
+        log(f"poisson {rate}/s, {name}: {lat.size} requests, latency p50 {classes[name]['p50']:.2f} ms "
+            f"p99 {classes[name]['p99']:.2f} ms")
+    log(f"{n_req} requests in {wall:.2f} s, failed {bad}")
+    for s in srvs:
+        s.close()
+
+    # 2. saturation: every request at once
+    srvs, ids = fresh_servers()
+    t1 = time.perf_counter()
+    for i in range(n_req):
+        submit(srvs, ids, i)
+    collect(srvs)
+    served_s = time.perf_counter() - t1
+    served_rate = sum(evals) / served_s
+    log(f"served saturation: {served_s * 1e3:.1f} ms for {n_req} requests, {served_rate:.3e} evals/s")
+    for s in srvs:
+        s.close()
+
+    # 3. plain per-request launches, back to back
+    stream = tsl.Stream()
+    filters = {}
+    for i in range(n_req):
+        key = (int(map_of[i]), -1 if is_global[i] else int(trk_of[i]))
+        if key not in filters:
+            f = tsl.DeviceFilter(evs[key[0]], robot[i].particles.shape[0])
+            f.set_particles(robot[i].particles)
+            filters[key] = f
+    t2 = time.perf_counter()
+    for i in range(n_req):
+        f = filters[(int(map_of[i]), -1 if is_global[i] else int(trk_of[i]))]
+        f.motion_async(motions[i], stream)
+        f.set_scan(robot[i].points, w.tf)
+        f.update_async(stream)
+        f.resample_async(u0[i], stream)
+        f.sync(stream)
+    plain_s = time.perf_counter() - t2
+    plain_rate = sum(evals) / plain_s
+    log(f"plain per-request: {plain_s * 1e3:.1f} ms, {plain_rate:.3e} evals/s; served/plain {served_rate / plain_rate:.2f}x")
+    out = {
+        "metric": "particle*point evals/s",
+        "value": served_rate,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": n_req,
+        "warmup": 2 * M,
+        "ms_per_step": served_s * 1e3 / n_req,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (four seeded building maps, ray-cast scans, particles around each tracking robot and "
+                "over the free space for each global robot)",
+        "config": {"workload": "mixed", "maps": M, "map_bytes_each": w.meta["map_bytes"][0],
+                   "map": "sparse 1024x1024x128 @0.05 m int16 each",
+                   "classes": {"tracking": "5000 particles x 10000 points", "global": "1000000 x 4000"},
+                   "class_ratio": "9:1", "requests": n_req, "arrival_rate_per_s": rate},
+        "latency_ms_by_class": classes,
+        "failed": int(bad),
+        "plain_value": plain_rate,
+        "speedup_vs_plain": served_rate / plain_rate,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="building", choices=["building", "room", "batched", "serving"])
+    ap.add_argument("--workload", default="building", choices=["building", "room", "batched", "serving", "mixed"])
+    ap.add_argument("--mixed-requests", type=int, default=400, help="mixed: number of requests")
+    ap.add_argument("--mixed-rate", type=float, default=50.0, help="mixed: Poisson arrival rate (requests/s)")
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--extra-rates", type=float, nargs="*", default=[1000.0, 2000.0],
@@ -392,6 +537,9 @@ def main():
         return
     if args.workload == "serving":
         bench_serving(args)
+        return
+    if args.workload == "mixed":
+        bench_mixed(args)
         return
 
     import tsdf_localization_amd as tsl

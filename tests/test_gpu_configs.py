@@ -1,9 +1,10 @@
 """The benchmark workloads as parity cases, built by the same seeded
 generators bench.py uses: global localisation (100,000 particles x 20,000
 points on the sparse building map) checked against the oracle at full size,
-and the batched-filters and serving workloads checked on the members and
-robots the oracle can recompute in seconds, plus the bit-equality of each
-member or robot with the same filter updated alone.
+and the batched-filters, serving and mixed (four maps, tracking and
+global-localisation robots) workloads checked on members and robots against
+the oracle, plus the bit-equality of each member or robot with the same
+filter updated alone.
 
 Bar: voxel indices and resampled indices bit-exact; weights within 1e-5
 relative of the oracle's float64 sum."""
@@ -154,3 +155,53 @@ def test_serving_robots(tsl):
                 assert np.max(rel_err(weighted[:, 6], w64 / w64.sum())) <= WEIGHT_RTOL
     finally:
         srv.close()
+
+
+def test_mixed_classes(tsl):
+    """The mixed workload (four maps, one server each, tracking and
+    global-localisation robots): one request of each class on two maps
+    through the servers, each result equal to the same robot's plain filter
+    update bit for bit; the global robot's weights (1,000,000 x 4,000 pairs)
+    and a tracking robot's against the oracle."""
+    from tsdf_localization_amd import synth
+    from tsdf_localization_amd.server import Server
+    w = synth.mixed()
+    s = tsl.Stream()
+    for k in (0, 3):
+        ev = tsl.TSDFEvaluator(w.maps[k])
+        trk, glb = w.tracking[k][1], w.global_robots[k]
+        gp = glb.particles.shape[0]
+        srv = Server(ev, max_robots=len(w.tracking[k]) + 1, max_particles=gp, max_batch=8, chunk_len=2048)
+        try:
+            ids = [srv.add_robot(r.particles, r.points.shape[0], w.tf) for r in (trk, glb)]
+            motions = [tsl.evaluator.motion_from_delta(0.05, 0.0, 0.0, 0.0, 0.0, 0.01, 0.01, 0.005, seed=90 + j)
+                       for j in range(2)]
+            u0 = [np.float32(0.43 / r.particles.shape[0]) for r in (trk, glb)]
+            for j, r in enumerate((trk, glb)):
+                srv.submit(ids[j], motions[j], r.points, u0[j], request_id=j)
+            res = {r["id"]: r for r in srv.wait(2)}
+            for j, r in enumerate((trk, glb)):
+                assert res[j]["status"] == 0
+                f = tsl.DeviceFilter(ev, r.particles.shape[0])
+                f.set_particles(r.particles)
+                f.set_tile(2, 2048, 1)
+                f.set_order(0)
+                f.motion_async(motions[j], s)
+                s.synchronize()
+                moved = f.get_particles()
+                f.set_scan(r.points, w.tf)
+                f.update_async(s)
+                pose, _ = f.sync(s)
+                weighted = f.get_particles()
+                f.resample_async(u0[j], s)
+                f.sync(s)
+                assert res[j]["pose"] == pose, (k, j)
+                assert np.array_equal(srv.get_particles(ids[j]).view(np.uint32), f.get_particles().view(np.uint32))
+                if k == 0:
+                    print(f"mixed map {k} robot class {j}: oracle {r.particles.shape[0]} x {r.points.shape[0]}",
+                          flush=True)
+                    om = ob.OracleMap.from_host_map(w.maps[k])
+                    _, w64 = om.evaluate(moved, r.points, w.tf)
+                    assert np.max(rel_err(weighted[:, 6], w64 / w64.sum())) <= WEIGHT_RTOL, (k, j)
+        finally:
+            srv.close()

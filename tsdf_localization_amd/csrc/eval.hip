@@ -408,6 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
 __global__ void k_pose_m(const Member* __restrict__ members)
 {
     const Member& m = members[blockIdx.y];
+    if (m.inv) return;  // pose matrices already written in the member's spatial order
     const uint32_t n = m.st->count;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -901,6 +902,13 @@ static int launch_pose_for_tile(tsl_filter* f, const Tile& tile, hipStream_t str
     TSL_HIP_CHECK(hipGetLastError());
     f->last_order = tile.order;
     return TSL_OK;
+}
+
+int launch_pose_spatial(tsl_filter* f, hipStream_t stream)
+{
+    Tile t{};
+    t.order = 1;
+    return launch_pose_for_tile(f, t, stream);
 }
 
 static int launch_eval_core(tsl_filter* f, hipStream_t stream, Tile& tile)

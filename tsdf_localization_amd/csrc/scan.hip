@@ -354,13 +354,19 @@ int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring,
     return TSL_OK;
 }
 
+int reserve_particle_order(tsl_filter* f)
+{
+    const size_t n = f->count_bound;
+    TSL_TRY(f->pkeys.reserve(sizeof(uint32_t) * ((size_t)kBuckets + n)));
+    TSL_TRY(f->porder.reserve(sizeof(uint32_t) * n));
+    return f->pinv.reserve(sizeof(uint32_t) * n);
+}
+
 int launch_particle_order(tsl_filter* f, hipStream_t stream)
 {
     const uint32_t n = (uint32_t)f->count_bound;
     if (n == 0) return TSL_OK;
-    TSL_TRY(f->pkeys.reserve(sizeof(uint32_t) * ((size_t)kBuckets + n)));
-    TSL_TRY(f->porder.reserve(sizeof(uint32_t) * n));
-    TSL_TRY(f->pinv.reserve(sizeof(uint32_t) * n));
+    TSL_TRY(reserve_particle_order(f));
     uint32_t* hist = f->pkeys.as<uint32_t>();
     uint32_t* bucket = hist + kBuckets;
     const uint32_t* count = &f->state.as<FilterState>()->count;

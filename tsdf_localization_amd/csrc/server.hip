@@ -28,6 +28,11 @@ namespace {
 
 constexpr int kMaxInflight = 3;
 constexpr uint32_t kSegmentedScans = 2;  // ticks with this many scans prepare them together (two device-wide sorts)
+// Robots with at least this many particle slots are evaluated in spatial
+// particle order (a global-localisation set spread over the whole map: the
+// order keeps one XCD's particles in the same part of the map); smaller sets
+// stay in their own order.  The order changes no result bit.
+constexpr size_t kSpatialOrderMin = 65536;
 
 struct Robot {
     tsl_filter* f = nullptr;
@@ -230,6 +235,7 @@ int dispatch(tsl_server* s, Slot* sl)
         h.ubuf = f->ubuf.as<float>();
         h.plan = f->plan.as<RsPlan>();
         h.u0_dev = sl->u0.as<float>() + i;
+        if (f->capacity >= kSpatialOrderMin) h.inv = f->pinv.as<uint32_t>();
         std::memcpy(h.tf, f->tf, sizeof(h.tf));
         h.N = (uint32_t)np;
         h.chunk_len = chunk;
@@ -262,6 +268,14 @@ int dispatch(tsl_server* s, Slot* sl)
     grid = std::max<uint64_t>(8, (grid + 7) / 8 * 8);
     l.grid = (uint32_t)grid;
     TSL_TRY(launch_motion_members(l.members, sl->motions.as<tsl_motion>(), n, max_cap, st));
+    for (uint32_t i = 0; i < n; ++i) {
+        tsl_filter* f = s->robots[sl->robots[i]].f;
+        if (f->capacity < kSpatialOrderMin) continue;
+        // the device count is at most the capacity; the order's buffers
+        // were reserved for it at admission
+        f->count_bound = f->capacity;
+        TSL_TRY(launch_pose_spatial(f, st));
+    }
     TSL_TRY(launch_batch_update(l, st));
     TSL_TRY(launch_resample_members(l.members, n, max_cap, 0.0f, true, st));
     hipLaunchKernelGGL(k_gather_states_s, dim3((n + 63) / 64), dim3(64), 0, st, l.members, n,
@@ -370,7 +384,7 @@ uint64_t filter_bytes(const tsl_filter* f)
 {
     const DevBuf* bufs[] = {&f->particles, &f->particles_alt, &f->L, &f->raw, &f->partial, &f->points, &f->points_in,
                             &f->perm, &f->sort_keys, &f->sort_vals, &f->sort_temp, &f->blocks, &f->state, &f->cumsum,
-                            &f->ubuf, &f->plan, &f->member};
+                            &f->ubuf, &f->plan, &f->member, &f->pkeys, &f->porder, &f->pinv};
     uint64_t b = 0;
     for (const DevBuf* d : bufs) b += d->bytes;
     return b;
@@ -491,6 +505,10 @@ int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n,
     // the reduction grid of a tick spans its largest member
     const uint64_t nb_max = (resample_bound(s->max_particles) + 255) / 256;
     if (rc == TSL_OK) rc = f->blocks.reserve(sizeof(double) * kReducePartials * nb_max);
+    if (rc == TSL_OK && f->capacity >= kSpatialOrderMin) {
+        f->count_bound = f->capacity;
+        rc = reserve_particle_order(f);
+    }
     if (rc == TSL_OK) std::memcpy(f->tf, tf, sizeof(float) * 16);
     if (rc != TSL_OK) {
         tsl_filter_destroy(f);

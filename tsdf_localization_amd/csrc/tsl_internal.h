@@ -356,6 +356,12 @@ struct Tile {
 int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const float tf[16],
                          float* d_out, hipStream_t stream);
 int launch_prep_points(tsl_filter* f, hipStream_t stream);
+// Buffers of the spatial particle order for f->count_bound particles.
+int reserve_particle_order(tsl_filter* f);
+// Spatial particle order of a filter and its pose matrices in that order
+// (f->L slot s = particle f->porder[s], f->pinv the inverse), for a filter
+// evaluated as a batch member whose inv points at f->pinv.
+int launch_pose_spatial(tsl_filter* f, hipStream_t stream);
 
 // Scan preparation of many scans at once (a serving tick): scan r is
 // points [off[r], off[r+1]) of pts; each is Morton-sorted within its own

@@ -334,3 +334,55 @@ def serving(seed=0x5EED0004, robots=256, particles=(2_000, 20_000), points=(5_00
                             "mean_evals_per_request": float(np.mean([r.particles.shape[0] * r.points.shape[0]
                                                                      for r in out]))})
 
+
+
+@dataclass
+class MixedWorkload:
+    name: str
+    maps: list            # CudaSubVoxelMap per map
+    tf: np.ndarray
+    tracking: list        # per map: list of Robot (particles around the robot's pose)
+    global_robots: list   # per map: one Robot with particles over the whole free space
+    meta: dict = field(default_factory=dict)
+
+
+def mixed(seed=0x5EED0005, maps=4, tracking=9, tracking_particles=5_000, tracking_points=10_000,
+          global_particles=1_000_000, global_points=4_000, beams=64, azimuths=1024):
+    """Several building maps served side by side, each with `tracking`
+    robots (particle sets of tracking_particles around their poses, scans of
+    tracking_points) and one global-localisation robot (global_particles
+    over the free space, a scan of global_points).  Maps are the
+    global-localisation building (51.2 x 51.2 x 6.4 m at 0.05 m) with a
+    different seed each."""
+    rng = np.random.default_rng(seed)
+    tf = tf_from_xyzrpy(**SCANNER_TF)
+    sx, sy = 51.2, 51.2
+    out_maps, out_tracking, out_global = [], [], []
+    for k in range(maps):
+        s = seed + 101 * k
+        w = building(seed=s, num_particles=global_particles, target_points=global_points)
+        out_maps.append(w.map)
+        out_global.append(Robot(w.particles, np.ascontiguousarray(w.points, np.float32), w.true_pose))
+        sc = Scene()
+        _s.synth_scene_build_building(sc.h, s, sx, sy, 3.0, 10.0, 4, 0.3)
+        robots = []
+        for i in range(tracking):
+            for _ in range(100):  # a free pose whose scan has enough points
+                for _ in range(1000):
+                    x, y = rng.uniform(2, sx - 2), rng.uniform(2, sy - 2)
+                    if sc.sdf([[x, y, 1.0]])[0] > 0.8:
+                        break
+                pose = np.array([x, y, 1.0, 0.0, 0.0, rng.uniform(-math.pi, math.pi)])
+                raw, rings = sc.raycast(compose_pose(pose, tf), beams, azimuths, -22.5, 22.5, 60.0, 0.01, s + i)
+                pts, rings = voxel_filter(raw, rings, 0.05, 1.0)
+                if pts.shape[0] >= tracking_points:
+                    break
+            if pts.shape[0] > tracking_points:
+                pts = pts[np.sort(rng.choice(pts.shape[0], tracking_points, replace=False))]
+            parts = particles_normal(s + 7 * i, tracking_particles, pose,
+                                     [0.2, 0.2, 0.05, math.radians(5), math.radians(1), math.radians(5)])
+            robots.append(Robot(parts, np.ascontiguousarray(pts, np.float32), pose))
+        out_tracking.append(robots)
+    return MixedWorkload("mixed", out_maps, tf, out_tracking, out_global,
+                         {"maps": maps, "tracking_per_map": tracking,
+                          "map_bytes": [int(m.dataSize()) * 2 for m in out_maps]})
