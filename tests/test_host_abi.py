@@ -160,3 +160,18 @@ def test_synthetic_workloads_are_deterministic():
     assert np.array_equal(a.map.grid_occ, b.map.grid_occ)
     assert np.array_equal(a.map.tsdf_mm, b.map.tsdf_mm)
     assert a.points.shape[1] == 3 and a.particles.shape[1] == 7
+
+
+def test_mixed_workload_shapes_and_determinism():
+    """The mixed workload's generator (one map here): one global robot with
+    its particle set over the free space and a 4,000-point scan, tracking
+    robots of 5,000 particles and 10,000-point scans, identical on a second
+    call."""
+    a, b = synth.mixed(maps=1, tracking=2), synth.mixed(maps=1, tracking=2)
+    g = a.global_robots[0]
+    assert g.particles.shape == (1_000_000, 7) and g.points.shape == (4000, 3)
+    assert [r.particles.shape for r in a.tracking[0]] == [(5000, 7)] * 2
+    assert [r.points.shape for r in a.tracking[0]] == [(10_000, 3)] * 2
+    assert np.array_equal(g.particles, b.global_robots[0].particles)
+    assert np.array_equal(a.tracking[0][1].points, b.tracking[0][1].points)
+    assert np.array_equal(a.maps[0].tsdf_mm, b.maps[0].tsdf_mm)
