@@ -17,9 +17,12 @@
 // Inside the reference tree define TSL_WITH_REFERENCE_HEADERS before
 // including this file: the classes then derive from the reference's own
 // Evaluator and Resampler and use its Particle, CudaPoint, ParticleCloud,
-// CudaSubVoxelMap and geometry_msgs types.  Without it (no ROS available),
-// minimal stand-ins with the same names, member names and layouts are
-// declared here.
+// CudaSubVoxelMap, geometry_msgs and sensor_msgs types.  Without it (no ROS
+// available), minimal stand-ins with the same names, member names, layouts
+// and virtual interfaces are declared here: Evaluator and Resampler have no
+// virtual destructor, exactly as in the reference, so owners keep deleting
+// through the concrete type (TSDFEvaluator's shared_ptr, constructed from
+// the derived pointer, does).
 #ifndef TSL_HIP_EVALUATOR_H
 #define TSL_HIP_EVALUATOR_H
 
@@ -42,6 +45,7 @@
 #include <tsdf_localization/resampling/resampler.h>
 #else
 #include <array>
+#include <cstdint>
 
 namespace geometry_msgs
 {
@@ -50,6 +54,26 @@ struct Quaternion { double x = 0, y = 0, z = 0, w = 1; };
 struct Pose { Point position; Quaternion orientation; };
 struct PoseWithCovariance { Pose pose; std::array<double, 36> covariance{}; };
 }  // namespace geometry_msgs
+
+// sensor_msgs/PointCloud2 and PointField: the fields a cloud reader uses
+namespace sensor_msgs
+{
+struct PointField {
+    enum : uint8_t { INT8 = 1, UINT8 = 2, INT16 = 3, UINT16 = 4, INT32 = 5, UINT32 = 6, FLOAT32 = 7, FLOAT64 = 8 };
+    std::string name;
+    uint32_t offset = 0;
+    uint8_t datatype = 0;
+    uint32_t count = 1;
+};
+struct PointCloud2 {
+    uint32_t height = 1, width = 0;
+    std::vector<PointField> fields;
+    bool is_bigendian = false;
+    uint32_t point_step = 0, row_step = 0;
+    std::vector<uint8_t> data;
+    bool is_dense = false;
+};
+}  // namespace sensor_msgs
 
 namespace tsdf_localization
 {
@@ -69,7 +93,12 @@ struct CudaPoint {
 class Evaluator
 {
 public:
-    virtual ~Evaluator() = default;
+    virtual geometry_msgs::PoseWithCovariance evaluate(std::vector<Particle>&, const sensor_msgs::PointCloud2&,
+                                                       float[16])
+    {
+        throw std::runtime_error("CUDA acceleration is not supported. Please install CUDA!");
+    }
+
     virtual geometry_msgs::PoseWithCovariance evaluate(std::vector<Particle>&, const std::vector<CudaPoint>&,
                                                        float[16])
     {
@@ -99,7 +128,7 @@ public:
         std::random_device device{};
         m_generator_ptr.reset(new std::mt19937(device()));
     }
-    virtual ~Resampler() = default;
+
     virtual void resample(ParticleCloud& particle_cloud) = 0;
 
 protected:
@@ -173,6 +202,29 @@ inline void throw_on(int rc)
     if (rc == TSL_ERR_NO_VALID_PARTICLE) throw std::runtime_error("No particle is valid!");
     throw std::runtime_error(tsl_last_error());
 }
+
+// Offset of a named field of a PointCloud2; PointCloud2ConstIterator throws
+// the same message for a missing field (point_cloud2_iterator.h).
+inline uint32_t cloud_field(const sensor_msgs::PointCloud2& cloud, const std::string& name)
+{
+    for (const auto& f : cloud.fields)
+        if (f.name == name) return f.offset;
+    throw std::runtime_error("Field " + name + " does not exist");
+}
+
+inline geometry_msgs::PoseWithCovariance to_pose(const tsl_pose& p)
+{
+    geometry_msgs::PoseWithCovariance out;
+    out.pose.position.x = p.position[0];
+    out.pose.position.y = p.position[1];
+    out.pose.position.z = p.position[2];
+    out.pose.orientation.x = p.orientation[0];
+    out.pose.orientation.y = p.orientation[1];
+    out.pose.orientation.z = p.orientation[2];
+    out.pose.orientation.w = p.orientation[3];
+    for (int i = 0; i < 36; ++i) out.covariance[i] = p.covariance[i];
+    return out;
+}
 }  // namespace tsl_detail
 
 class HipEvaluator : public Evaluator
@@ -212,13 +264,14 @@ public:
     HipEvaluator(const HipEvaluator&) = delete;
     HipEvaluator& operator=(const HipEvaluator&) = delete;
 
-    ~HipEvaluator() override
+    // The reference's Evaluator has no virtual destructor (cuda_evaluator.h:94-107).
+    virtual ~HipEvaluator()
     {
         tsl_evaluator_destroy(ev_);
         tsl_map_destroy(map_);
     }
 
-    // Evaluator::evaluate (cuda_evaluator.h:102)
+    // Evaluator::evaluate(particles, points, tf) (cuda_evaluator.h:102)
     geometry_msgs::PoseWithCovariance evaluate(std::vector<Particle>& particles, const std::vector<CudaPoint>& points,
                                                float tf_matrix[16]) override
     {
@@ -226,17 +279,39 @@ public:
         tsl_detail::throw_on(tsl_evaluate(ev_, reinterpret_cast<tsl_particle*>(particles.data()), particles.size(),
                                           reinterpret_cast<const tsl_point*>(points.data()), points.size(), tf_matrix,
                                           &p));
-        geometry_msgs::PoseWithCovariance out;
-        out.pose.position.x = p.position[0];
-        out.pose.position.y = p.position[1];
-        out.pose.position.z = p.position[2];
-        out.pose.orientation.x = p.orientation[0];
-        out.pose.orientation.y = p.orientation[1];
-        out.pose.orientation.z = p.orientation[2];
-        out.pose.orientation.w = p.orientation[3];
-        for (int i = 0; i < 36; ++i) out.covariance[i] = p.covariance[i];
-        return out;
+        return tsl_detail::to_pose(p);
     }
+
+    // Evaluator::evaluate(particles, cloud, tf) (cuda_evaluator.h:97-100,
+    // src/cuda/cuda_evaluator.cu:78-116): the cloud's x, y, z floats and
+    // its ring field read as int, as PointCloud2ConstIterator<float>(cloud,
+    // "x") and <int>(cloud, "ring") read them (one point per point_step
+    // bytes of data), then the 0.064 m centre reduction on the device and the
+    // evaluation.  The reference's centres come in hash-set order; here
+    // they are ordered by ring, then index (tsl_filter_set_scan_cloud).
+    geometry_msgs::PoseWithCovariance evaluate(std::vector<Particle>& particles,
+                                               const sensor_msgs::PointCloud2& cloud, float tf_matrix[16]) override
+    {
+        const uint32_t ox = tsl_detail::cloud_field(cloud, "x");
+        const uint32_t oring = tsl_detail::cloud_field(cloud, "ring");
+        const size_t step = cloud.point_step;
+        const size_t n = step ? cloud.data.size() / step : 0;
+        std::vector<tsl_point> pts(n);
+        std::vector<int32_t> rings(n);
+        for (size_t i = 0; i < n; ++i) {
+            const uint8_t* rec = cloud.data.data() + i * step;
+            if (ox + 12 <= step) std::memcpy(&pts[i], rec + ox, 12);
+            if (oring + 4 <= step) std::memcpy(&rings[i], rec + oring, 4);
+        }
+        tsl_pose p;
+        tsl_detail::throw_on(tsl_evaluate_cloud(ev_, reinterpret_cast<tsl_particle*>(particles.data()),
+                                                particles.size(), pts.data(), rings.data(), n, tf_matrix, &p));
+        return tsl_detail::to_pose(p);
+    }
+
+    // sin/cos of the particle angles (TSL_TRIG_*, default glibc's sinf/cosf
+    // as on x86-64 with FMA3, the reference CPU evaluator's binding).
+    void setTrig(int mode) { tsl_detail::throw_on(tsl_evaluator_set_trig(ev_, mode)); }
 
     tsl_evaluator* handle() { return ev_; }
 

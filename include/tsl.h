@@ -167,6 +167,36 @@ int tsl_evaluator_create(tsl_map* map, int per_point, float a_hit, float a_range
 
 int tsl_evaluator_destroy(tsl_evaluator* ev);
 
+/* sin/cos of the particle angles in the pose matrices
+ * (tsdf_evaluator.cpp:102-145), the pose estimate's circular means
+ * (:206-213) and the motion model's particle rotation
+ * (particle_cloud.cpp:549-581).  The reference's CPU evaluator calls
+ * unqualified sin(float)/cos(float), i.e. the C library's sinf/cosf; glibc
+ * (2.28+) computes them by one algorithm built two ways on x86-64, with FMA
+ * contraction (selected on CPUs with FMA3) and without.  The library
+ * restates both, bit-exact with glibc on every float, on host and device:
+ *   TSL_TRIG_GLIBC_FMA   glibc sinf/cosf as on x86-64 with FMA3 (default)
+ *   TSL_TRIG_GLIBC_NOFMA glibc sinf/cosf as on x86-64 without FMA3
+ *   TSL_TRIG_DOUBLE      (float)sin((double)x), correctly rounded but for rare ties
+ * The reference's CUDA path uses CUDA's device sinf/cosf
+ * (cuda_eval_particles.h:290-295), which has no counterpart here.  Applies
+ * to every filter, batch and server of the evaluator created afterwards and
+ * to its host-buffer entry points. */
+#define TSL_TRIG_GLIBC_FMA 0
+#define TSL_TRIG_GLIBC_NOFMA 1
+#define TSL_TRIG_DOUBLE 2
+int tsl_evaluator_set_trig(tsl_evaluator* ev, int mode);
+int tsl_evaluator_get_trig(const tsl_evaluator* ev, int* mode);
+
+/* sin and cos of n angles in a trig mode, on the device (on_device != 0) or
+ * by the host build of the same code; for parity checking.  Synchronous. */
+int tsl_trig(int mode, const float* x, size_t n, float* s, float* c, int on_device);
+
+/* Host self-check: the host build of a trig mode against this process's C
+ * library sinf/cosf over every non-NaN float whose bit pattern is a
+ * multiple of stride.  No counterpart in the reference. */
+int tsl_check_trig_libm(int mode, uint32_t stride, uint64_t* mismatches, uint64_t* checked);
+
 /* Evaluator::evaluate(particles, points, tf_matrix)
  * (cuda_evaluator.h:102; src/cuda/cuda_evaluator.cu:118-428;
  * CPU semantics src/evaluation/tsdf_evaluator.cpp:78-245).
@@ -178,6 +208,13 @@ int tsl_evaluator_destroy(tsl_evaluator* ev);
 int tsl_evaluate(tsl_evaluator* ev, tsl_particle* particles, size_t num_particles,
                  const tsl_point* points, size_t num_points, const float tf_matrix[16],
                  tsl_pose* out_pose);
+
+/* Evaluator::evaluate(particles, PointCloud2, tf)
+ * (cuda_evaluator.h:97-100; src/cuda/cuda_evaluator.cu:78-116): the cloud's
+ * x, y, z and ring fields as arrays, reduced as tsl_filter_set_scan_cloud
+ * describes, then evaluated as tsl_evaluate.  Synchronous. */
+int tsl_evaluate_cloud(tsl_evaluator* ev, tsl_particle* particles, size_t num_particles, const tsl_point* raw,
+                       const int32_t* ring, size_t num_raw, const float tf_matrix[16], tsl_pose* out_pose);
 
 /* Raw (unnormalised) per-particle weights of one evaluation, written to
  * weights_out[num_particles]; particles are not modified.  Synchronous. */
@@ -208,9 +245,10 @@ int tsl_trace_values(tsl_evaluator* ev, const tsl_particle* particles, size_t nu
 int tsl_check_sub_table(float resolution, uint32_t stride, uint64_t* mismatches, uint32_t* entries);
 int tsl_check_magic_divisor(uint32_t divisor, uint32_t stride, uint64_t* mismatches);
 
-/* Pose matrices L[12] (tsdf_evaluator.cpp:102-145) computed on the device. */
+/* Pose matrices L[12] (tsdf_evaluator.cpp:102-145) computed on the device
+ * with the sin/cos of trig mode `trig` (TSL_TRIG_*). */
 int tsl_pose_matrices(const tsl_particle* particles, size_t num_particles,
-                      const float tf_matrix[16], float* out);
+                      const float tf_matrix[16], int trig, float* out);
 
 /* ---- resampling: SystematicResampler ----------------------------------- */
 
@@ -258,8 +296,20 @@ int tsl_filter_set_scan(tsl_filter* f, const tsl_point* points, size_t n,
 int tsl_filter_set_scan_raw(tsl_filter* f, const tsl_point* raw, const int16_t* ring, size_t n, float cell,
                             const float tf_matrix[16]);
 
+/* The point reduction of CudaEvaluator::evaluate(particles, PointCloud2,
+ * tf) (src/cuda/cuda_evaluator.cu:78-116) on the device, then as
+ * tsl_filter_set_scan: every raw point becomes the centre of its 0.064 m
+ * voxel, floor(v / 0.064) * 0.064 + 0.032 evaluated in double and rounded
+ * to float, and each distinct centre is kept once.  The reference orders
+ * the centres by its hash set's iteration (an unordered_set, unspecified);
+ * here they are ordered by the ring (int32, the reference reads the field
+ * as int), then the index, of each centre's lowest-index point.  The order
+ * changes only the float summation order of the weights. */
+int tsl_filter_set_scan_cloud(tsl_filter* f, const tsl_point* raw, const int32_t* ring, size_t n,
+                              const float tf_matrix[16]);
+
 /* The filter's scan points in the order they were given (after the
- * reduction, for tsl_filter_set_scan_raw). */
+ * reduction, for tsl_filter_set_scan_raw / _cloud). */
 int tsl_filter_get_scan(tsl_filter* f, tsl_point* out, size_t capacity, size_t* n);
 
 /* Sensor update on `stream`: pose matrices, evaluation, normalisation and

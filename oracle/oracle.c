@@ -383,17 +383,21 @@ ORC_EXPORT void orc_chunks_to_cells(const int32_t* pos, const uint32_t* words, s
 /* ------------------------------------------------------------------------ */
 /* Pose matrix of one particle (tsdf_evaluator.cpp:102-145)                 */
 /* ------------------------------------------------------------------------ */
-/* mode 0: the shared sincos above; 1: glibc sinf/cosf; 2: (float)sin((double)x). */
+/* The reference calls unqualified sin/cos on float angles
+ * (tsdf_evaluator.cpp:108-113, 206-213; particle_cloud.cpp:262-267), i.e.
+ * std::sin(float) = the C library's sinf.
+ * mode 0 and 1: this process's C library sinf/cosf (glibc chooses its FMA
+ * or non-FMA build when it loads; the library's TSL_TRIG_GLIBC_FMA and
+ * TSL_TRIG_GLIBC_NOFMA restate the two);
+ * mode 2: the double-evaluated routine above (the library's TSL_TRIG_DOUBLE). */
 static void orc_trig(float a, int mode, float* s, float* c)
 {
-    if (mode == 1) {
-        *s = sinf(a);
-        *c = cosf(a);
-    } else if (mode == 2) {
-        *s = (float)sin((double)a);
-        *c = (float)cos((double)a);
-    } else {
+    if (mode == 2) {
         orc_sincosf(a, s, c);
+    } else {
+        volatile float av = a;  /* a call into the C library, never a folded constant */
+        *s = sinf(av);
+        *c = cosf(av);
     }
 }
 
@@ -503,6 +507,51 @@ ORC_EXPORT void orc_evaluate(const orc_coef* c, const int32_t* grid_occ, const f
         }
         free(copy);
     }
+}
+
+/* Two trig bindings compared on the same inputs: the number of (particle,
+ * point) pairs whose voxel index differs between trig modes a and b, and the
+ * largest relative difference of a particle's float64 weight sum.  A
+ * measurement of how much the choice of sin/cos matters, not a restatement. */
+ORC_EXPORT void orc_trig_binding_diff(const orc_coef* c, const int32_t* grid_occ, const float* data,
+                                      const orc_particle* particles, size_t num_particles,
+                                      const orc_point* points, size_t num_points, const float tf[16],
+                                      const orc_params* prm, int mode_a, int mode_b,
+                                      uint64_t* index_mismatches, uint64_t* matrix_mismatches,
+                                      double* max_rel_weight)
+{
+    float inv_mr, mr2;
+    orc_derived(prm, &inv_mr, &mr2);
+    uint64_t bad = 0, badm = 0;
+    double worst = 0.0;
+    #pragma omp parallel for schedule(dynamic) reduction(+:bad, badm) reduction(max:worst)
+    for (long i = 0; i < (long)num_particles; ++i) {
+        float La[12], Lb[12];
+        orc_pose_matrix(particles[i].first, tf, mode_a, La);
+        orc_pose_matrix(particles[i].first, tf, mode_b, Lb);
+        if (memcmp(La, Lb, sizeof(La)) != 0) {
+            ++badm;
+            for (size_t j = 0; j < num_points; ++j) {
+                float x = points[j].x, y = points[j].y, z = points[j].z;
+                uint64_t ia = orc_get_index(c, grid_occ, La[0] * x + La[1] * y + La[2] * z + La[3],
+                                            La[4] * x + La[5] * y + La[6] * z + La[7],
+                                            La[8] * x + La[9] * y + La[10] * z + La[11]);
+                uint64_t ib = orc_get_index(c, grid_occ, Lb[0] * x + Lb[1] * y + Lb[2] * z + Lb[3],
+                                            Lb[4] * x + Lb[5] * y + Lb[6] * z + Lb[7],
+                                            Lb[8] * x + Lb[9] * y + Lb[10] * z + Lb[11]);
+                bad += ia != ib;
+            }
+            float wa32, wb32;
+            double wa, wb;
+            orc_eval_pose(c, grid_occ, data, La, points, num_points, prm, inv_mr, mr2, &wa32, &wa);
+            orc_eval_pose(c, grid_occ, data, Lb, points, num_points, prm, inv_mr, mr2, &wb32, &wb);
+            double rel = wa != 0.0 ? fabs(wa - wb) / fabs(wa) : (wb != 0.0 ? 1.0 : 0.0);
+            if (rel > worst) worst = rel;
+        }
+    }
+    *index_mismatches = bad;
+    *matrix_mismatches = badm;
+    *max_rel_weight = worst;
 }
 
 /* Voxel index of every (particle, point) pair, row-major [particle][point]:
@@ -652,7 +701,7 @@ ORC_EXPORT double orc_motion_normal(uint64_t seed, uint64_t particle, uint32_t d
 }
 
 ORC_EXPORT void orc_motion(const orc_particle* in, size_t n, const double mean[6], const double stddev[6],
-                           uint64_t seed, orc_particle* out)
+                           uint64_t seed, int trig_mode, orc_particle* out)
 {
     for (size_t i = 0; i < n; ++i) {
         const float* last = in[i].first;
@@ -678,9 +727,9 @@ ORC_EXPORT void orc_motion(const orc_particle* in, size_t n, const double mean[6
         tf_odom[11] = (float)d[2];
         /* :549-581: particle rotation, no translation */
         float sin_alpha, cos_alpha, sin_beta, cos_beta, sin_gamma, cos_gamma;
-        orc_sincosf(last[3], &sin_alpha, &cos_alpha);
-        orc_sincosf(last[4], &sin_beta, &cos_beta);
-        orc_sincosf(last[5], &sin_gamma, &cos_gamma);
+        orc_trig(last[3], trig_mode, &sin_alpha, &cos_alpha);
+        orc_trig(last[4], trig_mode, &sin_beta, &cos_beta);
+        orc_trig(last[5], trig_mode, &sin_gamma, &cos_gamma);
         float tp[16];
         tp[0] = cos_beta * cos_gamma;
         tp[4] = cos_beta * sin_gamma;
@@ -813,6 +862,72 @@ ORC_EXPORT long orc_reduce_scan(const float* xyz, const int16_t* ring, size_t n,
         out_xyz[3 * i + 1] = v[i].orig[1];
         out_xyz[3 * i + 2] = v[i].orig[2];
     }
+    free(v);
+    return (long)w;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CudaEvaluator::evaluate(particles, PointCloud2, tf) point reduction      */
+/* (src/cuda/cuda_evaluator.cu:78-116):                                      */
+/*   - every point is kept (no range test), read with its ring as int;     */
+/*   - centre = floor(v / 0.064) * 0.064 + 0.032 per axis, in double (the  */
+/*     literals are double) and stored as float (:95-97);                  */
+/*   - an unordered_set<CudaPoint> keeps each distinct centre once (float  */
+/*     equality; NaN never equal), and the reduced scan is the centres.    */
+/* The set's iteration order is unspecified; the order written here is    */
+/* the library's documented one: by the ring, then the index, of each      */
+/* centre's lowest-index point.                                            */
+/* Returns the number of centres written.                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t ring;
+    uint32_t index;
+    float c[3];
+} orc_cloud;
+
+static int orc_cloud_cmp_key(const void* pa, const void* pb)
+{
+    const orc_cloud* a = (const orc_cloud*)pa;
+    const orc_cloud* b = (const orc_cloud*)pb;
+    for (int k = 0; k < 3; ++k) {
+        uint32_t ua, ub;
+        memcpy(&ua, &a->c[k], 4);
+        memcpy(&ub, &b->c[k], 4);
+        if (ua != ub) return ua < ub ? -1 : 1;
+    }
+    return a->index < b->index ? -1 : (a->index > b->index ? 1 : 0);
+}
+
+static int orc_cloud_cmp_out(const void* pa, const void* pb)
+{
+    const orc_cloud* a = (const orc_cloud*)pa;
+    const orc_cloud* b = (const orc_cloud*)pb;
+    if (a->ring != b->ring) return a->ring < b->ring ? -1 : 1;
+    return a->index < b->index ? -1 : (a->index > b->index ? 1 : 0);
+}
+
+ORC_EXPORT long orc_reduce_cloud(const float* xyz, const int32_t* ring, size_t n, float* out_xyz)
+{
+    orc_cloud* v = (orc_cloud*)malloc(sizeof(orc_cloud) * (n ? n : 1));
+    for (size_t j = 0; j < n; ++j) {
+        v[j].ring = ring[j];
+        v[j].index = (uint32_t)j;
+        for (int k = 0; k < 3; ++k)
+            v[j].c[k] = (float)(floor((double)xyz[3 * j + k] / 0.064) * 0.064 + 0.032);
+    }
+    qsort(v, n, sizeof(orc_cloud), orc_cloud_cmp_key);
+    size_t w = 0;
+    for (size_t i = 0; i < n; ++i) {
+        int nan = v[i].c[0] != v[i].c[0] || v[i].c[1] != v[i].c[1] || v[i].c[2] != v[i].c[2];
+        int first = i == 0 || nan || v[i].c[0] != v[i - 1].c[0] || v[i].c[1] != v[i - 1].c[1] ||
+                    v[i].c[2] != v[i - 1].c[2];
+        if (first)
+            v[w++] = v[i];
+    }
+    qsort(v, w, sizeof(orc_cloud), orc_cloud_cmp_out);
+    for (size_t i = 0; i < w; ++i)
+        for (int k = 0; k < 3; ++k)
+            out_xyz[3 * i + k] = v[i].c[k];
     free(v);
     return (long)w;
 }

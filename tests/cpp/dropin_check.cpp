@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -91,6 +92,44 @@ int main(int argc, char** argv)
         ParticleCloud cloud(particles);
         resampler.resample(cloud);
 
+        // Evaluator::evaluate(particles, PointCloud2, tf): the same points as
+        // a cloud of 16-byte records (x, y, z float32, ring uint16 at 12, two
+        // zero bytes), ring = point index % 16, evaluated on the initial weights
+        sensor_msgs::PointCloud2 pc2;
+        pc2.height = 1;
+        pc2.width = (uint32_t)N;
+        pc2.point_step = 16;
+        pc2.row_step = 16 * (uint32_t)N;
+        const char* names[4] = {"x", "y", "z", "ring"};
+        for (int k = 0; k < 4; ++k) {
+            sensor_msgs::PointField fld;
+            fld.name = names[k];
+            fld.offset = 4 * k;
+            fld.datatype = k < 3 ? sensor_msgs::PointField::FLOAT32 : sensor_msgs::PointField::UINT16;
+            fld.count = 1;
+            pc2.fields.push_back(fld);
+        }
+        pc2.data.assign(16 * N, 0);
+        for (uint64_t i = 0; i < N; ++i) {
+            std::memcpy(pc2.data.data() + 16 * i, &points[i], 12);
+            const uint16_t ring = (uint16_t)(i % 16);
+            std::memcpy(pc2.data.data() + 16 * i + 12, &ring, 2);
+        }
+        std::vector<Particle> cloud_particles(P);
+        {
+            FILE* fr = std::fopen(argv[1], "rb");
+            Reader rr{fr};
+            float skip[8];
+            rr.get(skip, 8);
+            const uint64_t nc = rr.get<uint64_t>();
+            std::vector<float> tmp(4 * nc);
+            rr.get(tmp.data(), tmp.size());
+            (void)rr.get<uint64_t>();
+            rr.get(reinterpret_cast<float*>(cloud_particles.data()), 7 * P);
+            std::fclose(fr);
+        }
+        const geometry_msgs::PoseWithCovariance cloud_pose = evaluator->evaluate(cloud_particles, pc2, tf);
+
         // error path: every weight zero (no range term, points in unknown space)
         int error_ok = 0;
         {
@@ -120,6 +159,10 @@ int main(int argc, char** argv)
         put(fo, reinterpret_cast<const float*>(cloud.particles().data()), 7 * n);
         const int32_t e = error_ok;
         put(fo, &e, 1);
+        for (uint64_t i = 0; i < P; ++i) w[i] = cloud_particles[i].second;
+        put(fo, w.data(), P);
+        const double cp[3] = {cloud_pose.pose.position.x, cloud_pose.pose.position.y, cloud_pose.pose.position.z};
+        put(fo, cp, 3);
         std::fclose(fo);
     } catch (const std::exception& ex) {
         std::fprintf(stderr, "dropin_check: %s\n", ex.what());

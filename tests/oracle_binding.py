@@ -16,6 +16,17 @@ if not os.path.exists(ORACLE_PATH):
 
 olib = ctypes.CDLL(ORACLE_PATH)
 
+# sin/cos of the particle angles: the C library's sinf/cosf (the reference's
+# unqualified sin(float)), or the double-evaluated routine
+TRIG_LIBM = 0
+TRIG_DOUBLE = 2
+
+
+def trig_mode_for(library_mode):
+    """Oracle trig mode that restates a library TSL_TRIG_* mode: the glibc
+    modes are the C library itself (the variant this process's glibc chose)."""
+    return TRIG_DOUBLE if int(library_mode) == 2 else TRIG_LIBM
+
 
 class OrcCoef(ctypes.Structure):
     _fields_ = [
@@ -61,7 +72,11 @@ _sigs = {
     "orc_systematic_u": (None, [_S, _F, _P, _S]),
     "orc_cumsum": (None, [_P, _S, _P]),
     "orc_motion_normal": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]),
-    "orc_motion": (None, [_P, _S, _P, _P, ctypes.c_uint64, _P]),
+    "orc_motion": (None, [_P, _S, _P, _P, ctypes.c_uint64, ctypes.c_int, _P]),
+    "orc_reduce_cloud": (ctypes.c_long, [_P, _P, _S, _P]),
+    "orc_trig_binding_diff": (None, [ctypes.POINTER(OrcCoef), _P, _P, _P, _S, _P, _S, _P, ctypes.POINTER(OrcParams),
+                                     ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                     ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]),
     "orc_reduce_scan": (ctypes.c_long, [_P, _P, _S, _F, _P]),
 }
 for _n, (_r, _a) in _sigs.items():
@@ -154,6 +169,22 @@ class OracleMap:
                           _ptr(w32), _ptr(w64))
         return w32, w64
 
+    def trig_binding_diff(self, particles, points, tf, mode_a, mode_b, a_hit=0.9, a_range=0.1, a_max=0.0,
+                          max_range=100.0):
+        """(index mismatches, pose-matrix mismatches, max relative weight
+        difference) between two trig modes on the same inputs."""
+        p = np.ascontiguousarray(particles, np.float32)
+        pts = np.ascontiguousarray(points, np.float32)
+        tf = np.ascontiguousarray(tf, np.float32)
+        prm = OrcParams(a_hit, a_range, a_max, max_range)
+        bad = ctypes.c_uint64()
+        badm = ctypes.c_uint64()
+        worst = ctypes.c_double()
+        olib.orc_trig_binding_diff(ctypes.byref(self.coef), _ptr(self.grid), _ptr(self.data), _ptr(p), p.shape[0],
+                                   _ptr(pts), pts.shape[0], _ptr(tf), ctypes.byref(prm), int(mode_a), int(mode_b),
+                                   ctypes.byref(bad), ctypes.byref(badm), ctypes.byref(worst))
+        return bad.value, badm.value, worst.value
+
     def trace(self, particles, points, tf, trig_mode=0):
         p = np.ascontiguousarray(particles, np.float32)
         pts = np.ascontiguousarray(points, np.float32)
@@ -231,14 +262,24 @@ def cumsum(weights):
     return out
 
 
-def motion(particles, mean, stddev, seed):
+def motion(particles, mean, stddev, seed, trig_mode=0):
     """ParticleCloud::apply_model with the counter-based draws."""
     p = np.ascontiguousarray(particles, np.float32)
     out = np.empty_like(p)
     m = np.ascontiguousarray(mean, np.float64)
     sd = np.ascontiguousarray(stddev, np.float64)
-    olib.orc_motion(_ptr(p), p.shape[0], _ptr(m), _ptr(sd), int(seed), _ptr(out))
+    olib.orc_motion(_ptr(p), p.shape[0], _ptr(m), _ptr(sd), int(seed), int(trig_mode), _ptr(out))
     return out
+
+
+def reduce_cloud(points, rings):
+    """Point reduction of CudaEvaluator::evaluate(PointCloud2): distinct
+    0.064 m voxel centres, ordered by (ring, index) of their first point."""
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    r = np.ascontiguousarray(rings, np.int32).reshape(-1)
+    out = np.empty((max(p.shape[0], 1), 3), np.float32)
+    n = olib.orc_reduce_cloud(_ptr(p), _ptr(r), p.shape[0], _ptr(out))
+    return out[:n].copy()
 
 
 def reduce_scan(points, rings, cell=0.064):

@@ -79,8 +79,10 @@ def _read_output(path, P):
     U = np.frombuffer(b, np.float32, 1, o)[0]; o += 4
     n = struct.unpack_from("<Q", b, o)[0]; o += 8
     res = np.frombuffer(b, np.float32, 7 * n, o).reshape(n, 7); o += 28 * n
-    err = struct.unpack_from("<i", b, o)[0]
-    return pose, cov, w, U, res, err
+    err = struct.unpack_from("<i", b, o)[0]; o += 4
+    wc = np.frombuffer(b, np.float32, P, o); o += 4 * P
+    cpose = np.frombuffer(b, np.float64, 3, o)
+    return pose, cov, w, U, res, err, wc, cpose
 
 
 @pytest.mark.parametrize("seed", [7, 8])
@@ -93,7 +95,7 @@ def test_cpp_dropin_matches_oracle(tmp_path, seed):
     print(f"dropin_check seed={seed} P={parts.shape[0]} N={pts.shape[0]}", flush=True)
     r = subprocess.run([BIN, fin, fout], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    pose, cov, w, U, resampled, err = _read_output(fout, parts.shape[0])
+    pose, cov, w, U, resampled, err, wc, cpose = _read_output(fout, parts.shape[0])
 
     c, grid, data = ob.set_data(box[0], box[1], box[2], box[3], cells, vals)
     om = ob.OracleMap(c, grid, data)
@@ -116,3 +118,13 @@ def test_cpp_dropin_matches_oracle(tmp_path, seed):
     assert np.array_equal(resampled, expect[idx])
 
     assert err == 1, "zero weight sum must raise 'No particle is valid!'"
+
+    # the PointCloud2 overload: the oracle's 0.064 m centre reduction of the
+    # same points (rings = index % 16), evaluated on the initial particles
+    centres = ob.reduce_cloud(pts, np.arange(pts.shape[0]) % 16)
+    assert 0 < centres.shape[0] <= pts.shape[0]
+    _, wc64 = om.evaluate(parts, centres, tf, *params)
+    refc = wc64 / wc64.sum()
+    assert np.max(np.abs(wc.astype(np.float64) - refc) / refc) <= WEIGHT_RTOL
+    _, prefc = ob.normalize(parts, wc64.astype(np.float32))
+    assert np.allclose(cpose, prefc[:3], rtol=0, atol=1e-4)

@@ -7,11 +7,12 @@ include/tsl.h.  This package is the Python mirror of the reference's
 evaluator/resampler interface over that ABI.
 """
 from .evaluator import (Batch, CudaSubVoxelMap, DeviceFilter, Event, ParticleCloud, Stream, SystematicResampler,
-                        TSDFEvaluator, coef_init, likelihood_init, likelihood_value, pose_matrices,
-                        resample_systematic)
-from ._lib import TslError, device_count
+                        TSDFEvaluator, check_trig_libm, coef_init, likelihood_init, likelihood_value, pose_matrices,
+                        resample_systematic, trig)
+from ._lib import TRIG_DOUBLE, TRIG_GLIBC_FMA, TRIG_GLIBC_NOFMA, TslError, device_count
 from . import evaluator  # noqa: F401  (motion helpers: evaluator.motion_from_delta, ...)
 
 __all__ = ["Batch", "CudaSubVoxelMap", "DeviceFilter", "Event", "ParticleCloud", "Stream", "SystematicResampler",
-           "TSDFEvaluator", "TslError", "coef_init", "device_count", "likelihood_init", "likelihood_value",
-           "pose_matrices", "resample_systematic"]
+           "TRIG_DOUBLE", "TRIG_GLIBC_FMA", "TRIG_GLIBC_NOFMA", "TSDFEvaluator", "TslError", "check_trig_libm",
+           "coef_init", "device_count", "likelihood_init", "likelihood_value", "pose_matrices", "resample_systematic",
+           "trig"]

@@ -19,6 +19,11 @@ TSL_ERR_CAPACITY = 5
 TSL_ERR_UNSUPPORTED = 6
 TSL_ERR_ADMISSION = 7
 
+# sin/cos of the particle angles (include/tsl.h TSL_TRIG_*)
+TRIG_GLIBC_FMA = 0
+TRIG_GLIBC_NOFMA = 1
+TRIG_DOUBLE = 2
+
 
 class MapCoef(ctypes.Structure):
     """tsl_map_coef == CudaSubVoxelMap<float,float>::MapCoef (cuda_sub_voxel_map.h:22-50)."""
@@ -97,7 +102,14 @@ _SIGS = {
     "tsl_trace_indices": (_I, [_P, _P, _S, _P, _S, _P, _P]),
     "tsl_trace_values": (_I, [_P, _P, _S, _P, _S, _P, _P]),
     "tsl_check_magic_divisor": (_I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
-    "tsl_pose_matrices": (_I, [_P, _S, _P, _P]),
+    "tsl_pose_matrices": (_I, [_P, _S, _P, _I, _P]),
+    "tsl_evaluator_set_trig": (_I, [_P, _I]),
+    "tsl_evaluator_get_trig": (_I, [_P, ctypes.POINTER(_I)]),
+    "tsl_trig": (_I, [_I, _P, _S, _P, _P, _I]),
+    "tsl_check_trig_libm": (_I, [_I, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
+                                 ctypes.POINTER(ctypes.c_uint64)]),
+    "tsl_evaluate_cloud": (_I, [_P, _P, _S, _P, _P, _S, _P, ctypes.POINTER(Pose)]),
+    "tsl_filter_set_scan_cloud": (_I, [_P, _P, _P, _S, _P]),
     "tsl_resample_systematic": (_I, [_P, _S, _F, _P, _S, _P, ctypes.POINTER(_S)]),
     "tsl_resample_capacity": (_I, [_S, _F, ctypes.POINTER(_S)]),
     "tsl_filter_create": (_I, [_P, _S, ctypes.POINTER(_P)]),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -101,6 +102,7 @@ int filter_member(tsl_filter* f, Member** d_member, uint32_t chunk_len, uint32_t
     h.plan = f->plan.as<RsPlan>();
     h.u0_dev = nullptr;
     std::memcpy(h.tf, f->tf, sizeof(h.tf));
+    h.trig = f->ev->trig;
     h.N = (uint32_t)f->num_points;
     h.chunk_len = chunks ? chunk_len : f->member_host.chunk_len;
     h.chunks = chunks ? chunks : f->member_host.chunks;
@@ -854,6 +856,82 @@ int tsl_evaluator_create(tsl_map* map, int per_point, float a_hit, float a_range
     return TSL_OK;
 }
 
+int tsl_evaluator_set_trig(tsl_evaluator* ev, int mode)
+{
+    if (!ev) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL evaluator");
+    if (mode < TSL_TRIG_GLIBC_FMA || mode > TSL_TRIG_DOUBLE) return set_error(TSL_ERR_INVALID_ARGUMENT, "bad trig mode");
+    ev->trig = mode;
+    return TSL_OK;
+}
+
+int tsl_evaluator_get_trig(const tsl_evaluator* ev, int* mode)
+{
+    if (!ev || !mode) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    *mode = ev->trig;
+    return TSL_OK;
+}
+
+int tsl_trig(int mode, const float* x, size_t n, float* s, float* c, int on_device)
+{
+    if (n && (!x || !s || !c)) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (mode < TSL_TRIG_GLIBC_FMA || mode > TSL_TRIG_DOUBLE) return set_error(TSL_ERR_INVALID_ARGUMENT, "bad trig mode");
+    if (n > 0x7fffffffull) return set_error(TSL_ERR_UNSUPPORTED, "too many angles");
+    if (n == 0) return TSL_OK;
+    if (!on_device) {
+        for (size_t i = 0; i < n; ++i) sincosf_mode(mode, x[i], &s[i], &c[i]);
+        return TSL_OK;
+    }
+    DevBuf d;
+    TSL_TRY(d.reserve(sizeof(float) * 3 * n));
+    float* dx = d.as<float>();
+    TSL_HIP_CHECK(hipMemcpy(dx, x, sizeof(float) * n, hipMemcpyHostToDevice));
+    TSL_TRY(launch_trig(dx, (uint32_t)n, mode, dx + n, dx + 2 * n, 0));
+    TSL_HIP_CHECK(hipDeviceSynchronize());
+    TSL_HIP_CHECK(hipMemcpy(s, dx + n, sizeof(float) * n, hipMemcpyDeviceToHost));
+    TSL_HIP_CHECK(hipMemcpy(c, dx + 2 * n, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return TSL_OK;
+}
+
+int tsl_check_trig_libm(int mode, uint32_t stride, uint64_t* mismatches, uint64_t* checked)
+{
+    if (!mismatches || !checked || stride == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "bad argument");
+    if (mode < TSL_TRIG_GLIBC_FMA || mode > TSL_TRIG_DOUBLE) return set_error(TSL_ERR_INVALID_ARGUMENT, "bad trig mode");
+    const unsigned nt = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    std::vector<uint64_t> bad(nt, 0), seen(nt, 0);
+    std::vector<std::thread> th;
+    const uint64_t span = 0x100000000ull;
+    for (unsigned t = 0; t < nt; ++t) {
+        th.emplace_back([&, t]() {
+            const uint64_t per = (span / stride + nt - 1) / nt;
+            const uint64_t k0 = per * t, k1 = std::min(per * (t + 1), (span + stride - 1) / stride);
+            uint64_t b = 0, n = 0;
+            for (uint64_t k = k0; k < k1; ++k) {
+                const uint32_t u = (uint32_t)(k * stride);
+                float x;
+                std::memcpy(&x, &u, 4);
+                if (x != x) continue;
+                float s, c;
+                sincosf_mode(mode, x, &s, &c);
+                volatile float xv = x;
+                const float ls = ::sinf(xv), lc = ::cosf(xv);
+                if (std::memcmp(&s, &ls, 4) != 0 || std::memcmp(&c, &lc, 4) != 0) ++b;
+                ++n;
+            }
+            bad[t] = b;
+            seen[t] = n;
+        });
+    }
+    for (auto& x : th) x.join();
+    uint64_t b = 0, n = 0;
+    for (unsigned t = 0; t < nt; ++t) {
+        b += bad[t];
+        n += seen[t];
+    }
+    *mismatches = b;
+    *checked = n;
+    return TSL_OK;
+}
+
 int tsl_evaluator_destroy(tsl_evaluator* ev)
 {
     if (!ev) return TSL_OK;
@@ -869,6 +947,25 @@ static int scratch_filter(tsl_evaluator* ev, size_t P, tsl_filter** f)
     return TSL_OK;
 }
 
+// The evaluation of a host particle set against the scan already on the
+// evaluator's scratch filter: weights overwritten with the normalised
+// weights (cuda_evaluator.cu:405-408), pose returned.
+static int evaluate_filter_scan(tsl_evaluator* ev, tsl_filter* f, tsl_particle* particles, size_t P,
+                                tsl_pose* out_pose)
+{
+    (void)ev;
+    if (P == 0) return set_error(TSL_ERR_NO_VALID_PARTICLE, "No particle is valid!");
+    TSL_TRY(tsl_filter_set_particles(f, particles, P));
+    TSL_TRY(tsl_filter_update_async(f, nullptr));
+    int rc = tsl_filter_sync(f, nullptr, out_pose, nullptr);
+    if (rc != TSL_OK) return rc;
+    // the normalised weights, also kept contiguous by the normalisation
+    std::vector<float> w(P);
+    TSL_HIP_CHECK(hipMemcpy(w.data(), f->raw.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < P; ++i) particles[i].second = w[i];
+    return TSL_OK;
+}
+
 int tsl_evaluate(tsl_evaluator* ev, tsl_particle* particles, size_t P, const tsl_point* points, size_t N,
                  const float tf[16], tsl_pose* out_pose)
 {
@@ -879,19 +976,10 @@ int tsl_evaluate(tsl_evaluator* ev, tsl_particle* particles, size_t P, const tsl
         std::memset(out_pose, 0, sizeof(*out_pose));
         return TSL_OK;
     }
-    if (P == 0) return set_error(TSL_ERR_NO_VALID_PARTICLE, "No particle is valid!");
     tsl_filter* f;
     TSL_TRY(scratch_filter(ev, P, &f));
-    TSL_TRY(tsl_filter_set_particles(f, particles, P));
     TSL_TRY(tsl_filter_set_scan(f, points, N, tf));
-    TSL_TRY(tsl_filter_update_async(f, nullptr));
-    int rc = tsl_filter_sync(f, nullptr, out_pose, nullptr);
-    if (rc != TSL_OK) return rc;
-    // the normalised weights, also kept contiguous by the normalisation
-    std::vector<float> w(P);
-    TSL_HIP_CHECK(hipMemcpy(w.data(), f->raw.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < P; ++i) particles[i].second = w[i];  // cuda_evaluator.cu:405-408
-    return TSL_OK;
+    return evaluate_filter_scan(ev, f, particles, P, out_pose);
 }
 
 int tsl_evaluate_raw(tsl_evaluator* ev, const tsl_particle* particles, size_t P, const tsl_point* points,
@@ -944,15 +1032,16 @@ int tsl_trace_values(tsl_evaluator* ev, const tsl_particle* particles, size_t P,
     return trace_common(ev, particles, P, points, N, tf, out, sizeof(float), 1);
 }
 
-int tsl_pose_matrices(const tsl_particle* particles, size_t P, const float tf[16], float* out)
+int tsl_pose_matrices(const tsl_particle* particles, size_t P, const float tf[16], int trig, float* out)
 {
     if (!tf || (P && (!particles || !out))) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (trig < TSL_TRIG_GLIBC_FMA || trig > TSL_TRIG_DOUBLE) return set_error(TSL_ERR_INVALID_ARGUMENT, "bad trig mode");
     if (P == 0) return TSL_OK;
     DevBuf d_p, d_l;
     TSL_TRY(d_p.reserve(sizeof(tsl_particle) * P));
     TSL_TRY(d_l.reserve(sizeof(float) * 12 * P));
     TSL_HIP_CHECK(hipMemcpy(d_p.ptr, particles, sizeof(tsl_particle) * P, hipMemcpyHostToDevice));
-    TSL_TRY(launch_pose_matrices(d_p.as<tsl_particle>(), (uint32_t)P, tf, d_l.as<float>(), 0));
+    TSL_TRY(launch_pose_matrices(d_p.as<tsl_particle>(), (uint32_t)P, tf, trig, d_l.as<float>(), 0));
     TSL_HIP_CHECK(hipDeviceSynchronize());
     TSL_HIP_CHECK(hipMemcpy(out, d_l.ptr, sizeof(float) * 12 * P, hipMemcpyDeviceToHost));
     return TSL_OK;
@@ -964,8 +1053,8 @@ int tsl_resample_capacity(size_t M, float u0, size_t* capacity)
     return resample_capacity(M, u0, capacity);
 }
 
-int tsl_filter_set_scan_raw(tsl_filter* f, const tsl_point* raw, const int16_t* ring, size_t n, float cell,
-                            const float tf[16])
+static int set_scan_reduced(tsl_filter* f, int mode, const tsl_point* raw, const void* ring, size_t ring_bytes,
+                            size_t n, float cell, const float tf[16])
 {
     if (!f || !tf || (n && (!raw || !ring))) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
     if (n > 0x7fffffffull) return set_error(TSL_ERR_UNSUPPORTED, "too many points");
@@ -974,15 +1063,16 @@ int tsl_filter_set_scan_raw(tsl_filter* f, const tsl_point* raw, const int16_t* 
     TSL_HIP_CHECK(hipDeviceSynchronize());
     TSL_TRY(filter_reserve_points(f, n));
     TSL_TRY(f->raw_in.reserve(sizeof(tsl_point) * (n ? n : 1)));
-    TSL_TRY(f->raw_ring.reserve(sizeof(int16_t) * (n ? n : 1) + 16));  // rings, then two aligned counters
-    uint32_t* counters = reinterpret_cast<uint32_t*>(f->raw_ring.as<char>() + ((sizeof(int16_t) * (n ? n : 1) + 3) & ~(size_t)3));
+    const size_t ring_room = (ring_bytes * (n ? n : 1) + 3) & ~(size_t)3;
+    TSL_TRY(f->raw_ring.reserve(ring_room + 16));  // rings, then two aligned counters
+    uint32_t* counters = reinterpret_cast<uint32_t*>(f->raw_ring.as<char>() + ring_room);
     const uint32_t zero[2] = {0u, 0u};
     TSL_HIP_CHECK(hipMemcpy(counters, zero, sizeof(zero), hipMemcpyHostToDevice));
     uint32_t host[2] = {0u, 0u};
     if (n) {
         TSL_HIP_CHECK(hipMemcpy(f->raw_in.ptr, raw, sizeof(tsl_point) * n, hipMemcpyHostToDevice));
-        TSL_HIP_CHECK(hipMemcpy(f->raw_ring.ptr, ring, sizeof(int16_t) * n, hipMemcpyHostToDevice));
-        TSL_TRY(launch_reduce_scan(f, f->raw_in.as<tsl_point>(), f->raw_ring.as<int16_t>(), (uint32_t)n, cell,
+        TSL_HIP_CHECK(hipMemcpy(f->raw_ring.ptr, ring, ring_bytes * n, hipMemcpyHostToDevice));
+        TSL_TRY(launch_reduce_scan(f, mode, f->raw_in.as<tsl_point>(), f->raw_ring.ptr, (uint32_t)n, cell,
                                    f->points_in.as<tsl_point>(), counters, counters + 1, 0));
         TSL_HIP_CHECK(hipDeviceSynchronize());
         TSL_HIP_CHECK(hipMemcpy(host, counters, sizeof(host), hipMemcpyDeviceToHost));
@@ -993,6 +1083,34 @@ int tsl_filter_set_scan_raw(tsl_filter* f, const tsl_point* raw, const int16_t* 
     TSL_TRY(launch_prep_points(f, 0));
     TSL_HIP_CHECK(hipDeviceSynchronize());
     return TSL_OK;
+}
+
+int tsl_filter_set_scan_raw(tsl_filter* f, const tsl_point* raw, const int16_t* ring, size_t n, float cell,
+                            const float tf[16])
+{
+    return set_scan_reduced(f, kReduceEvaluator, raw, ring, sizeof(int16_t), n, cell, tf);
+}
+
+int tsl_filter_set_scan_cloud(tsl_filter* f, const tsl_point* raw, const int32_t* ring, size_t n,
+                              const float tf[16])
+{
+    return set_scan_reduced(f, kReduceCloud, raw, ring, sizeof(int32_t), n, 0.064f, tf);
+}
+
+int tsl_evaluate_cloud(tsl_evaluator* ev, tsl_particle* particles, size_t P, const tsl_point* raw,
+                       const int32_t* ring, size_t n, const float tf[16], tsl_pose* out_pose)
+{
+    if (!ev || !out_pose || !tf || (P && !particles) || (n && (!raw || !ring)))
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    clear_error();
+    tsl_filter* f;
+    TSL_TRY(scratch_filter(ev, P, &f));
+    TSL_TRY(tsl_filter_set_scan_cloud(f, raw, ring, n, tf));
+    if (f->num_points == 0) {  // evaluate(particles, {}, tf): cuda_evaluator.cu:122-125
+        std::memset(out_pose, 0, sizeof(*out_pose));
+        return TSL_OK;
+    }
+    return evaluate_filter_scan(ev, f, particles, P, out_pose);
 }
 
 int tsl_filter_get_scan(tsl_filter* f, tsl_point* out, size_t capacity, size_t* n)

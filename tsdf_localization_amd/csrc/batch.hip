@@ -106,6 +106,7 @@ int prepare(tsl_batch* b)
         h.u0_dev = b->u0_dev.as<float>() + i;
         h.snapshot = f->snapshot.as<tsl_particle>();
         std::memcpy(h.tf, f->tf, sizeof(h.tf));
+        h.trig = f->ev->trig;
         h.N = (uint32_t)f->num_points;
         h.chunk_len = t.chunk_len;
         h.chunks = t.chunks;

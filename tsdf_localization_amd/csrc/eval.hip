@@ -46,7 +46,7 @@ struct Tf12 {
 };
 
 __global__ void k_pose(const tsl_particle* __restrict__ particles, const uint32_t* __restrict__ count,
-                       uint32_t n_static, Tf12 tf, float* __restrict__ L)
+                       uint32_t n_static, Tf12 tf, int trig, float* __restrict__ L)
 {
     const uint32_t n = count ? *count : n_static;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,7 +55,7 @@ __global__ void k_pose(const tsl_particle* __restrict__ particles, const uint32_
 #pragma unroll
     for (int k = 0; k < 6; ++k) pose[k] = particles[i].first[k];
     float out[12];
-    pose_matrix(pose, tf.m, out);
+    pose_matrix(pose, tf.m, out, trig);
 #pragma unroll
     for (int k = 0; k < 12; ++k) L[(size_t)i * 12 + k] = out[k];
 }
@@ -75,7 +75,7 @@ __device__ __forceinline__ uint64_t slow_index(const DevMap& map, const SubEntry
 // inv[order[s]] = s lets the reduction read each particle's partial
 // weights back in the caller's order.
 __global__ void k_pose_ordered(const tsl_particle* __restrict__ particles, const uint32_t* __restrict__ count,
-                               const uint32_t* __restrict__ order, Tf12 tf, float* __restrict__ L,
+                               const uint32_t* __restrict__ order, Tf12 tf, int trig, float* __restrict__ L,
                                uint32_t* __restrict__ inv)
 {
     const uint32_t n = *count;
@@ -86,7 +86,7 @@ __global__ void k_pose_ordered(const tsl_particle* __restrict__ particles, const
 #pragma unroll
     for (int k = 0; k < 6; ++k) pose[k] = particles[o].first[k];
     float out[12];
-    pose_matrix(pose, tf.m, out);
+    pose_matrix(pose, tf.m, out, trig);
 #pragma unroll
     for (int k = 0; k < 12; ++k) L[(size_t)i * 12 + k] = out[k];
     inv[o] = i;
@@ -419,7 +419,7 @@ __global__ void k_pose_m(const Member* __restrict__ members)
 #pragma unroll
     for (int k = 0; k < 16; ++k) tf[k] = m.tf[k];
     float out[12];
-    pose_matrix(pose, tf, out);
+    pose_matrix(pose, tf, out, m.trig);
 #pragma unroll
     for (int k = 0; k < 12; ++k) m.L[(size_t)i * 12 + k] = out[k];
 }
@@ -544,7 +544,7 @@ __device__ __forceinline__ void normalize_slice(const Member& m, uint32_t P, uin
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float s, c;
-            sincosf_exact(particles[i].first[3 + a], &s, &c);
+            sincosf_mode(m.trig, particles[i].first[3 + a], &s, &c);
             q[3 + 2 * a] = (double)s * wd;
             q[4 + 2 * a] = (double)c * wd;
         }
@@ -870,13 +870,34 @@ int launch_eval_kernel(const Tile& tile, const tsl_filter* f, size_t P, size_t N
 
 }  // namespace
 
-int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const float tf[16], float* d_out,
+namespace {
+__global__ void k_trig(const float* __restrict__ x, uint32_t n, int mode, float* __restrict__ s,
+                       float* __restrict__ c)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float sv, cv;
+    sincosf_mode(mode, x[i], &sv, &cv);
+    s[i] = sv;
+    c[i] = cv;
+}
+}  // namespace
+
+int launch_trig(const float* d_x, uint32_t n, int mode, float* d_s, float* d_c, hipStream_t stream)
+{
+    if (n == 0) return TSL_OK;
+    hipLaunchKernelGGL(k_trig, dim3((n + 255) / 256), dim3(256), 0, stream, d_x, n, mode, d_s, d_c);
+    TSL_HIP_CHECK(hipGetLastError());
+    return TSL_OK;
+}
+
+int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const float tf[16], int trig, float* d_out,
                          hipStream_t stream)
 {
     if (n == 0) return TSL_OK;
     Tf12 t;
     for (int k = 0; k < 16; ++k) t.m[k] = tf[k];
-    hipLaunchKernelGGL(k_pose, dim3((n + 255) / 256), dim3(256), 0, stream, d_particles, nullptr, n, t,
+    hipLaunchKernelGGL(k_pose, dim3((n + 255) / 256), dim3(256), 0, stream, d_particles, nullptr, n, t, trig,
                        d_out);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
@@ -894,10 +915,11 @@ static int launch_pose_for_tile(tsl_filter* f, const Tile& tile, hipStream_t str
     if (tile.order) {
         TSL_TRY(launch_particle_order(f, stream));
         hipLaunchKernelGGL(k_pose_ordered, dim3(g), dim3(256), 0, stream, f->particles.as<tsl_particle>(),
-                           &st->count, f->porder.as<uint32_t>(), t, f->L.as<float>(), f->pinv.as<uint32_t>());
+                           &st->count, f->porder.as<uint32_t>(), t, f->ev->trig, f->L.as<float>(),
+                           f->pinv.as<uint32_t>());
     } else {
         hipLaunchKernelGGL(k_pose, dim3(g), dim3(256), 0, stream, f->particles.as<tsl_particle>(), &st->count,
-                           (uint32_t)P, t, f->L.as<float>());
+                           (uint32_t)P, t, f->ev->trig, f->L.as<float>());
     }
     TSL_HIP_CHECK(hipGetLastError());
     f->last_order = tile.order;
@@ -1079,8 +1101,8 @@ int launch_trace(tsl_filter* f, hipStream_t stream, void* out_device, int values
     const size_t P = f->count_bound;
     const size_t N = f->num_points;
     if (P == 0 || N == 0) return TSL_OK;
-    TSL_TRY(launch_pose_matrices(f->particles.as<tsl_particle>(), (uint32_t)P, f->tf, f->L.as<float>(),
-                                 stream));
+    TSL_TRY(launch_pose_matrices(f->particles.as<tsl_particle>(), (uint32_t)P, f->tf, f->ev->trig,
+                                 f->L.as<float>(), stream));
     const Tile tile = choose_tile(f, P, N);
     if (values) return launch_eval_kernel<2>(tile, f, P, N, stream, nullptr, out_device);
     return launch_eval_kernel<1>(tile, f, P, N, stream, nullptr, out_device);

@@ -22,7 +22,7 @@ __global__ void k_motion_m(const Member* __restrict__ members, const tsl_motion*
 #pragma unroll
     for (int k = 0; k < 6; ++k) d[k] = motion_normal(mo.seed, i, (uint32_t)k) * mo.stddev[k] + mo.mean[k];
     float out[6];
-    apply_motion(p.first, d, out);
+    apply_motion(p.first, d, out, m.trig);
 #pragma unroll
     for (int k = 0; k < 6; ++k) p.first[k] = out[k];
     m.particles[i] = p;

@@ -185,11 +185,17 @@ __global__ void k_bucket_scatter(const uint32_t* __restrict__ count, const uint3
 // ---- scan reduction (TSDFEvaluator::evaluateParticles,
 // src/evaluation/tsdf_evaluator.cpp:304-376) ---------------------------------
 
-// keep[j]: the reference skips points closer than 1 m (sqrtf, :316-322)
-__global__ void k_red_keep(const tsl_point* __restrict__ raw, uint32_t n, uint32_t* __restrict__ keep)
+// keep[j]: the reference skips points closer than 1 m (sqrtf, :316-322);
+// the cloud reduction of CudaEvaluator keeps every point
+// (src/cuda/cuda_evaluator.cu:84-91)
+__global__ void k_red_keep(const tsl_point* __restrict__ raw, uint32_t n, int mode, uint32_t* __restrict__ keep)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
+    if (mode == kReduceCloud) {
+        keep[j] = 1u;
+        return;
+    }
     const tsl_point p = raw[j];
     const float dist = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
     keep[j] = (dist < 1.0f) ? 0u : 1u;
@@ -198,11 +204,16 @@ __global__ void k_red_keep(const tsl_point* __restrict__ raw, uint32_t n, uint32
 // Kept point j becomes entry k = kidx[j] (its index in the reference); it
 // reads ring[k] because the reference's ring iterator only advances on kept
 // points (:319-331).  Entries past the kept count are padded with ring 255.
-__global__ void k_red_build(const tsl_point* __restrict__ raw, const int16_t* __restrict__ ring, uint32_t n,
-                            const uint32_t* __restrict__ keep, const uint32_t* __restrict__ kidx, float cell,
-                            float half, uint64_t* __restrict__ key_a, uint32_t* __restrict__ ring_out,
-                            uint32_t* __restrict__ cx_out, tsl_point* __restrict__ orig, uint32_t* __restrict__ vals,
-                            uint32_t* __restrict__ bad)
+// ring_out is the dedup part of the second sort key, ring_ord the order key.
+// Cloud mode (cuda_evaluator.cu:78-116): every point kept, rings are int32
+// (the reference reads the field as int) used for the order only, the
+// dedup key is the centre alone, computed in double with the literal
+// 0.064 m cell (:95-97), and the centre itself is the reduced point.
+__global__ void k_red_build(const tsl_point* __restrict__ raw, const void* __restrict__ ring_any, uint32_t n,
+                            int mode, const uint32_t* __restrict__ keep, const uint32_t* __restrict__ kidx,
+                            float cell, float half, uint64_t* __restrict__ key_a, uint32_t* __restrict__ ring_out,
+                            uint32_t* __restrict__ ring_ord, uint32_t* __restrict__ cx_out,
+                            tsl_point* __restrict__ orig, uint32_t* __restrict__ vals, uint32_t* __restrict__ bad)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -211,18 +222,33 @@ __global__ void k_red_build(const tsl_point* __restrict__ raw, const int16_t* __
     if (j >= kept_total) {  // padding entry (filled by the thread of the same slot)
         key_a[j] = ~0ull;
         ring_out[j] = 255u;
+        ring_ord[j] = 0xffffffffu;
         cx_out[j] = 0xffffffffu;
     }
     if (!keep[j]) return;
     const uint32_t k = kidx[j];
     const tsl_point p = raw[j];
-    const int r = ring[k];
+    if (mode == kReduceCloud) {
+        const int r = static_cast<const int32_t*>(ring_any)[k];
+        const double c = 0.064, h = 0.032;
+        const float cx = (float)(floor((double)p.x / c) * c + h);
+        const float cy = (float)(floor((double)p.y / c) * c + h);
+        const float cz = (float)(floor((double)p.z / c) * c + h);
+        key_a[k] = ((uint64_t)__float_as_uint(cy) << 32) | __float_as_uint(cz);
+        ring_out[k] = 0u;
+        ring_ord[k] = (uint32_t)r ^ 0x80000000u;  // signed order
+        cx_out[k] = __float_as_uint(cx);
+        orig[k] = tsl_point{cx, cy, cz};
+        return;
+    }
+    const int r = static_cast<const int16_t*>(ring_any)[k];
     if (r < 0 || r >= 64) atomicOr(bad, 1u);
     const float cx = floorf(p.x / cell) * cell + half;
     const float cy = floorf(p.y / cell) * cell + half;
     const float cz = floorf(p.z / cell) * cell + half;
     key_a[k] = ((uint64_t)__float_as_uint(cy) << 32) | __float_as_uint(cz);
     ring_out[k] = (uint32_t)(r & 0xff);
+    ring_ord[k] = (uint32_t)(r & 0xff);
     cx_out[k] = __float_as_uint(cx);
     orig[k] = p;
 }
@@ -237,7 +263,8 @@ __global__ void k_red_key_b(const uint32_t* __restrict__ perm, const uint32_t* _
     key_b[t] = ((uint64_t)ring[e] << 32) | cx[e];
 }
 
-// first entry of each (ring, centre) group wins; NaN centres never match
+// first entry of each (ring, centre) group wins; NaN centres never match;
+// winners are ordered by (ring_ord, entry)
 __global__ void k_red_mark(const uint32_t* __restrict__ perm, const uint64_t* __restrict__ key_a,
                            const uint64_t* __restrict__ key_b_sorted, const uint32_t* __restrict__ ring,
                            const uint32_t* __restrict__ cx, const uint32_t* __restrict__ keep,
@@ -258,7 +285,7 @@ __global__ void k_red_mark(const uint32_t* __restrict__ perm, const uint64_t* __
         w = t == 0 || nan || key_b_sorted[t] != key_b_sorted[t - 1] || ka != key_a[perm[t - 1]];
     }
     win[t] = w ? 1u : 0u;
-    out_key[t] = w ? (((uint64_t)ring[e] << 32) | e) : ~0ull;
+    out_key[t] = w ? (((uint64_t)ring[e] << 32) | e) : ~0ull;  // ring = ring_ord here
 }
 
 __global__ void k_red_order(const uint64_t* __restrict__ sorted_key, uint32_t n, uint32_t* __restrict__ order)
@@ -281,13 +308,13 @@ __global__ void k_red_gather(const uint32_t* __restrict__ order, const tsl_point
 // Reduce a raw scan on the device: writes the reduced points to out (room
 // for n) and their number to *count_dev, and 1 to *bad_dev if a kept point's
 // ring is outside [0, 64).  Scratch buffers belong to the caller's filter.
-int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring, uint32_t n, float cell,
+int launch_reduce_scan(tsl_filter* f, int mode, const tsl_point* raw, const void* ring, uint32_t n, float cell,
                        tsl_point* out, uint32_t* count_dev, uint32_t* bad_dev, hipStream_t stream)
 {
     if (n == 0) return TSL_OK;
     const float half = cell / 2;
     DevBuf& sc = f->red_scratch;
-    // layout: keep, kidx, ring, cx, vals, perm1, perm2, win, order (uint32 x n each),
+    // layout: keep, kidx, ring, ring_ord, cx, vals, perm1, perm2, win, order (uint32 x n each),
     //         key_a, key_b, key_b_sorted, out_key, out_key_sorted (uint64 x n each), orig (tsl_point x n)
     const size_t u32 = sizeof(uint32_t) * (size_t)n, u64 = sizeof(uint64_t) * (size_t)n;
     size_t t_scan = 0, t_sort = 0, t_sum = 0, t_keys = 0;
@@ -299,7 +326,7 @@ int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring,
                                                      (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 64, stream));
     TSL_HIP_CHECK(hipcub::DeviceReduce::Sum(nullptr, t_sum, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, stream));
     const size_t temp = std::max(std::max(t_scan, t_keys), std::max(t_sort, t_sum));
-    const size_t total = 9 * u32 + 5 * u64 + sizeof(tsl_point) * (size_t)n + temp + 16 * 256;
+    const size_t total = 10 * u32 + 5 * u64 + sizeof(tsl_point) * (size_t)n + temp + 17 * 256;
     TSL_TRY(sc.reserve(total));
     char* base = sc.as<char>();
     auto take = [&](size_t bytes) {
@@ -310,6 +337,7 @@ int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring,
     uint32_t* keep = (uint32_t*)take(u32);
     uint32_t* kidx = (uint32_t*)take(u32);
     uint32_t* ring_e = (uint32_t*)take(u32);
+    uint32_t* ring_ord = (uint32_t*)take(u32);
     uint32_t* cx = (uint32_t*)take(u32);
     uint32_t* vals = (uint32_t*)take(u32);
     uint32_t* perm1 = (uint32_t*)take(u32);
@@ -325,12 +353,12 @@ int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring,
     void* tmp = take(temp);
     if ((size_t)(base - sc.as<char>()) > sc.bytes) return set_error(TSL_ERR_HIP, "scan reduction scratch layout");
     const unsigned g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_red_keep, dim3(g), dim3(256), 0, stream, raw, n, keep);
+    hipLaunchKernelGGL(k_red_keep, dim3(g), dim3(256), 0, stream, raw, n, mode, keep);
     TSL_HIP_CHECK(hipGetLastError());
     size_t tb = temp;
     TSL_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, keep, kidx, (int)n, stream));
-    hipLaunchKernelGGL(k_red_build, dim3(g), dim3(256), 0, stream, raw, ring, n, keep, kidx, cell, half, key_a, ring_e,
-                       cx, orig, vals, bad_dev);
+    hipLaunchKernelGGL(k_red_build, dim3(g), dim3(256), 0, stream, raw, ring, n, mode, keep, kidx, cell, half, key_a,
+                       ring_e, ring_ord, cx, orig, vals, bad_dev);
     TSL_HIP_CHECK(hipGetLastError());
     // pass A: (y centre, z centre), stable; pass B: (ring, x centre), stable
     tb = temp;
@@ -339,8 +367,8 @@ int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring,
     TSL_HIP_CHECK(hipGetLastError());
     tb = temp;
     TSL_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key_b, key_b_sorted, perm1, perm2, (int)n, 0, 64, stream));
-    hipLaunchKernelGGL(k_red_mark, dim3(g), dim3(256), 0, stream, perm2, key_a, key_b_sorted, ring_e, cx, keep, kidx, n,
-                       out_key, win);
+    hipLaunchKernelGGL(k_red_mark, dim3(g), dim3(256), 0, stream, perm2, key_a, key_b_sorted, ring_ord, cx, keep, kidx,
+                       n, out_key, win);
     TSL_HIP_CHECK(hipGetLastError());
     tb = temp;
     TSL_HIP_CHECK(hipcub::DeviceReduce::Sum(tmp, tb, win, count_dev, (int)n, stream));

@@ -237,6 +237,7 @@ int dispatch(tsl_server* s, Slot* sl)
         h.u0_dev = sl->u0.as<float>() + i;
         if (f->capacity >= kSpatialOrderMin) h.inv = f->pinv.as<uint32_t>();
         std::memcpy(h.tf, f->tf, sizeof(h.tf));
+        h.trig = f->ev->trig;
         h.N = (uint32_t)np;
         h.chunk_len = chunk;
         h.chunks = chunks;

@@ -112,17 +112,161 @@ TSL_HD void sincosf_exact(float xf, float* s_out, float* c_out)
 }
 
 // ---------------------------------------------------------------------------
+// sin / cos of a float angle as glibc (2.28 and later) computes sinf / cosf.
+// The reference's CPU evaluator calls unqualified sin(float) / cos(float)
+// (tsdf_evaluator.cpp:108-113, 206-213), which libstdc++ resolves to
+// std::sin(float), i.e. the C library's sinf (gcc may fuse a sin/cos pair
+// into sincosf, which gives the same bits).  glibc's sinf/cosf is the
+// algorithm of ARM's optimized-routines (Szabolcs Nagy, 2018): evaluate in
+// double; |x| < 0.75 (top 12 bits below those of pi/4): odd/even polynomial
+// directly (|x| < 2^-12: sin = x, cos = 1); |x| < 120: n = round(x 2/pi)
+// from a 2^24-scaled product, r = x - n pi/2; otherwise Payne-Hanek
+// reduction by 2/pi from a table of 24-bit windows of its bits; then a
+// degree-7 sin or degree-8 cos polynomial on r, chosen and signed by n mod 4,
+// rounded once to float.  On x86-64 glibc picks at load time between a build
+// of that C code with FMA contraction (CPUs with FMA3) and one without; the
+// two differ on 34 of the 2^32 inputs.  kFma selects the variant; both are
+// checked bit for bit against glibc on every non-NaN float.
+// ---------------------------------------------------------------------------
+enum TrigMode : int {
+    kTrigGlibcFma = 0,    // glibc sinf/cosf on x86-64 with FMA3 (the default)
+    kTrigGlibcNoFma = 1,  // glibc sinf/cosf on x86-64 without FMA3
+    kTrigDouble = 2,      // (float)sin((double)x): sincosf_exact
+};
+
+template <bool kFma>
+TSL_HD double glibc_mad(double a, double b, double c)
+{
+    if (kFma) return __builtin_fma(a, b, c);
+    return a * b + c;
+}
+
+TSL_HD uint32_t f32_top12(float f)
+{
+    union { float f; uint32_t u; } v;
+    v.f = f;
+    return (v.u >> 20) & 0x7ffu;
+}
+
+// polynomial on the reduced argument: n even -> sin, odd -> cos; neg picks
+// the cos coefficients negated (n & 2 in the caller)
+template <bool kFma>
+TSL_HD float glibc_trig_poly(double x, double x2, bool neg, int n)
+{
+    if ((n & 1) == 0) {
+        const double s1 = -0x1.555545995a603p-3, s2 = 0x1.1107605230bc4p-7, s3 = -0x1.994eb3774cf24p-13;
+        const double x3 = x * x2;
+        const double t = glibc_mad<kFma>(x2, s3, s2);
+        const double x7 = x3 * x2;
+        const double s = glibc_mad<kFma>(x3, s1, x);
+        return (float)glibc_mad<kFma>(x7, t, s);
+    }
+    const double sg = neg ? -1.0 : 1.0;
+    const double c0 = sg * 0x1p0, c1 = sg * -0x1.ffffffd0c621cp-2, c2 = sg * 0x1.55553e1068f19p-5,
+                 c3 = sg * -0x1.6c087e89a359dp-10, c4 = sg * 0x1.99343027bf8c3p-16;
+    const double x4 = x2 * x2;
+    const double t2 = glibc_mad<kFma>(x2, c4, c3);
+    const double t1 = glibc_mad<kFma>(x2, c1, c0);
+    const double x6 = x4 * x2;
+    const double c = glibc_mad<kFma>(x4, c2, t1);
+    return (float)glibc_mad<kFma>(x6, t2, c);
+}
+
+// 2/pi in hex: 0.A2F9836E 4E441529 FC2757D1 F534DDC0 DB629599 3C439041 ...;
+// word i is the 32 bits ending 8(i + 1) bits after the point
+TSL_HD uint32_t two_over_pi_word(int i)
+{
+    const uint32_t b[7] = {0u, 0xa2f9836eu, 0x4e441529u, 0xfc2757d1u, 0xf534ddc0u, 0xdb629599u, 0x3c439041u};
+    const int end = 8 * (i + 1);         // bit position after the point
+    const int w = end >> 5, sh = end & 31;  // the window is the low bits of word w plus word w-1's
+    const uint64_t hi = b[w], lo = (w + 1 < 7) ? b[w + 1] : 0u;
+    const uint64_t two = (hi << 32) | lo;
+    return (uint32_t)(two >> (32 - sh));
+}
+
+// Payne-Hanek reduction of |x| >= 120: r in double and the quadrant n
+TSL_HD double glibc_reduce_large(uint32_t xi, int* np)
+{
+    const int base = (int)((xi >> 26) & 15);
+    const int shift = (int)((xi >> 23) & 7);
+    xi = (xi & 0xffffffu) | 0x800000u;
+    xi <<= shift;
+    uint64_t r0 = (uint32_t)(xi * two_over_pi_word(base));
+    const uint64_t r1 = (uint64_t)xi * two_over_pi_word(base + 4);
+    const uint64_t r2 = (uint64_t)xi * two_over_pi_word(base + 8);
+    r0 = (r2 >> 32) | (r0 << 32);
+    r0 += r1;
+    const uint64_t n = (r0 + (1ull << 61)) >> 62;
+    r0 -= n << 62;
+    *np = (int)n;
+    return (double)(int64_t)r0 * 0x1.921fb54442d18p-62;
+}
+
+template <bool kFma>
+TSL_HD void sincosf_glibc(float y, float* s_out, float* c_out)
+{
+    double x = (double)y;
+    const uint32_t top = f32_top12(y);
+    if (top < 0x3f4u) {  // |y| < 0.75
+        const double x2 = x * x;
+        if (top < 0x398u) {  // |y| < 2^-12
+            *s_out = y;
+            *c_out = 1.0f;
+            return;
+        }
+        *s_out = glibc_trig_poly<kFma>(x, x2, false, 0);
+        *c_out = glibc_trig_poly<kFma>(x, x2, false, 1);
+        return;
+    }
+    int n;
+    double sgn;
+    bool neg;
+    if (top < 0x42fu) {  // |y| < 120
+        const double r = x * 0x1.45f306dc9c883p+23;
+        n = (int)(((int32_t)r + 0x800000) >> 24);
+        x = glibc_mad<kFma>(-(double)n, 0x1.921fb54442d18p0, x);
+        sgn = ((n & 3) == 1 || (n & 3) == 2) ? -1.0 : 1.0;
+        neg = (n & 2) != 0;
+    } else if (top < 0x7f8u) {
+        union { float f; uint32_t u; } v;
+        v.f = y;
+        const int sign = (int)(v.u >> 31);
+        x = glibc_reduce_large(v.u, &n);
+        const int q = (n + sign) & 3;
+        sgn = (q == 1 || q == 2) ? -1.0 : 1.0;
+        neg = (q & 2) != 0;
+    } else {  // inf or NaN
+        *s_out = (y - y) / (y - y);
+        *c_out = *s_out;
+        return;
+    }
+    const double xs = x * sgn;
+    *s_out = glibc_trig_poly<kFma>(xs, x * x, neg, n);
+    *c_out = glibc_trig_poly<kFma>(xs, x * x, neg, n ^ 1);
+}
+
+TSL_HD void sincosf_mode(int mode, float x, float* s, float* c)
+{
+    if (mode == kTrigGlibcNoFma)
+        sincosf_glibc<false>(x, s, c);
+    else if (mode == kTrigDouble)
+        sincosf_exact(x, s, c);
+    else
+        sincosf_glibc<true>(x, s, c);
+}
+
+// ---------------------------------------------------------------------------
 // Pose matrix of a particle composed with the scanner->base transform
 // (tsdf_evaluator.cpp:102-145): L = [R | t] * T, R = Rz(yaw) Ry(pitch)
 // Rx(roll), left-to-right float evaluation exactly as the reference writes it.
 // pose = (x, y, z, roll, pitch, yaw); tf = row-major, entries 0..11 used.
 // ---------------------------------------------------------------------------
-TSL_HD void pose_matrix(const float* pose, const float* tf, float* L)
+TSL_HD void pose_matrix(const float* pose, const float* tf, float* L, int trig = kTrigGlibcFma)
 {
     float sa, ca, sb, cb, sg, cg;
-    sincosf_exact(pose[3], &sa, &ca);
-    sincosf_exact(pose[4], &sb, &cb);
-    sincosf_exact(pose[5], &sg, &cg);
+    sincosf_mode(trig, pose[3], &sa, &ca);
+    sincosf_mode(trig, pose[4], &sb, &cb);
+    sincosf_mode(trig, pose[5], &sg, &cg);
     float p[12];
     p[0] = cb * cg;
     p[4] = cb * sg;
@@ -180,7 +324,7 @@ TSL_HD double motion_normal(uint64_t seed, uint64_t particle, uint32_t draw)
 // rotated into the particle's frame (translation added, rotation composed)
 // and the new angles are read back as getAngleFromMat does
 // (src/util/util.cpp:86-111).  Float arithmetic left to right, no FMA.
-TSL_HD void apply_motion(const float last[6], const double d[6], float out[6])
+TSL_HD void apply_motion(const float last[6], const double d[6], float out[6], int trig = kTrigGlibcFma)
 {
     const float sin_roll = (float)sin(d[3]), cos_roll = (float)cos(d[3]);
     const float sin_pitch = (float)sin(d[4]), cos_pitch = (float)cos(d[4]);
@@ -199,9 +343,9 @@ TSL_HD void apply_motion(const float last[6], const double d[6], float out[6])
     o[10] = cos_roll * cos_pitch;
     o[11] = (float)d[2];
     float sa, ca, sb, cb, sg, cg;
-    sincosf_exact(last[3], &sa, &ca);
-    sincosf_exact(last[4], &sb, &cb);
-    sincosf_exact(last[5], &sg, &cg);
+    sincosf_mode(trig, last[3], &sa, &ca);
+    sincosf_mode(trig, last[4], &sb, &cb);
+    sincosf_mode(trig, last[5], &sg, &cg);
     float p[12];
     p[0] = cb * cg;
     p[4] = cb * sg;

@@ -229,7 +229,8 @@ struct Member {
     uint32_t N, chunk_len, chunks, stride;
     uint32_t cap_limit;        // room of particles/alt/ubuf for resampling output
     uint32_t snapshot_count;
-    uint32_t pad[2];
+    int32_t trig;              // TrigMode of the pose matrices and pose sums
+    uint32_t pad;
 };
 
 constexpr uint32_t kMaxMembers = 1024;  // filters in one batch
@@ -280,6 +281,7 @@ struct tsl_evaluator {
     int per_point = 0;
     float a_hit = 0.9f, a_range = 0.1f, a_max = 0.0f, max_range = 100.0f;
     float inv_max_range = 0.01f, max_range_squared = 10000.0f, range_term = 0.0f;
+    int trig = tsl::kTrigGlibcFma;  // sin/cos of the particle angles (TrigMode)
     tsl_filter* scratch = nullptr;  // backs the host-buffer entry points
 };
 
@@ -353,8 +355,10 @@ struct Tile {
     uint32_t chunk_len;
     uint32_t chunks;
 };
-int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const float tf[16],
+int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const float tf[16], int trig,
                          float* d_out, hipStream_t stream);
+// sin and cos of n angles on the device in a TrigMode.
+int launch_trig(const float* d_x, uint32_t n, int mode, float* d_s, float* d_c, hipStream_t stream);
 int launch_prep_points(tsl_filter* f, hipStream_t stream);
 // Buffers of the spatial particle order for f->count_bound particles.
 int reserve_particle_order(tsl_filter* f);
@@ -377,7 +381,13 @@ int launch_prep_points_batch(const tsl_evaluator* ev, const tsl_point* pts, cons
                              void* temp, size_t temp_bytes, hipStream_t stream);
 int launch_particle_order(tsl_filter* f, hipStream_t stream);
 int filter_reserve_scan(tsl_filter* f, size_t n);
-int launch_reduce_scan(tsl_filter* f, const tsl_point* raw, const int16_t* ring, uint32_t n, float cell,
+// Scan reductions: kReduceEvaluator is TSDFEvaluator::evaluateParticles
+// (src/evaluation/tsdf_evaluator.cpp:304-376; ring int16), kReduceCloud the
+// centre reduction of CudaEvaluator::evaluate(PointCloud2)
+// (src/cuda/cuda_evaluator.cu:78-116; ring int32, cell fixed at 0.064 m).
+constexpr int kReduceEvaluator = 0;
+constexpr int kReduceCloud = 1;
+int launch_reduce_scan(tsl_filter* f, int mode, const tsl_point* raw, const void* ring, uint32_t n, float cell,
                        tsl_point* out, uint32_t* count_dev, uint32_t* bad_dev, hipStream_t stream);
 
 // Resampling (resample.hip): members[0..n) resampled in one sequence of

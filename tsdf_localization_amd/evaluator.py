@@ -16,7 +16,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import MapCoef, Motion, Pose, check, lib, ptr
+from ._lib import TRIG_DOUBLE, TRIG_GLIBC_FMA, TRIG_GLIBC_NOFMA, MapCoef, Motion, Pose, check, lib, ptr  # noqa: F401
 
 
 def _particles(a):
@@ -150,13 +150,18 @@ class TSDFEvaluator:
     """Sensor update with a TSDF map (tsdf_evaluator.h:72-115)."""
 
     def __init__(self, map_ptr, per_point=False, a_hit=0.9, a_range=0.1, a_max=0.0, max_range=100.0,
-                 reduction_cell_size=0.064):
+                 reduction_cell_size=0.064, trig=TRIG_GLIBC_FMA):
+        """trig: sin/cos of the particle angles, TRIG_GLIBC_FMA (default: the
+        C library's sinf/cosf the reference CPU evaluator calls, as glibc
+        computes them on x86-64 with FMA3), TRIG_GLIBC_NOFMA or TRIG_DOUBLE."""
         self.map = map_ptr
         self._dev_map = map_ptr.device_map()
         self.reduction_cell_size = reduction_cell_size
         self.handle = ctypes.c_void_p()
         check(lib.tsl_evaluator_create(self._dev_map.handle, int(bool(per_point)), float(a_hit), float(a_range),
                                        float(a_max), float(max_range), ctypes.byref(self.handle)))
+        check(lib.tsl_evaluator_set_trig(self.handle, int(trig)))
+        self.trig = int(trig)
 
     def evaluate(self, particles, points, tf_matrix, use_cuda=True):
         """TSDFEvaluator::evaluate: writes normalised weights into particles[:, 6]
@@ -172,6 +177,25 @@ class TSDFEvaluator:
         pose = Pose()
         check(lib.tsl_evaluate(self.handle, ptr(particles), particles.shape[0], ptr(pts), pts.shape[0],
                                ptr(tf), ctypes.byref(pose)))
+        return _pose_dict(pose)
+
+    def evaluate_cloud(self, particles, points, rings, tf_matrix):
+        """Evaluator::evaluate(particles, PointCloud2, tf) of the reference's
+        CUDA evaluator (cuda_evaluator.cu:78-116): the cloud's points reduced
+        to distinct 0.064 m voxel centres on the device, then evaluated as
+        evaluate(); weights normalised in place, pose returned."""
+        if not isinstance(particles, np.ndarray) or particles.dtype != np.float32 or \
+                not particles.flags.c_contiguous:
+            raise ValueError("particles must be a C-contiguous float32 (P, 7) array (updated in place)")
+        _particles(particles)
+        pts = _points(points)
+        r = np.ascontiguousarray(rings, np.int32).reshape(-1)
+        if r.size != pts.shape[0]:
+            raise ValueError("one ring number per point")
+        tf = _tf(tf_matrix)
+        pose = Pose()
+        check(lib.tsl_evaluate_cloud(self.handle, ptr(particles), particles.shape[0], ptr(pts), ptr(r),
+                                     pts.shape[0], ptr(tf), ctypes.byref(pose)))
         return _pose_dict(pose)
 
     def evaluate_raw(self, particles, points, tf_matrix):
@@ -207,13 +231,32 @@ class TSDFEvaluator:
             self.handle = None
 
 
-def pose_matrices(particles, tf_matrix):
+def pose_matrices(particles, tf_matrix, trig=TRIG_GLIBC_FMA):
     """Pose matrices L (P, 12) computed on the device (tsdf_evaluator.cpp:102-145)."""
     p = _particles(particles)
     tf = _tf(tf_matrix)
     out = np.empty((p.shape[0], 12), np.float32)
-    check(lib.tsl_pose_matrices(ptr(p), p.shape[0], ptr(tf), ptr(out)))
+    check(lib.tsl_pose_matrices(ptr(p), p.shape[0], ptr(tf), int(trig), ptr(out)))
     return out
+
+
+def trig(angles, mode=TRIG_GLIBC_FMA, on_device=True):
+    """(sin, cos) of float32 angles in a trig mode, on the device or by the
+    host build of the same code."""
+    x = np.ascontiguousarray(angles, np.float32).reshape(-1)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    check(lib.tsl_trig(int(mode), ptr(x), x.size, ptr(s), ptr(c), int(bool(on_device))))
+    return s, c
+
+
+def check_trig_libm(mode, stride):
+    """Host build of a trig mode against this process's sinf/cosf over every
+    non-NaN float whose bit pattern is a multiple of stride: (mismatches, checked)."""
+    bad = ctypes.c_uint64()
+    n = ctypes.c_uint64()
+    check(lib.tsl_check_trig_libm(int(mode), int(stride), ctypes.byref(bad), ctypes.byref(n)))
+    return bad.value, n.value
 
 
 def resample_systematic(particles, u0):
@@ -340,6 +383,16 @@ class DeviceFilter:
             raise ValueError("one ring number per point")
         tf = _tf(tf_matrix)
         check(lib.tsl_filter_set_scan_raw(self.handle, ptr(pts), ptr(r), pts.shape[0], float(cell), ptr(tf)))
+
+    def set_scan_cloud(self, points, rings, tf_matrix):
+        """Raw cloud through CudaEvaluator's centre reduction
+        (cuda_evaluator.cu:78-116): distinct 0.064 m voxel centres."""
+        pts = _points(points)
+        r = np.ascontiguousarray(rings, np.int32).reshape(-1)
+        if r.size != pts.shape[0]:
+            raise ValueError("one ring number per point")
+        tf = _tf(tf_matrix)
+        check(lib.tsl_filter_set_scan_cloud(self.handle, ptr(pts), ptr(r), pts.shape[0], ptr(tf)))
 
     def get_scan(self):
         n = ctypes.c_size_t()
