@@ -264,15 +264,22 @@ ORC_EXPORT int orc_set_data(orc_coef* c, int32_t* grid_occ, const float* cell_xy
             return -1;
         grid_occ[up_index] = 0;
     }
-    int32_t current_offset = 0;
-    int32_t sub_size = (int32_t)(c->sub_dim * c->sub_dim * c->sub_dim);
+    /* :202-211 keep the running offset in an int: a map whose voxels do not
+     * fit in int32 overflows it (undefined); refused here instead (-2) */
+    uint64_t sub_size = c->sub_dim * c->sub_dim * c->sub_dim;
+    uint64_t blocks = 0;
+    for (uint64_t i = 0; i < c->grid_occ_size; ++i)
+        blocks += grid_occ[i] >= 0;
+    if (sub_size != 0 && blocks > (uint64_t)INT32_MAX / sub_size)
+        return -2;
+    int64_t current_offset = 0;
     for (uint64_t i = 0; i < c->grid_occ_size; ++i) {
         if (grid_occ[i] >= 0) {
-            grid_occ[i] = current_offset;
-            current_offset += sub_size;
+            grid_occ[i] = (int32_t)current_offset;
+            current_offset += (int64_t)sub_size;
         }
     }
-    c->data_size = (uint64_t)(int64_t)current_offset;
+    c->data_size = (uint64_t)current_offset;
     float* data = (float*)malloc(sizeof(float) * (c->data_size ? c->data_size : 1));
     for (uint64_t i = 0; i < c->data_size; ++i)
         data[i] = c->init_value;

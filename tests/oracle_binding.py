@@ -217,6 +217,8 @@ def set_data(min_xyz, max_xyz, res, init, cells_xyz, values):
     vals = np.ascontiguousarray(values, np.float32)
     out = ctypes.c_void_p()
     rc = olib.orc_set_data(ctypes.byref(c), _ptr(grid), _ptr(cells), _ptr(vals), cells.shape[0], ctypes.byref(out))
+    if rc == -2:
+        raise OverflowError("block offsets exceed int32")
     if rc != 0:
         raise RuntimeError("Upper voxel index overflow!")
     n = int(c.data_size)

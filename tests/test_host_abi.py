@@ -175,3 +175,38 @@ def test_mixed_workload_shapes_and_determinism():
     assert np.array_equal(g.particles, b.global_robots[0].particles)
     assert np.array_equal(a.tracking[0][1].points, b.tracking[0][1].points)
     assert np.array_equal(a.maps[0].tsdf_mm, b.maps[0].tsdf_mm)
+
+
+def _one_cell_per_block(nx, ny):
+    """Cells at the centre of every 1 m block of an nx x ny x 1 m map."""
+    xs, ys = np.meshgrid(np.arange(nx) + 0.5, np.arange(ny) + 0.5, indexing="ij")
+    return np.stack([xs.ravel(), ys.ravel(), np.full(xs.size, 0.5)], 1).astype(np.float32)
+
+
+def test_set_data_refuses_offsets_past_int32():
+    """At 0.01 m a block holds 100^3 voxels: 2,148 allocated blocks need
+    offsets up to 2.148e9 > 2^31 - 1, which the reference's int block table
+    (cuda_sub_voxel_map.h:13, .tcc:202-211) cannot hold; both the library and
+    the oracle refuse the map instead of overflowing.  2,147 blocks still
+    fit (data size 2.147e9, reported through the capacity query)."""
+    res = 0.01
+    cells = _one_cell_per_block(2148, 1)
+    vals = np.ones(cells.shape[0], np.float32)
+    c = tev.coef_init((0, 0, 0), (2148, 1, 1), res, 0.0)
+    assert c.sub_dim == 100
+    grid = np.empty(c.grid_occ_size, np.int32)
+    rc = _lib.lib.tsl_map_set_data(ctypes.byref(c), _lib.ptr(cells), _lib.ptr(vals), cells.shape[0],
+                                   _lib.ptr(grid), None, 0)
+    assert rc == _lib.TSL_ERR_UNSUPPORTED
+    assert b"int32" in _lib.lib.tsl_last_error()
+    assert np.all(grid == 0)  # blocks marked, no offset written
+    with pytest.raises(OverflowError):
+        ob.set_data((0, 0, 0), (2148, 1, 1), res, 0.0, cells, vals)
+    # one block fewer fits: the size query reports 2,147 x 10^6 voxels
+    c = tev.coef_init((0, 0, 0), (2147, 1, 1), res, 0.0)
+    grid = np.empty(c.grid_occ_size, np.int32)
+    rc = _lib.lib.tsl_map_set_data(ctypes.byref(c), _lib.ptr(cells[:2147]), _lib.ptr(vals[:2147]), 2147,
+                                   _lib.ptr(grid), None, 0)
+    assert rc == _lib.TSL_ERR_CAPACITY
+    assert c.data_size == 2147 * 10 ** 6
+    assert grid[-1] == 2146 * 10 ** 6

@@ -553,15 +553,25 @@ int tsl_map_set_data(tsl_map_coef* c, const float* cell_xyz, const float* cell_v
         if (up >= c->grid_occ_size) return set_error(TSL_ERR_INVALID_ARGUMENT, "Upper voxel index overflow!");
         grid_occ[up] = 0;
     }
-    int32_t offset = 0;
-    const int32_t sub_size = (int32_t)(c->sub_dim * c->sub_dim * c->sub_dim);
+    // the reference's offsets are int (OCC_T, cuda_sub_voxel_map.h:13) and so
+    // is its running offset (.tcc:202-211): a map whose voxels do not fit
+    // in int32 has no representation there and is refused before any
+    // offset is written
+    const uint64_t sub_size = c->sub_dim * c->sub_dim * c->sub_dim;
+    uint64_t blocks = 0;
+    for (uint64_t i = 0; i < c->grid_occ_size; ++i) blocks += grid_occ[i] >= 0;
+    if (sub_size != 0 && blocks > (uint64_t)INT32_MAX / sub_size)
+        return set_error(TSL_ERR_UNSUPPORTED,
+                         "%llu blocks of %llu voxels exceed the int32 block offsets of the reference's map",
+                         (unsigned long long)blocks, (unsigned long long)sub_size);
+    int64_t offset = 0;
     for (uint64_t i = 0; i < c->grid_occ_size; ++i) {
         if (grid_occ[i] >= 0) {
-            grid_occ[i] = offset;
-            offset += sub_size;
+            grid_occ[i] = (int32_t)offset;
+            offset += (int64_t)sub_size;
         }
     }
-    c->data_size = (uint64_t)(int64_t)offset;
+    c->data_size = (uint64_t)offset;
     if (!data || data_capacity < c->data_size)
         return set_error(TSL_ERR_CAPACITY, "data needs %llu floats", (unsigned long long)c->data_size);
     for (uint64_t i = 0; i < c->data_size; ++i) data[i] = c->init_value;
