@@ -222,6 +222,22 @@ int tsl_evaluate_raw(tsl_evaluator* ev, const tsl_particle* particles, size_t nu
                      const tsl_point* points, size_t num_points, const float tf_matrix[16],
                      float* weights_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tsl_evaluate_raw on device-resident inputs.
+ * particles (num_particles tsl_particle) and points (num_points tsl_point)
+ * are device pointers read in place; weights_out (num_particles floats,
+ * device) receives the raw weights, bit-identical to tsl_evaluate_raw's.
+ * Every intermediate lives in a fixed 64 MiB device array per GPU, which
+ * the library holds until the process ends; TSL_ERR_CAPACITY if the call
+ * needs more (about 52 B per particle, 60 B per point, plus 4 B per particle
+ * per scan chunk).  Enqueued on `stream` with no allocation, no host copy
+ * and no synchronisation.  Calls that overlap on one GPU share the array
+ * and race: serialise them (one stream, or events). */
+int tsl_evaluate_raw_noalloc(tsl_evaluator* ev, const tsl_particle* particles, size_t num_particles,
+                             const tsl_point* points, size_t num_points, const float tf_matrix[16],
+                             float* weights_out, void* stream);
+
 /* Voxel index that CudaSubVoxelMap::getIndex (cuda_sub_voxel_map.tcc:51-137)
  * returns for every (particle, point) pair, row-major [particle][point]
  * (coef.data_size = rejected).  Runs the production lookup on the device; for

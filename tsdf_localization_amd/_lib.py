@@ -99,6 +99,7 @@ _SIGS = {
     "tsl_evaluator_destroy": (_I, [_P]),
     "tsl_evaluate": (_I, [_P, _P, _S, _P, _S, _P, ctypes.POINTER(Pose)]),
     "tsl_evaluate_raw": (_I, [_P, _P, _S, _P, _S, _P, _P]),
+    "tsl_evaluate_raw_noalloc": (_I, [_P, _P, _S, _P, _S, _P, _P, _P]),
     "tsl_trace_indices": (_I, [_P, _P, _S, _P, _S, _P, _P]),
     "tsl_trace_values": (_I, [_P, _P, _S, _P, _S, _P, _P]),
     "tsl_check_magic_divisor": (_I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),

@@ -34,7 +34,8 @@ HIP_FLAGS = [
     "-Wno-unused-function",
 ]
 
-HIP_SOURCES = ["api.hip", "eval.hip", "resample.hip", "tables.hip", "scan.hip", "batch.hip", "motion.hip", "mcl.hip", "server.hip", "chunks.hip"]
+HIP_SOURCES = ["api.hip", "eval.hip", "resample.hip", "tables.hip", "scan.hip", "batch.hip", "motion.hip", "mcl.hip", "server.hip", "chunks.hip",
+               "noalloc.hip"]
 SYNTH_SOURCES = ["synth.cpp"]
 
 LIBTSL = os.path.join(HERE, "libtsl.so")

@@ -81,7 +81,7 @@ static FilterState* host_state(tsl_filter* f) { return static_cast<FilterState*>
 
 // Device descriptor of the filter, uploaded when any of its buffers or its
 // scan changed since the last upload.
-int filter_member(tsl_filter* f, Member** d_member, uint32_t chunk_len, uint32_t chunks)
+int filter_member(tsl_filter* f, Member** d_member, uint32_t chunk_len, uint32_t chunks, hipStream_t stream)
 {
     TSL_TRY(f->plan.reserve(sizeof(RsPlan)));
     TSL_TRY(f->member.reserve(sizeof(Member)));
@@ -111,7 +111,10 @@ int filter_member(tsl_filter* f, Member** d_member, uint32_t chunk_len, uint32_t
     h.snapshot_count = (uint32_t)f->snapshot_count;
     const size_t ucap = f->ubuf.bytes / sizeof(float);
     h.cap_limit = (uint32_t)std::min(f->capacity, ucap);
-    if (f->member_dirty || std::memcmp(&h, &f->member_host, sizeof(h)) != 0) {
+    if (f->member_on_stream) {
+        TSL_TRY(launch_store_member(h, f->member.as<Member>(), stream));
+        f->member_host = h;
+    } else if (f->member_dirty || std::memcmp(&h, &f->member_host, sizeof(h)) != 0) {
         TSL_HIP_CHECK(hipMemcpy(f->member.ptr, &h, sizeof(h), hipMemcpyHostToDevice));
         f->member_host = h;
         f->member_dirty = false;

@@ -1066,7 +1066,7 @@ int launch_update(tsl_filter* f, hipStream_t stream)
     const unsigned nb = (unsigned)((P + kBlock - 1) / kBlock);
     TSL_TRY(f->blocks.reserve(sizeof(double) * kReducePartials * (size_t)nb));
     Member* dm;
-    TSL_TRY(filter_member(f, &dm, tile.chunk_len, tile.chunks));
+    TSL_TRY(filter_member(f, &dm, tile.chunk_len, tile.chunks, stream));
     if (nb <= kFinishSmallSlices) {
         hipLaunchKernelGGL(k_finish_small, dim3(1), dim3(kBlock * kFinishGroups), 0, stream, dm, nb);
         TSL_HIP_CHECK(hipGetLastError());
@@ -1090,7 +1090,7 @@ int launch_raw(tsl_filter* f, hipStream_t stream)
     const unsigned nb = (unsigned)((P + kBlock - 1) / kBlock);
     TSL_TRY(f->blocks.reserve(sizeof(double) * kReducePartials * (size_t)nb));
     Member* dm;
-    TSL_TRY(filter_member(f, &dm, tile.chunk_len, tile.chunks));
+    TSL_TRY(filter_member(f, &dm, tile.chunk_len, tile.chunks, stream));
     hipLaunchKernelGGL(k_reduce_m, dim3(nb), dim3(kBlock), 0, stream, dm);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
@@ -1114,6 +1114,17 @@ int launch_restore(tsl_filter* f, hipStream_t stream)
     hipLaunchKernelGGL(k_restore, dim3((n + 255) / 256 ? (n + 255) / 256 : 1), dim3(256), 0, stream,
                        f->snapshot.as<tsl_particle>(), f->particles.as<tsl_particle>(), n,
                        f->state.as<FilterState>());
+    TSL_HIP_CHECK(hipGetLastError());
+    return TSL_OK;
+}
+
+namespace {
+__global__ void k_store_member(Member m, Member* __restrict__ dst) { *dst = m; }
+}  // namespace
+
+int launch_store_member(const Member& m, Member* dst, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_store_member, dim3(1), dim3(1), 0, stream, m, dst);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
 }

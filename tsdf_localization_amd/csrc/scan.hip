@@ -438,6 +438,14 @@ int filter_reserve_scan(tsl_filter* f, size_t n)
     return TSL_OK;
 }
 
+size_t prep_points_temp_bytes(uint32_t n)
+{
+    size_t temp = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                             (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 63, (hipStream_t)0);
+    return temp;
+}
+
 int launch_prep_points(tsl_filter* f, hipStream_t stream)
 {
     const uint32_t n = (uint32_t)f->num_points;

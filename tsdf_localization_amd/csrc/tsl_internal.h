@@ -45,19 +45,30 @@ void clear_error();
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool owned = true;  // false: a view of memory someone else holds; never allocated or freed here
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release()
     {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && owned) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
+    }
+    // Point at n bytes of caller-held memory (no allocation, no free).
+    void view(void* p, size_t n)
+    {
+        release();
+        ptr = p;
+        bytes = n;
+        owned = false;
     }
     int reserve(size_t n)
     {
         if (n <= bytes && ptr) return TSL_OK;
+        if (!owned)
+            return set_error(TSL_ERR_CAPACITY, "a fixed buffer of %zu bytes cannot hold %zu bytes", bytes, n);
         release();
         size_t want = n ? n : 256;
         hipError_t e = hipMalloc(&ptr, want);
@@ -334,6 +345,7 @@ struct tsl_filter {
     tsl::DevBuf member;      // Member (device descriptor of this filter)
     tsl::Member member_host{};  // last uploaded descriptor
     bool member_dirty = true;
+    bool member_on_stream = false;  // write the descriptor with a kernel on the launch stream, every time
     void* host_state = nullptr;  // pinned FilterState mirror
 };
 
@@ -342,6 +354,8 @@ namespace tsl {
 // Kernel launchers (eval.hip)
 int launch_update(tsl_filter* f, hipStream_t stream);
 int launch_raw(tsl_filter* f, hipStream_t stream);
+// Set the device particle count of a filter with a kernel on `stream`.
+int launch_set_count(tsl_filter* f, uint32_t n, hipStream_t stream);
 int launch_trace(tsl_filter* f, hipStream_t stream, void* out_device, int values);
 int launch_restore(tsl_filter* f, hipStream_t stream);
 int autotune_tile(tsl_filter* f, hipStream_t stream, int reps, int* best_order, int* best_k, int* best_p,
@@ -360,6 +374,8 @@ int launch_pose_matrices(const tsl_particle* d_particles, uint32_t n, const floa
 // sin and cos of n angles on the device in a TrigMode.
 int launch_trig(const float* d_x, uint32_t n, int mode, float* d_s, float* d_c, hipStream_t stream);
 int launch_prep_points(tsl_filter* f, hipStream_t stream);
+// Temporary storage the point sort of launch_prep_points needs for n points.
+size_t prep_points_temp_bytes(uint32_t n);
 // Buffers of the spatial particle order for f->count_bound particles.
 int reserve_particle_order(tsl_filter* f);
 // Spatial particle order of a filter and its pose matrices in that order
@@ -423,7 +439,11 @@ Tile choose_tile(const tsl_filter* f, size_t P, size_t N);
 
 // Upload the filter's device descriptor if it changed; chunk_len/chunks
 // describe the partial-weight layout of the last evaluation (0 = keep).
-int filter_member(tsl_filter* f, Member** d_member, uint32_t chunk_len = 0, uint32_t chunks = 0);
+int filter_member(tsl_filter* f, Member** d_member, uint32_t chunk_len = 0, uint32_t chunks = 0,
+                  hipStream_t stream = nullptr);
+// Write a Member descriptor to device memory with a one-thread kernel on
+// `stream` (ordered with the stream's other work, no host copy).
+int launch_store_member(const Member& m, Member* dst, hipStream_t stream);
 
 // Motion update of members[0..n) with motions[0..n) (motion.hip).
 int launch_motion_members(const Member* d_members, const tsl_motion* d_motions, uint32_t n, uint32_t grid_cap,
