@@ -61,10 +61,29 @@ def cpu_baseline(w, sample_particles, repeats, threads):
                       f"median of {repeats} after warm-up (both forms); oracle (C/OpenMP restatement of "
                       f"TSDFEvaluator::evaluate), {threads} threads",
             "faithful_8_threads_value": faithful,
-            "host_cpu": _cpu_model()}
+            "host_cpu": _cpu_model(), "host_cores": host_cores()}
+
+
+def host_cores():
+    """The host cores this process may use: its CPU affinity, the OpenMP
+    thread setting of the environment, and the machine's logical CPUs."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    omp = int(omp) if omp and omp.isdigit() else None
+    return {"affinity": aff, "omp_num_threads": omp, "logical_cpus": os.cpu_count(),
+            "used": min(aff, omp) if omp else aff}
 
 
 def _cpu_model():
+    """Model name as lscpu reports it (or /proc/cpuinfo)."""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
@@ -82,29 +101,45 @@ VALU_FILE = os.path.join(ROOT, "profiles", "k_eval_global_localisation_valu.md")
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 
-def measured_traffic():
+L2_FILE = os.path.join(ROOT, "profiles", "k_eval_global_localisation_l2.md")
+
+# the evaluation tile the default run uses (the autotuner's pick on this
+# workload); the counter profiles under profiles/ are taken with it
+PINNED_TILE = {"order": 1, "k": 1, "p": 2, "chunk": 0, "lookup": -1}
+
+
+def tile_key(t):
+    return f"order={t['order']} k={t['k']} p={t['p']} chunk={t['chunk']} lookup={t['lookup']}"
+
+
+def measured_traffic(tile):
     """HBM bytes per evaluation-kernel launch from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes (scaled by the plain-copy calibration),
-    or None if that summary is absent."""
-    return _profile_value(TRAFFIC_FILE, "traffic_bytes_per_launch")
+    or None if that summary is absent or was taken with another tile."""
+    return _profile_value(TRAFFIC_FILE, "traffic_bytes_per_launch", tile)
 
 
-def _profile_value(path, key):
+def _profile_value(path, key, tile):
+    """A value recorded in a profile summary, if the summary's tile line
+    names the tile of this run."""
+    found, same_tile = None, False
     try:
         with open(path) as fh:
             for line in fh:
+                if line.startswith("tile:"):
+                    same_tile = line.split(":", 1)[1].strip() == tile_key(tile)
                 if line.startswith(key + ":"):
-                    return float(line.split(":", 1)[1])
+                    found = float(line.split(":", 1)[1])
     except OSError:
-        pass
-    return None
+        return None
+    return found if same_tile else None
 
 
-def valu_issue(kern_avg_ms):
+def valu_issue(kern_avg_ms, tile):
     """Issue-rate view of the evaluation kernel: VALU instructions per
     launch from the committed rocprofv3 SQ_INSTS_VALU pass (same workload and
     tile), x 64 lanes, over the measured launch time, against the VALU peak."""
-    instr = _profile_value(VALU_FILE, "valu_instructions_per_launch")
+    instr = _profile_value(VALU_FILE, "valu_instructions_per_launch", tile)
     if instr is None or kern_avg_ms <= 0:
         return None
     achieved = instr * 64 / (kern_avg_ms / 1000.0)
@@ -113,17 +148,51 @@ def valu_issue(kern_avg_ms):
             "source": "profiles/k_eval_global_localisation_valu.md"}
 
 
+def gather_probe(sizes_mib=(128, 16384), reps=5):
+    """Random-line gather ceiling (tsl_probe_gather) for buffers the size of
+    a map's code array (L2 + Infinity Cache) and of many GB (HBM)."""
+    import ctypes
+    from tsdf_localization_amd._lib import check, lib
+    out = {}
+    for mib in sizes_mib:
+        lps, ms = ctypes.c_double(), ctypes.c_double()
+        check(lib.tsl_probe_gather(ctypes.c_uint64(mib << 20), reps, ctypes.byref(lps), ctypes.byref(ms)))
+        out[f"{mib}MiB"] = lps.value
+        log(f"gather probe {mib} MiB: {lps.value:.3e} lines/s ({ms.value:.2f} ms per launch)")
+    return out
+
+
+def l2_view(kern_avg_ms, tile, probe):
+    """Gather view of the evaluation kernel: L1->L2 read requests per launch
+    and per vector-memory read instruction (rocprofv3 TCP_TCC_READ_REQ_sum,
+    SQ_INSTS_VMEM_RD), and the 128-byte lines it misses in L2 (calibrated
+    FETCH_SIZE / 128) per second against the random-line probe on a buffer
+    of the map's size (lines that miss L2 and are served by the Infinity
+    Cache), same workload and tile."""
+    req = _profile_value(L2_FILE, "l2_read_requests_per_launch", tile)
+    instr = _profile_value(L2_FILE, "vmem_read_instructions_per_launch", tile)
+    lines = _profile_value(TRAFFIC_FILE, "fetch_lines_per_launch", tile)
+    if req is None or lines is None or kern_avg_ms <= 0:
+        return None
+    sec = kern_avg_ms / 1000.0
+    peak = probe.get("128MiB") if probe else None
+    return {"l2_read_requests_per_launch": req, "l2_requests_per_s": req / sec,
+            "l2_requests_per_vmem_instruction": req / instr if instr else None,
+            "l2_miss_lines_per_launch": lines, "l2_miss_lines_per_s": lines / sec,
+            "probe_lines_per_s": peak, "frac_of_probe": (lines / sec) / peak if peak else None,
+            "probe_lines_per_s_hbm": probe.get("16384MiB") if probe else None,
+            "source": "profiles/k_eval_global_localisation_l2.md, profiles/k_eval_global_localisation_traffic.md"}
+
+
 def copy_calibration(mib, reps):
-    """Plain device-to-device copies of known size, for scaling the
-    FETCH_SIZE / WRITE_SIZE counters: each copy reads and writes mib MiB."""
-    import torch
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
-    b = torch.empty_like(a)
-    for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes": n * 4, "reps": reps}), flush=True)
+    """Plain device-to-device copies of known size (tsl_probe_copy), for
+    scaling the FETCH_SIZE / WRITE_SIZE counters: each copy reads and writes
+    mib MiB."""
+    import ctypes
+    from tsdf_localization_amd._lib import check, lib
+    ms = ctypes.c_double()
+    check(lib.tsl_probe_copy(ctypes.c_uint64(mib << 20), reps, ctypes.byref(ms)))
+    print(json.dumps({"copy_calibration_bytes": mib << 20, "reps": reps, "ms_per_copy": ms.value}), flush=True)
 
 
 def bench_batched(args):
@@ -218,7 +287,7 @@ def bench_batched(args):
         "batched_eager_ms_per_step": eager_ms,
         "speedup_vs_plain": plain_ms / graph_ms,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_serving(args):
@@ -348,7 +417,7 @@ def bench_serving(args):
         "plain_value": plain_rate,
         "speedup_vs_plain": served_rate / plain_rate,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_mixed(args):
@@ -514,11 +583,15 @@ def main():
     ap.add_argument("--graph-step", action="store_true", help="also time the step replayed from a HIP graph")
     ap.add_argument("--cpu-sample", type=int, default=10000)
     ap.add_argument("--cpu-repeats", type=int, default=10)
-    ap.add_argument("--tile-k", type=int, default=None, help="particles per wavefront (1, 2); default autotune")
-    ap.add_argument("--tile-p", type=int, default=1, help="points per lane per iteration (1, 2, 4) with --tile-k")
-    ap.add_argument("--order", type=int, default=-1, help="particle order with --tile-k: 0 caller, 1 spatial, -1 auto")
-    ap.add_argument("--lookup", type=int, default=-1, help="block lookup with --tile-k: 0 table, 1 LDS rank, -1 auto")
-    ap.add_argument("--chunk", type=int, default=0, help="scan points per wavefront (0 = automatic)")
+    ap.add_argument("--autotune", action="store_true", help="pick the tile with tsl_filter_autotune")
+    ap.add_argument("--tile-k", type=int, default=PINNED_TILE["k"], help="particles per wavefront (1, 2)")
+    ap.add_argument("--tile-p", type=int, default=PINNED_TILE["p"], help="points per lane per iteration (1, 2, 4)")
+    ap.add_argument("--order", type=int, default=PINNED_TILE["order"], help="particle order: 0 caller, 1 spatial, -1 auto")
+    ap.add_argument("--lookup", type=int, default=PINNED_TILE["lookup"], help="block lookup: 0 table, 1 LDS rank, -1 auto")
+    ap.add_argument("--chunk", type=int, default=PINNED_TILE["chunk"], help="scan points per wavefront (0 = automatic)")
+    ap.add_argument("--no-headline", action="store_true", help="skip the batched and serving forms in the default run")
+    ap.add_argument("--headline-steps", type=int, default=10, help="batched steps timed in the default run")
+    ap.add_argument("--no-probe", action="store_true", help="skip the gather probe")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run plain copies of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -533,10 +606,10 @@ def main():
         return
 
     if args.workload == "batched":
-        bench_batched(args)
+        print(json.dumps(bench_batched(args)), flush=True)
         return
     if args.workload == "serving":
-        bench_serving(args)
+        print(json.dumps(bench_serving(args)), flush=True)
         return
     if args.workload == "mixed":
         bench_mixed(args)
@@ -560,15 +633,20 @@ def main():
     f.set_scan(w.points, w.tf)
     f.snapshot()
     stream = tsl.Stream()
-    if args.tile_k is None:
+    lookup = args.lookup
+    if args.autotune:
         order, k, pp, chunk, tms = f.autotune(stream, reps=3)
+        lookup = -1
+        picked_by = "tsl_filter_autotune"
         log(f"autotuned: order={order} K={k} points/lane={pp} chunk={chunk or 'auto'} "
             f"({tms:.3f} ms per update for evaluation and particle order)")
     else:
         order, k, pp, chunk = args.order, args.tile_k, args.tile_p, args.chunk
+        picked_by = "pinned" if (order, k, pp, chunk, lookup) == tuple(PINNED_TILE.values()) else "command line"
         f.set_tile(k, chunk, pp)
         f.set_order(order)
-        f.set_block_lookup(args.lookup)
+        f.set_block_lookup(lookup)
+    tile = {"order": order, "k": k, "p": pp, "chunk": chunk or 0, "lookup": lookup}
     rng = np.random.default_rng(12345)
     u0s = [np.float32(rng.uniform(0.0, 1.0 / P)) for _ in range(args.warmup + args.steps)]
 
@@ -642,10 +720,27 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cores()["used"]
         cpu = cpu_baseline(w, min(args.cpu_sample, P), args.cpu_repeats, threads)
         log(f"cpu baseline: {cpu['value']:.3e} evals/s on {threads} threads; faithful 8-thread "
             f"{cpu['faithful_8_threads_value']:.3e}")
+
+    probe = gather_probe() if not args.no_probe else None
+
+    # the north star's forms beside the plain update: batched filters and
+    # the serving loop, each against plain per-filter / per-request launches
+    headline = {}
+    if not args.no_headline and w.name == "building":
+        hb = argparse.Namespace(**vars(args))
+        hb.steps, hb.warmup = args.headline_steps, 2
+        b = bench_batched(hb)
+        headline["batched"] = {k2: b[k2] for k2 in ("value", "ms_per_step", "plain_value", "plain_ms_per_step",
+                                                    "speedup_vs_plain", "batched_eager_ms_per_step")}
+        headline["batched"]["config"] = b["config"]
+        s = bench_serving(argparse.Namespace(**dict(vars(args), extra_rates=[1000.0])))
+        headline["serving"] = {k2: s[k2] for k2 in ("value", "plain_value", "speedup_vs_plain", "latency_ms",
+                                                    "latency_ms_by_rate")}
+        headline["serving"]["config"] = s["config"]
 
     out = {
         "metric": "particle*point evals/s",
@@ -667,18 +762,32 @@ def main():
                    "map_voxels": int(w.map.coef().data_size), "block_fraction": w.meta.get("block_fraction"),
                    "step": "restore + pose + evaluate + normalise + resample",
                    "tile": {"particle_order": {0: "caller", 1: "spatial", -1: "auto"}[order],
-                            "particles_per_wavefront": k, "points_per_lane": pp, "chunk_points": chunk or "auto"}},
+                            "particles_per_wavefront": k, "points_per_lane": pp, "chunk_points": chunk or "auto",
+                            "block_lookup": {0: "table", 1: "LDS rank", -1: "auto"}[lookup],
+                            "picked_by": picked_by, "profile_key": tile_key(tile)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic() if w.name == "building" else None,
+                     "traffic": measured_traffic(tile) if w.name == "building" else None,
                      "kernel": "k_eval", "kernel_avg_ms": kern_avg_ms,
                      "bytes_per_eval": BYTES_PER_EVAL, "evals_per_launch": pairs // max(launches, 1)},
-        "valu": valu_issue(kern_avg_ms) if w.name == "building" else None,
+        "valu": valu_issue(kern_avg_ms, tile) if w.name == "building" else None,
+        "l2": l2_view(kern_avg_ms, tile, probe) if w.name == "building" else None,
+        "gather_probe_lines_per_s": probe,
         "cpu_baseline": cpu,
         "graph_step_ms": graph_ms,
         "dropin_host_buffers": dropin,
         "resampled_count": n_after,
     }
+    # the limiter the counters show: the larger of the HBM, VALU-issue and
+    # L2-request fractions (the roofline object keeps the HBM form)
+    fracs = {"hbm": out["roofline"]["frac"]}
+    if out["valu"]:
+        fracs["valu_issue"] = out["valu"]["frac"]
+    if out["l2"] and out["l2"]["frac_of_probe"]:
+        fracs["l2_gather"] = out["l2"]["frac_of_probe"]
+    out["roofline"]["measured_limiter"] = max(fracs, key=fracs.get)
+    out["roofline"]["fractions"] = fracs
+    out.update(headline)
     print(json.dumps(out), flush=True)
 
 

@@ -560,6 +560,21 @@ int tsl_mcl_write(const char* path, const tsl_point* points, const int32_t* ring
 int tsl_mcl_read(const char* path, tsl_point* points, int32_t* rings, size_t points_capacity, tsl_particle* particles,
                  size_t particles_capacity, float tf[16], float pose[7], size_t* num_points, size_t* num_particles);
 
+/* Random-line gather probe (measurement, no counterpart in the reference):
+ * wavefronts whose 64 lanes each read 2 bytes from a different pseudo-random
+ * 128-byte line of a device buffer of buffer_bytes (rounded down to a power
+ * of two of lines), 64 loads per lane, 8.4M lanes per launch, reps timed
+ * launches.  Reports lines returned per second and the launch time: the
+ * ceiling the voxel-code gathers of the evaluation work against (a buffer
+ * the size of a map's code array is served by the L2 and Infinity Cache, a
+ * many-GB one by HBM).  Allocates its buffer; synchronous. */
+int tsl_probe_gather(uint64_t buffer_bytes, uint32_t reps, double* lines_per_s, double* ms_per_launch);
+
+/* reps plain device-to-device copies of a bytes-sized buffer (hipMemcpyAsync):
+ * the known-size reference for scaling memory-traffic counters.  No
+ * counterpart in the reference.  Allocates; synchronous. */
+int tsl_probe_copy(uint64_t bytes, uint32_t reps, double* ms_per_copy);
+
 /* ---- streams and timing (plumbing for benchmarks) ---------------------- */
 int tsl_stream_create(void** stream);
 int tsl_stream_destroy(void* stream);

@@ -164,6 +164,9 @@ _SIGS = {
                                     ctypes.POINTER(ctypes.c_uint64)]),
     "tsl_check_sub_table": (_I, [_F, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
                                  ctypes.POINTER(ctypes.c_uint32)]),
+    "tsl_probe_gather": (_I, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_double)]),
+    "tsl_probe_copy": (_I, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "tsl_stream_create": (_I, [ctypes.POINTER(_P)]),
     "tsl_stream_destroy": (_I, [_P]),
     "tsl_stream_synchronize": (_I, [_P]),

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["api.hip", "eval.hip", "resample.hip", "tables.hip", "scan.hip", "batch.hip", "motion.hip", "mcl.hip", "server.hip", "chunks.hip",
-               "noalloc.hip"]
+               "noalloc.hip", "probe.hip"]
 SYNTH_SOURCES = ["synth.cpp"]
 
 LIBTSL = os.path.join(HERE, "libtsl.so")
