@@ -118,6 +118,19 @@ int tsl_map_create(const tsl_map_coef* coef, const int32_t* grid_occ, const floa
 int tsl_map_create_tsdf(const tsl_map_coef* coef, const int32_t* grid_occ,
                         const int16_t* tsdf_mm, int truncation_mm, float sigma, tsl_map** out);
 
+/* The 64-bit extension of tsl_map_create_tsdf, for maps of more voxels than
+ * the reference's int block offsets can address (OCC_T = int,
+ * cuda_sub_voxel_map.h:13; setData's int running offset, .tcc:202-211).
+ * grid_occ: grid_occ_size int64 block offsets (-1 = empty block), each a
+ * whole multiple of sub_dim^3 with its block inside the data_size voxels;
+ * tsdf_mm: data_size int16 millimetres in the reference's linear layout
+ * (offset + x + y*sub_dim + z*sub_dim^2).  sub_dim must be a multiple of 4 in
+ * [4, 256].  Lookups report 64-bit reference indices and read the same
+ * values as tsl_map_create_tsdf would.  The millimetres are converted on the
+ * device a few hundred MB at a time. */
+int tsl_map_create_tsdf_wide(const tsl_map_coef* coef, const int64_t* grid_occ, const int16_t* tsdf_mm,
+                             int truncation_mm, float sigma, tsl_map** out);
+
 /* Build the block table and voxel array from a list of cells exactly as
  * CudaSubVoxelMap::setData does (cuda_sub_voxel_map.tcc:171-231: cells are
  * placed through the float lookup of their corner coordinates).  coef is

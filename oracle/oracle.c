@@ -188,11 +188,43 @@ ORC_EXPORT void orc_coef_init(float min_x, float min_y, float min_z,
     c->data_size = 0;
 }
 
+/* A map as the oracle reads it: the reference's int32 block table with float
+ * voxel values, or the 64-bit extension past the reference's OCC_T = int
+ * (cuda_sub_voxel_map.h:13): an int64 block table with the voxels given as
+ * int16 TSDF millimetres, read through the likelihood value of each
+ * millimetre (map_util.h:116-129; INT16_MIN = never set, |mm| >= t = outside
+ * the truncation band: both read init_value). */
+typedef struct orc_map {
+    const orc_coef* c;
+    const int32_t* grid_occ;    /* reference form (NULL: 64-bit form) */
+    const float* data;
+    const int64_t* grid_occ64;  /* 64-bit form */
+    const int16_t* tsdf_mm;
+    const float* lut;           /* likelihood of mm in (-t, t) at mm + t - 1 */
+    int truncation_mm;
+} orc_map;
+
+static uint64_t orc_index_m(const orc_map* m, float x, float y, float z);
+
 /* CudaSubVoxelMap::getIndex, cuda_sub_voxel_map.tcc:51-137 (track_reuse off).
  * Returns data_size for every rejected lookup, like the reference. */
 ORC_EXPORT uint64_t orc_get_index(const orc_coef* c, const int32_t* grid_occ,
                                   float x, float y, float z)
 {
+    orc_map m = {c, grid_occ, NULL, NULL, NULL, NULL, 0};
+    return orc_index_m(&m, x, y, z);
+}
+
+/* getIndex on a 64-bit block table. */
+ORC_EXPORT uint64_t orc_get_index64(const orc_coef* c, const int64_t* grid_occ64, float x, float y, float z)
+{
+    orc_map m = {c, NULL, NULL, grid_occ64, NULL, NULL, 0};
+    return orc_index_m(&m, x, y, z);
+}
+
+static uint64_t orc_index_m(const orc_map* m, float x, float y, float z)
+{
+    const orc_coef* c = m->c;
     /* :53-62 integer-metre bounds check */
     uint64_t grid_x = orc_cvt_f32_u64(x - c->min_x);
     uint64_t grid_y = orc_cvt_f32_u64(y - c->min_y);
@@ -216,26 +248,41 @@ ORC_EXPORT uint64_t orc_get_index(const orc_coef* c, const int32_t* grid_occ,
     uint64_t up_index = up_x + up_y * c->up_dim_x + up_z * c->up_dim_2;
     if (up_index >= c->grid_occ_size)
         return c->data_size;
-    int32_t sub_index = grid_occ[up_index];
+    int64_t sub_index = m->grid_occ ? (int64_t)m->grid_occ[up_index] : m->grid_occ64[up_index];
     if (sub_index < 0)
         return c->data_size;
     /* :130-136 voxel inside the block */
     uint64_t sub_x = orc_cvt_f32_u64(sub_pos_x / c->resolution);
     uint64_t sub_y = orc_cvt_f32_u64(sub_pos_y / c->resolution);
     uint64_t sub_z = orc_cvt_f32_u64(sub_pos_z / c->resolution);
-    return (uint64_t)(int64_t)sub_index + sub_x + sub_y * c->sub_dim + sub_z * c->sub_dim_2;
+    return (uint64_t)sub_index + sub_x + sub_y * c->sub_dim + sub_z * c->sub_dim_2;
 }
 
 /* CudaSubVoxelMap::getEntry, cuda_sub_voxel_map.tcc:140-158. */
+static float orc_entry_m(const orc_map* m, float x, float y, float z)
+{
+    const orc_coef* c = m->c;
+    if (m->grid_occ ? m->data == NULL : m->tsdf_mm == NULL)
+        return c->init_value;
+    uint64_t index = orc_index_m(m, x, y, z);
+    if (index >= c->data_size)
+        return c->init_value;
+    if (m->grid_occ)
+        return m->data[index];
+    int mm = m->tsdf_mm[index];
+    int t = m->truncation_mm;
+    if (mm == INT16_MIN || mm <= -t || mm >= t)
+        return c->init_value;
+    return m->lut[mm + t - 1];
+}
+
 ORC_EXPORT float orc_get_entry(const orc_coef* c, const int32_t* grid_occ, const float* data,
                                float x, float y, float z)
 {
     if (data == NULL || grid_occ == NULL)
         return c->init_value;
-    uint64_t index = orc_get_index(c, grid_occ, x, y, z);
-    if (index < c->data_size)
-        return data[index];
-    return c->init_value;
+    orc_map m = {c, grid_occ, data, NULL, NULL, NULL, 0};
+    return orc_entry_m(&m, x, y, z);
 }
 
 /* CudaSubVoxelMap::setData, cuda_sub_voxel_map.tcc:171-231: mark the blocks
@@ -457,10 +504,8 @@ static void orc_derived(const orc_params* p, float* inv_mr, float* mr2)
 
 /* evaluatePose (:27-76) for one pose matrix.  Returns the reference's float
  * sequential sum and the float64 sum of the same float terms. */
-static void orc_eval_pose(const orc_coef* c, const int32_t* grid_occ, const float* data,
-                          const float L[12], const orc_point* pts, size_t n,
-                          const orc_params* prm, float inv_mr, float mr2,
-                          float* w32, double* w64)
+static void orc_eval_pose(const orc_map* m, const float L[12], const orc_point* pts, size_t n,
+                          const orc_params* prm, float inv_mr, float mr2, float* w32, double* w64)
 {
     float eval_sum = 0.0f;
     double eval_sum_d = 0.0;
@@ -469,7 +514,7 @@ static void orc_eval_pose(const orc_coef* c, const int32_t* grid_occ, const floa
         float tx = L[0] * x + L[1] * y + L[2] * z + L[3];
         float ty = L[4] * x + L[5] * y + L[6] * z + L[7];
         float tz = L[8] * x + L[9] * y + L[10] * z + L[11];
-        float value = orc_get_entry(c, grid_occ, data, tx, ty, tz);
+        float value = orc_entry_m(m, tx, ty, tz);
         float square_dist = x * x + y * y + z * z;
         if (square_dist < mr2)
             value = prm->a_hit * value + prm->a_range * inv_mr;
@@ -486,11 +531,10 @@ static void orc_eval_pose(const orc_coef* c, const int32_t* grid_occ, const floa
  * w32[i]: the reference's float weight; w64[i]: float64 sum of the same terms.
  * threads <= 0 uses all cores.  faithful_copy != 0 copies the point vector per
  * particle, as the by-value parameter of evaluatePose does (tsdf_evaluator.h:59). */
-ORC_EXPORT void orc_evaluate(const orc_coef* c, const int32_t* grid_occ, const float* data,
-                             const orc_particle* particles, size_t num_particles,
-                             const orc_point* points, size_t num_points, const float tf[16],
-                             const orc_params* prm, int trig_mode, int threads, int faithful_copy,
-                             float* w32, double* w64)
+static void orc_evaluate_m(const orc_map* m, const orc_particle* particles, size_t num_particles,
+                           const orc_point* points, size_t num_points, const float tf[16],
+                           const orc_params* prm, int trig_mode, int threads, int faithful_copy,
+                           float* w32, double* w64)
 {
     float inv_mr, mr2;
     orc_derived(prm, &inv_mr, &mr2);
@@ -510,10 +554,33 @@ ORC_EXPORT void orc_evaluate(const orc_coef* c, const int32_t* grid_occ, const f
                 memcpy(copy, points, sizeof(orc_point) * num_points);
                 pts = copy;
             }
-            orc_eval_pose(c, grid_occ, data, L, pts, num_points, prm, inv_mr, mr2, &w32[i], &w64[i]);
+            orc_eval_pose(m, L, pts, num_points, prm, inv_mr, mr2, &w32[i], &w64[i]);
         }
         free(copy);
     }
+}
+
+ORC_EXPORT void orc_evaluate(const orc_coef* c, const int32_t* grid_occ, const float* data,
+                             const orc_particle* particles, size_t num_particles,
+                             const orc_point* points, size_t num_points, const float tf[16],
+                             const orc_params* prm, int trig_mode, int threads, int faithful_copy,
+                             float* w32, double* w64)
+{
+    orc_map m = {c, grid_occ, data, NULL, NULL, NULL, 0};
+    orc_evaluate_m(&m, particles, num_particles, points, num_points, tf, prm, trig_mode, threads, faithful_copy,
+                   w32, w64);
+}
+
+/* The same on a map of the 64-bit form: int64 block table, int16 TSDF
+ * millimetres with truncation t read through lut (likelihood of mm at
+ * mm + t - 1, 2t - 1 entries). */
+ORC_EXPORT void orc_evaluate64(const orc_coef* c, const int64_t* grid_occ64, const int16_t* tsdf_mm,
+                               const float* lut, int truncation_mm, const orc_particle* particles,
+                               size_t num_particles, const orc_point* points, size_t num_points, const float tf[16],
+                               const orc_params* prm, int trig_mode, int threads, float* w32, double* w64)
+{
+    orc_map m = {c, NULL, NULL, grid_occ64, tsdf_mm, lut, truncation_mm};
+    orc_evaluate_m(&m, particles, num_particles, points, num_points, tf, prm, trig_mode, threads, 0, w32, w64);
 }
 
 /* Two trig bindings compared on the same inputs: the number of (particle,
@@ -550,8 +617,9 @@ ORC_EXPORT void orc_trig_binding_diff(const orc_coef* c, const int32_t* grid_occ
             }
             float wa32, wb32;
             double wa, wb;
-            orc_eval_pose(c, grid_occ, data, La, points, num_points, prm, inv_mr, mr2, &wa32, &wa);
-            orc_eval_pose(c, grid_occ, data, Lb, points, num_points, prm, inv_mr, mr2, &wb32, &wb);
+            orc_map m = {c, grid_occ, data, NULL, NULL, NULL, 0};
+            orc_eval_pose(&m, La, points, num_points, prm, inv_mr, mr2, &wa32, &wa);
+            orc_eval_pose(&m, Lb, points, num_points, prm, inv_mr, mr2, &wb32, &wb);
             double rel = wa != 0.0 ? fabs(wa - wb) / fabs(wa) : (wb != 0.0 ? 1.0 : 0.0);
             if (rel > worst) worst = rel;
         }
@@ -563,10 +631,8 @@ ORC_EXPORT void orc_trig_binding_diff(const orc_coef* c, const int32_t* grid_occ
 
 /* Voxel index of every (particle, point) pair, row-major [particle][point]:
  * what getIndex returns for the transformed point (data_size = rejected). */
-ORC_EXPORT void orc_trace_indices(const orc_coef* c, const int32_t* grid_occ,
-                                  const orc_particle* particles, size_t num_particles,
-                                  const orc_point* points, size_t num_points, const float tf[16],
-                                  int trig_mode, uint64_t* out)
+static void orc_trace_m(const orc_map* m, const orc_particle* particles, size_t num_particles,
+                        const orc_point* points, size_t num_points, const float tf[16], int trig_mode, uint64_t* out)
 {
     #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)num_particles; ++i) {
@@ -577,9 +643,27 @@ ORC_EXPORT void orc_trace_indices(const orc_coef* c, const int32_t* grid_occ,
             float tx = L[0] * x + L[1] * y + L[2] * z + L[3];
             float ty = L[4] * x + L[5] * y + L[6] * z + L[7];
             float tz = L[8] * x + L[9] * y + L[10] * z + L[11];
-            out[(size_t)i * num_points + j] = orc_get_index(c, grid_occ, tx, ty, tz);
+            out[(size_t)i * num_points + j] = orc_index_m(m, tx, ty, tz);
         }
     }
+}
+
+ORC_EXPORT void orc_trace_indices(const orc_coef* c, const int32_t* grid_occ,
+                                  const orc_particle* particles, size_t num_particles,
+                                  const orc_point* points, size_t num_points, const float tf[16],
+                                  int trig_mode, uint64_t* out)
+{
+    orc_map m = {c, grid_occ, NULL, NULL, NULL, NULL, 0};
+    orc_trace_m(&m, particles, num_particles, points, num_points, tf, trig_mode, out);
+}
+
+ORC_EXPORT void orc_trace_indices64(const orc_coef* c, const int64_t* grid_occ64,
+                                    const orc_particle* particles, size_t num_particles,
+                                    const orc_point* points, size_t num_points, const float tf[16],
+                                    int trig_mode, uint64_t* out)
+{
+    orc_map m = {c, NULL, NULL, grid_occ64, NULL, NULL, 0};
+    orc_trace_m(&m, particles, num_particles, points, num_points, tf, trig_mode, out);
 }
 
 /* Normalisation and pose estimate (:159-221).  The reference reduces with an

@@ -74,6 +74,10 @@ _sigs = {
     "orc_motion_normal": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]),
     "orc_motion": (None, [_P, _S, _P, _P, ctypes.c_uint64, ctypes.c_int, _P]),
     "orc_reduce_cloud": (ctypes.c_long, [_P, _P, _S, _P]),
+    "orc_get_index64": (ctypes.c_uint64, [ctypes.POINTER(OrcCoef), _P, _F, _F, _F]),
+    "orc_evaluate64": (None, [ctypes.POINTER(OrcCoef), _P, _P, _P, ctypes.c_int, _P, _S, _P, _S, _P,
+                              ctypes.POINTER(OrcParams), ctypes.c_int, ctypes.c_int, _P, _P]),
+    "orc_trace_indices64": (None, [ctypes.POINTER(OrcCoef), _P, _P, _S, _P, _S, _P, ctypes.c_int, _P]),
     "orc_trig_binding_diff": (None, [ctypes.POINTER(OrcCoef), _P, _P, _P, _S, _P, _S, _P, ctypes.POINTER(OrcParams),
                                      ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]),
@@ -192,6 +196,47 @@ class OracleMap:
         out = np.empty((p.shape[0], pts.shape[0]), np.uint64)
         olib.orc_trace_indices(ctypes.byref(self.coef), _ptr(self.grid), _ptr(p), p.shape[0], _ptr(pts),
                                pts.shape[0], _ptr(tf), trig_mode, _ptr(out))
+        return out
+
+
+class OracleMap64:
+    """A map in the 64-bit form (int64 block table, int16 TSDF millimetres
+    read through the likelihood values), without a float copy of the voxels."""
+
+    def __init__(self, coef, grid_occ64, tsdf_mm, truncation_mm, sigma=0.1):
+        self.coef = coef_from(coef)
+        self.grid = np.ascontiguousarray(grid_occ64, np.int64)
+        self.mm = np.ascontiguousarray(tsdf_mm, np.int16)
+        self.t = int(truncation_mm)
+        self.lut = likelihood_lut(self.t, sigma)
+
+    @classmethod
+    def from_host_map(cls, m):
+        return cls(m.coef(), m.grid_occ, m.tsdf_mm, m.truncation_mm, m.sigma)
+
+    def get_index(self, x, y, z):
+        return olib.orc_get_index64(ctypes.byref(self.coef), _ptr(self.grid), float(x), float(y), float(z))
+
+    def evaluate(self, particles, points, tf, a_hit=0.9, a_range=0.1, a_max=0.0, max_range=100.0, trig_mode=0,
+                 threads=0):
+        p = np.ascontiguousarray(particles, np.float32)
+        pts = np.ascontiguousarray(points, np.float32)
+        tf = np.ascontiguousarray(tf, np.float32)
+        prm = OrcParams(a_hit, a_range, a_max, max_range)
+        w32 = np.empty(p.shape[0], np.float32)
+        w64 = np.empty(p.shape[0], np.float64)
+        olib.orc_evaluate64(ctypes.byref(self.coef), _ptr(self.grid), _ptr(self.mm), _ptr(self.lut), self.t,
+                            _ptr(p), p.shape[0], _ptr(pts), pts.shape[0], _ptr(tf), ctypes.byref(prm), trig_mode,
+                            threads, _ptr(w32), _ptr(w64))
+        return w32, w64
+
+    def trace(self, particles, points, tf, trig_mode=0):
+        p = np.ascontiguousarray(particles, np.float32)
+        pts = np.ascontiguousarray(points, np.float32)
+        tf = np.ascontiguousarray(tf, np.float32)
+        out = np.empty((p.shape[0], pts.shape[0]), np.uint64)
+        olib.orc_trace_indices64(ctypes.byref(self.coef), _ptr(self.grid), _ptr(p), p.shape[0], _ptr(pts),
+                                 pts.shape[0], _ptr(tf), trig_mode, _ptr(out))
         return out
 
 

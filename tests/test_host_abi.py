@@ -163,11 +163,13 @@ def test_synthetic_workloads_are_deterministic():
 
 
 def test_mixed_workload_shapes_and_determinism():
-    """The mixed workload's generator (one map here): one global robot with
-    its particle set over the free space and a 4,000-point scan, tracking
-    robots of 5,000 particles and 10,000-point scans, identical on a second
-    call."""
-    a, b = synth.mixed(maps=1, tracking=2), synth.mixed(maps=1, tracking=2)
+    """The mixed workload's generator (one 2 x 2-tile campus here instead of
+    12 x 11): one global robot with its particle set over the free space and
+    a 4,000-point scan, tracking robots of 5,000 particles and 10,000-point
+    scans, identical on a second call."""
+    a = synth.mixed(maps=1, tracking=2, campus_tiles=(2, 2))
+    b = synth.mixed(maps=1, tracking=2, campus_tiles=(2, 2))
+    assert a.maps[0].wide and a.maps[0].grid_occ.dtype == np.int64
     g = a.global_robots[0]
     assert g.particles.shape == (1_000_000, 7) and g.points.shape == (4000, 3)
     assert [r.particles.shape for r in a.tracking[0]] == [(5000, 7)] * 2

@@ -88,6 +88,7 @@ _SIGS = {
     "tsl_map_coef_init": (_I, [_F, _F, _F, _F, _F, _F, _F, _F, ctypes.POINTER(MapCoef)]),
     "tsl_map_create": (_I, [ctypes.POINTER(MapCoef), _P, _P, ctypes.POINTER(_P)]),
     "tsl_map_create_tsdf": (_I, [ctypes.POINTER(MapCoef), _P, _P, _I, _F, ctypes.POINTER(_P)]),
+    "tsl_map_create_tsdf_wide": (_I, [ctypes.POINTER(MapCoef), _P, _P, _I, _F, ctypes.POINTER(_P)]),
     "tsl_map_set_data": (_I, [ctypes.POINTER(MapCoef), _P, _P, _S, _P, _P, _S]),
     "tsl_map_chunks_to_cells": (_I, [_P, _P, _S, _F, ctypes.POINTER(MapCoef), _P, _P, _S, ctypes.POINTER(_S),
                                      _P, _S, ctypes.POINTER(_S)]),

@@ -222,6 +222,9 @@ static int validate_coef(const tsl_map_coef* c)
     return TSL_OK;
 }
 
+static int map_finish(tsl_map* m, const int32_t* grid_occ, const std::vector<float>& palette, uint16_t init_code,
+                      bool brick, uint64_t phys_size);
+
 static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uint16_t>& codes,
                       const std::vector<float>& palette, uint16_t init_code)
 {
@@ -267,6 +270,18 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     if (!palette.empty())
         TSL_HIP_CHECK(hipMemcpy(m->palette.ptr, palette.data(), sizeof(float) * palette.size(),
                                 hipMemcpyHostToDevice));
+    return map_finish(m, grid_occ, palette, init_code, brick, phys_size);
+}
+
+// The tables and the device views of an uploaded map (codes, palette and
+// block table already on the device).  grid_occ: the int32 block table on
+// the host, or NULL for a wide map (slot table on the device).
+static int map_finish(tsl_map* m, const int32_t* grid_occ, const std::vector<float>& palette, uint16_t init_code,
+                      bool brick, uint64_t phys_size)
+{
+    const tsl_map_coef& c = m->coef;
+    const bool wide = grid_occ == nullptr;
+    const uint64_t sd = c.sub_dim, sd3 = sd * sd * sd;
     SubTable st;
     TSL_TRY(build_sub_table(c.resolution, &st));
     TSL_TRY(m->sub_tab.reserve(sizeof(SubEntry) * st.entries.size()));
@@ -274,6 +289,9 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
                             hipMemcpyHostToDevice));
     DevMapFull fm{};
     fm.grid_occ = m->grid_occ.as<int32_t>();
+    fm.slots = m->slots.as<uint32_t>();
+    fm.sd3 = sd3;
+    fm.wide = wide ? 1u : 0u;
     fm.res = c.resolution;
     fm.dim_x = c.dim_x; fm.dim_y = c.dim_y; fm.dim_z = c.dim_z;
     fm.up_dim_x = c.up_dim_x;
@@ -290,6 +308,8 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     TSL_HIP_CHECK(hipMemcpy(m->full.ptr, &fm, sizeof(DevMapFull), hipMemcpyHostToDevice));
     DevMap& d = m->dev;
     d.grid_occ = m->grid_occ.as<int32_t>();
+    d.slots = m->slots.as<uint32_t>();
+    d.wide = wide ? 1u : 0u;
     d.codes = m->codes.as<uint16_t>();
     d.palette = m->palette.as<float>();
     d.sub_tab = m->sub_tab.as<SubEntry>();
@@ -318,14 +338,14 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     d.up_dim_x = (uint32_t)c.up_dim_x;
     d.up_dim_2 = (uint32_t)c.up_dim_2;
     d.occ = (uint32_t)c.grid_occ_size;
-    d.ds = (uint32_t)c.data_size;
+    d.ds = (uint32_t)std::min<uint64_t>(c.data_size, 0xffffffffull);
     d.sub_dim = (int32_t)c.sub_dim;
     d.sub_dim_2 = (int32_t)c.sub_dim_2;
     d.data_size = c.data_size;
     d.brick = brick ? 1u : 0u;
     d.sd3 = (uint32_t)sd3;
     d.sd3b = (uint32_t)(2 * sd3);
-    d.sentinel = (uint32_t)phys_size;
+    d.sentinel = phys_size;
     d.sentinel_code = init_code * 4u;
     {
         // occupancy-rank form of the block table
@@ -348,7 +368,7 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
             nwords *= 2;
             ++log2w;
         }
-        bool rank = nwords <= kMaxOccWords && 2 * sd3 < (1ull << 24);
+        bool rank = !wide && nwords <= kMaxOccWords && 2 * sd3 < (1ull << 24);
         std::vector<OccWord> words;
         if (rank) {
             words.assign(nwords, OccWord{0u, 0u});
@@ -520,6 +540,119 @@ int tsl_map_create_tsdf(const tsl_map_coef* coef, const int32_t* grid_occ, const
     return TSL_OK;
 }
 
+namespace {
+
+// Codes of a wide TSDF map from its millimetres: workgroup b converts block
+// slot s0 + b, whose voxels are mm[(b * sd3) ..] of the staged chunk in the
+// reference's linear layout (x fastest), writing each voxel's palette byte
+// offset at its bricked position; elements at or past `limit` (the map's
+// data size, relative to the chunk) are left alone.
+__global__ void k_wide_codes(const int16_t* __restrict__ mm, uint64_t s0, uint64_t limit, int t, uint16_t init_off,
+                             uint32_t sd, uint32_t sd3, uint32_t bdim, uint16_t* __restrict__ codes)
+{
+    const uint64_t slot = s0 + blockIdx.x;
+    const uint64_t first = (uint64_t)blockIdx.x * sd3;
+    for (uint32_t rem = threadIdx.x; rem < sd3; rem += blockDim.x) {
+        if (first + rem >= limit) break;
+        const uint32_t lz = rem / (sd * sd), ly = (rem / sd) % sd, lx = rem % sd;
+        const uint32_t off = ((lx >> 2) + (ly >> 2) * bdim + (lz >> 2) * bdim * bdim) * 64u + (lx & 3u) +
+                             (ly & 3u) * 4u + (lz & 3u) * 16u;
+        const int v = mm[first + rem];
+        codes[slot * sd3 + off] = (v != INT16_MIN && v > -t && v < t) ? (uint16_t)((v + t - 1) * 4) : init_off;
+    }
+}
+
+__global__ void k_fill_u16(uint16_t* __restrict__ p, uint64_t n, uint16_t v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace
+
+int tsl_map_create_tsdf_wide(const tsl_map_coef* coef, const int64_t* grid_occ, const int16_t* tsdf_mm,
+                             int truncation_mm, float sigma, tsl_map** out)
+{
+    if (!coef || !grid_occ || !out || (coef->data_size && !tsdf_mm))
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    const tsl_map_coef& c = *coef;
+    if (!(c.resolution > 0.0f) || !std::isfinite(c.resolution))
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "resolution must be positive and finite");
+    if (c.grid_occ_size != c.up_dim_x * c.up_dim_y * c.up_dim_z || c.grid_occ_size == 0 ||
+        c.up_dim_2 != c.up_dim_x * c.up_dim_y || c.sub_dim_2 != c.sub_dim * c.sub_dim)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "inconsistent map dims");
+    if (c.grid_occ_size > 0x7fffffffull || c.sub_dim < 4 || c.sub_dim > 256 || c.sub_dim % 4 != 0 ||
+        c.data_size >= (1ull << 46))
+        return set_error(TSL_ERR_UNSUPPORTED,
+                         "the 64-bit map form needs sub_dim a multiple of 4 in [4, 256], fewer than 2^31 blocks "
+                         "and fewer than 2^46 voxels");
+    if (truncation_mm < 1 || 2 * (uint32_t)truncation_mm > kMaxPalette)
+        return set_error(TSL_ERR_UNSUPPORTED, "truncation_mm must be in [1, %u]", kMaxPalette / 2);
+    const uint64_t sd = c.sub_dim, sd3 = sd * sd * sd;
+    // slot table: a block's offset must be a whole multiple of sub_dim^3
+    // with the whole block inside the voxel array
+    std::vector<uint32_t> slots(c.grid_occ_size);
+    for (uint64_t b = 0; b < c.grid_occ_size; ++b) {
+        const int64_t off = grid_occ[b];
+        if (off < 0) {
+            slots[b] = kNoSlot;
+            continue;
+        }
+        if ((uint64_t)off % sd3 != 0 || (uint64_t)off + sd3 > c.data_size || (uint64_t)off / sd3 >= kNoSlot)
+            return set_error(TSL_ERR_UNSUPPORTED,
+                             "block %llu: offset %lld is not a multiple of sub_dim^3 inside the voxel array",
+                             (unsigned long long)b, (long long)off);
+        slots[b] = (uint32_t)((uint64_t)off / sd3);
+    }
+    const int t = truncation_mm;
+    std::vector<float> palette(2 * t);
+    for (int mm = -(t - 1); mm <= t - 1; ++mm) palette[mm + t - 1] = tsl_likelihood_value((int16_t)mm, sigma);
+    const uint16_t init_code = (uint16_t)(2 * t - 1);
+    palette[init_code] = c.init_value;
+    tsl_map* m = new (std::nothrow) tsl_map();
+    if (!m) return set_error(TSL_ERR_OUT_OF_MEMORY, "out of host memory");
+    (void)hipGetDevice(&m->device);
+    m->coef = c;
+    int rc = [&]() -> int {
+        const uint64_t nslots = (c.data_size + sd3 - 1) / sd3;
+        const uint64_t phys_size = nslots * sd3;
+        const uint16_t init_off = (uint16_t)(init_code * 4u);
+        TSL_TRY(m->slots.reserve(sizeof(uint32_t) * c.grid_occ_size));
+        TSL_HIP_CHECK(hipMemcpy(m->slots.ptr, slots.data(), sizeof(uint32_t) * c.grid_occ_size,
+                                hipMemcpyHostToDevice));
+        TSL_TRY(m->codes.reserve(sizeof(uint16_t) * (phys_size + 1)));
+        // every element, padding and the sentinel after the voxels included,
+        // starts as the code of init_value
+        hipLaunchKernelGGL(k_fill_u16, dim3(8192), dim3(256), 0, 0, m->codes.as<uint16_t>(), phys_size + 1, init_off);
+        TSL_HIP_CHECK(hipGetLastError());
+        // millimetres staged a whole number of blocks at a time (about 256 MB)
+        const uint64_t per = std::max<uint64_t>(1, (128ull << 20) / sd3);
+        DevBuf stage;
+        TSL_TRY(stage.reserve(sizeof(int16_t) * per * sd3));
+        for (uint64_t s0 = 0; s0 < nslots; s0 += per) {
+            const uint64_t s1 = std::min(nslots, s0 + per);
+            const uint64_t e0 = s0 * sd3, e1 = std::min(s1 * sd3, c.data_size);
+            TSL_HIP_CHECK(hipMemcpy(stage.ptr, tsdf_mm + e0, sizeof(int16_t) * (e1 - e0), hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_wide_codes, dim3((unsigned)(s1 - s0)), dim3(256), 0, 0, stage.as<int16_t>(), s0,
+                               e1 - e0, t, init_off, (uint32_t)sd, (uint32_t)sd3, (uint32_t)(sd / 4),
+                               m->codes.as<uint16_t>());
+            TSL_HIP_CHECK(hipGetLastError());
+        }
+        TSL_HIP_CHECK(hipDeviceSynchronize());
+        TSL_TRY(m->palette.reserve(sizeof(float) * palette.size()));
+        TSL_HIP_CHECK(hipMemcpy(m->palette.ptr, palette.data(), sizeof(float) * palette.size(),
+                                hipMemcpyHostToDevice));
+        return map_finish(m, nullptr, palette, init_code, true, phys_size);
+    }();
+    if (rc != TSL_OK) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return TSL_OK;
+}
+
 // CudaSubVoxelMap::getIndex on the host, used by the setData replay.
 static uint64_t host_get_index(const tsl_map_coef& c, const int32_t* grid_occ, float x, float y, float z)
 {
@@ -589,7 +722,7 @@ int tsl_map_device_bytes(const tsl_map* m, uint64_t* bytes)
 {
     if (!m || !bytes) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
     *bytes = m->grid_occ.bytes + m->codes.bytes + m->palette.bytes + m->sub_tab.bytes + m->full.bytes +
-             m->neg_tab.bytes + m->occ_words.bytes + m->axis_tab.bytes;
+             m->neg_tab.bytes + m->occ_words.bytes + m->axis_tab.bytes + m->slots.bytes;
     return TSL_OK;
 }
 

@@ -62,12 +62,12 @@ __global__ void k_pose(const tsl_particle* __restrict__ particles, const uint32_
 
 // Exact lookup of one (particle, point) pair outside the fast tier: wrap
 // tier if every axis offset is within R, else the general tier.
-template <bool kU24, bool kRank>
+template <bool kU24, bool kRank, bool kWide>
 __device__ __forceinline__ uint64_t slow_index(const DevMap& map, const SubEntry* __restrict__ tab,
                                                const uint64_t* __restrict__ neg, const OccWord* __restrict__ occ,
                                                float ox, float oy, float oz)
 {
-    if (wrap_ok(ox, oy, oz, map.range_bits)) return wrap_index<kU24, kRank>(map, tab, neg, occ, ox, oy, oz);
+    if (wrap_ok(ox, oy, oz, map.range_bits)) return wrap_index<kU24, kRank, kWide>(map, tab, neg, occ, ox, oy, oz);
     return lookup_index_general(*map.full, ox, oy, oz);
 }
 
@@ -171,8 +171,9 @@ inline size_t lds_bytes(const DevMap& d, bool rank)
 // with the same tiling, tier selection and addressing as the production
 // mode (kMode 0); trace_out is indexed by the caller's point order (perm).
 // kMode 0 writes the chunk's partial weight of particle p0 + k to
-// partial_row[p0 + k].
-template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank>
+// partial_row[p0 + k].  kWide: the map's block table holds slots and its
+// voxel and byte offsets need 64 bits (bricked layout, no rank table).
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false>
 __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t, const float* __restrict__ L,
                                           uint32_t P, uint32_t p0, bool active, const float4* __restrict__ pts,
                                           uint32_t N, uint32_t j0, uint32_t j1, float a_hit,
@@ -235,18 +236,23 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
             }
         }
         if (__builtin_amdgcn_ballot_w64(!fast) == 0) {
+            // byte offsets into the code array: 32 bits, or 64 for wide maps
+            using Off = std::conditional_t<kWide, uint64_t, uint32_t>;
             float v[kP][K];
             uint64_t full[kP][K];
-            uint32_t addr[kP][K];
+            Off addr[kP][K];
             bool ok[kP][K];
-            uint32_t sb[kP][K];
+            Off sb[kP][K];
             bool blk[kP][K];
             bool any = false;
 #pragma unroll
             for (int u = 0; u < kP; ++u) {
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    fast_block<kU24, kRank>(map, occ, ox[u][k], oy[u][k], oz[u][k], sb[u][k], blk[u][k]);
+                    if constexpr (kWide)
+                        fast_block_wide<kU24>(map, ox[u][k], oy[u][k], oz[u][k], sb[u][k], blk[u][k]);
+                    else
+                        fast_block<kU24, kRank>(map, occ, ox[u][k], oy[u][k], oz[u][k], sb[u][k], blk[u][k]);
                     any = any | blk[u][k];
                 }
             }
@@ -270,23 +276,28 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
                 for (int k = 0; k < K; ++k) {
                     uint32_t pc, kx, ky, kz;
                     fast_inblock(map, t.atab, ox[u][k], oy[u][k], oz[u][k], pc, kx, ky, kz);
-                    const uint32_t s = sb[u][k];
+                    const Off s = sb[u][k];
                     // byte address of the code; a lane whose reference index
                     // leaves its block row (carry: an axis poisoned its part)
                     // recomputes it from the linear index, and only such a
                     // lane can fall past data_size (a whole row lies inside
                     // the array)
-                    uint32_t a = s + pc;
+                    Off a = s + pc;
                     bool valid = blk[u][k];
-                    const bool carry = a >= map.carry_limit;
+                    const bool carry = kWide ? pc >= kCarryPoison : a >= map.carry_limit;
                     if (kTrace || __builtin_expect(carry, 0)) {
-                        const uint32_t s_el = s >> 1;
+                        const Off s_el = s >> 1;
                         const uint32_t small = kx + mul_u<kU24>(ky, (uint32_t)map.sub_dim) +
                                                mul_u<kU24>(kz, (uint32_t)map.sub_dim_2);
-                        const uint32_t sum = s_el + small;
+                        const Off sum = s_el + small;
                         if (carry) {
-                            valid = valid & (sum < map.ds);
-                            a = (kBrick ? phys_from_linear(map, sum) : sum) << 1;
+                            if constexpr (kWide) {
+                                valid = valid & (sum < map.data_size);
+                                a = phys_from_linear64(map, sum) << 1;
+                            } else {
+                                valid = valid & (sum < map.ds);
+                                a = (kBrick ? phys_from_linear(map, sum) : sum) << 1;
+                            }
                         }
                         if (kTrace) full[u][k] = blk[u][k] ? (uint64_t)s_el + small : map.data_size;
                     }
@@ -302,7 +313,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     code[u][k] = map.sentinel_code;
-                    // 32-bit byte offset from the array base (data_size < 2^31)
+                    // byte offset from the array base (32-bit unless wide)
                     if (ok[u][k])
                         code[u][k] = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(map.codes) +
                                                                         addr[u][k]);
@@ -338,11 +349,16 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
                 const float4 qq = q[u];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const uint64_t full = slow_index<kU24, kRank>(map, tab, neg, occ, ox[u][k], oy[u][k], oz[u][k]);
+                    const uint64_t full =
+                        slow_index<kU24, kRank, kWide>(map, tab, neg, occ, ox[u][k], oy[u][k], oz[u][k]);
                     float v = kTrace ? map.init : a_hit * map.init;
                     if (full < map.data_size) {
-                        const uint32_t x = (uint32_t)full;
-                        v = palette_at(lut, map.codes[kBrick ? phys_from_linear(map, x) : x]);
+                        if constexpr (kWide) {
+                            v = palette_at(lut, map.codes[phys_from_linear64(map, full)]);
+                        } else {
+                            const uint32_t x = (uint32_t)full;
+                            v = palette_at(lut, map.codes[kBrick ? phys_from_linear(map, x) : x]);
+                        }
                     }
                     if (kMode == 1) {
                         if (a && p0 + k < P) static_cast<uint64_t*>(trace_out)[(size_t)(p0 + k) * N + perm[j]] = full;
@@ -369,7 +385,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
     }
 }
 
-template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank>
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false>
 __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __restrict__ L,
                                                  const uint32_t* __restrict__ count,
                                                  const float4* __restrict__ pts, uint32_t N,
@@ -393,9 +409,9 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
     if (g * kWaves * K >= P) return;  // the whole workgroup is idle (uniform across its waves)
     const uint32_t j0 = c * chunk_len;
     const uint32_t j1 = min(N, j0 + chunk_len);
-    eval_wave<K, kP, kMode, kU24, kBrick, kRank>(map, t, L, P, p0, p0 < P, pts, N, j0, j1, a_hit,
-                                                 kMode == 0 ? partial + (size_t)c * stride : nullptr,
-                                                 trace_out, perm, t.tile);
+    eval_wave<K, kP, kMode, kU24, kBrick, kRank, kWide>(map, t, L, P, p0, p0 < P, pts, N, j0, j1, a_hit,
+                                                        kMode == 0 ? partial + (size_t)c * stride : nullptr,
+                                                        trace_out, perm, t.tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -667,7 +683,7 @@ __global__ __launch_bounds__(1024) void k_batch_plan(const Member* __restrict__ 
 // ... through their XCD's counter, so all workgroups sweep the members
 // together, as plain per-filter launches would, and the tail balances
 // itself; per item the same wave body as k_eval.
-template <int K, int kP, bool kU24, bool kBrick, bool kRank>
+template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false>
 __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member* __restrict__ members,
                                                        BatchPlan* __restrict__ plan, float a_hit)
 {
@@ -704,8 +720,8 @@ __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member*
         const uint32_t p0 = (g * kWaves + wave) * K;
         const uint32_t j0 = c * m.chunk_len;
         const uint32_t j1 = min(m.N, j0 + m.chunk_len);
-        eval_wave<K, kP, 0, kU24, kBrick, kRank>(map, t, m.L, P, p0, p0 < P, m.points, m.N, j0, j1, a_hit,
-                                                 m.partial + (size_t)c * m.stride, nullptr, nullptr, tile);
+        eval_wave<K, kP, 0, kU24, kBrick, kRank, kWide>(map, t, m.L, P, p0, p0 < P, m.points, m.N, j0, j1, a_hit,
+                                                        m.partial + (size_t)c * m.stride, nullptr, nullptr, tile);
     }
 }
 
@@ -793,14 +809,14 @@ bool palette_at_zero(const void* kernel)
         if (!tsl_lds0_) return set_error(TSL_ERR_UNSUPPORTED, "evaluation kernel declares static LDS"); \
     } while (0)
 
-template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank>
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false>
 int launch_eval_t(dim3 grid, size_t lds, hipStream_t stream, const tsl_filter* f, const Tile& tile, size_t N,
                   float* partial, void* trace_out)
 {
-    TSL_CHECK_PALETTE_AT_ZERO((&k_eval<K, kP, kMode, kU24, kBrick, kRank>));
+    TSL_CHECK_PALETTE_AT_ZERO((&k_eval<K, kP, kMode, kU24, kBrick, kRank, kWide>));
     const tsl_map* map = f->ev->map;
     FilterState* st = f->state.as<FilterState>();
-    hipLaunchKernelGGL((k_eval<K, kP, kMode, kU24, kBrick, kRank>), grid, dim3(kBlock), lds, stream, map->dev,
+    hipLaunchKernelGGL((k_eval<K, kP, kMode, kU24, kBrick, kRank, kWide>), grid, dim3(kBlock), lds, stream, map->dev,
                        f->L.as<float>(), &st->count, f->points.as<float4>(), (uint32_t)N, tile.chunk_len,
                        tile.chunks, f->ev->a_hit, partial, (uint32_t)f->capacity, trace_out,
                        f->perm.as<uint32_t>());
@@ -813,6 +829,10 @@ int launch_eval_flags(bool u24, bool brick, bool rank, dim3 grid, size_t lds, hi
                       const tsl_filter* f, const Tile& tile, size_t N, float* partial, void* trace_out)
 {
 #define TSL_EVAL_ARGS grid, lds, stream, f, tile, N, partial, trace_out
+    if (f->ev->map->dev.wide) {  // bricked, slot table, no rank table
+        if (u24) return launch_eval_t<K, kP, kMode, true, true, false, true>(TSL_EVAL_ARGS);
+        return launch_eval_t<K, kP, kMode, false, true, false, true>(TSL_EVAL_ARGS);
+    }
     if (u24) {
         if (brick) {
             if (rank) return launch_eval_t<K, kP, kMode, true, true, true>(TSL_EVAL_ARGS);
@@ -1144,15 +1164,15 @@ int launch_set_count(tsl_filter* f, uint32_t n, hipStream_t stream)
 namespace {
 
 // stream == nullptr && grid_out: report the persistent grid size only.
-template <int K, int kP, bool kU24, bool kBrick, bool kRank>
+template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false>
 int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid_out)
 {
-    TSL_CHECK_PALETTE_AT_ZERO((&k_eval_batch<K, kP, kU24, kBrick, kRank>));
+    TSL_CHECK_PALETTE_AT_ZERO((&k_eval_batch<K, kP, kU24, kBrick, kRank, kWide>));
     const size_t lds = lds_bytes(b.map->dev, kRank);
     if (grid_out) {
         int per_cu = 0;
         TSL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&k_eval_batch<K, kP, kU24, kBrick, kRank>), kBlock, lds));
+            &per_cu, reinterpret_cast<const void*>(&k_eval_batch<K, kP, kU24, kBrick, kRank, kWide>), kBlock, lds));
         if (per_cu < 1) per_cu = 1;
         int cus = 0;
         TSL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b.map->device));
@@ -1164,7 +1184,7 @@ int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid
         *grid_out = (uint32_t)grid;
         return TSL_OK;
     }
-    hipLaunchKernelGGL((k_eval_batch<K, kP, kU24, kBrick, kRank>), dim3(b.grid), dim3(kBlock), lds, stream,
+    hipLaunchKernelGGL((k_eval_batch<K, kP, kU24, kBrick, kRank, kWide>), dim3(b.grid), dim3(kBlock), lds, stream,
                        b.map->dev, b.members, b.plan, b.a_hit);
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
@@ -1176,6 +1196,9 @@ int launch_eval_batch_flags(const BatchLaunch& b, hipStream_t stream, uint32_t* 
     const DevMap& d = b.map->dev;
     const bool u24 = d.up_dim_x < (1u << 23) && d.up_dim_2 < (1u << 23) && d.sub_dim_2 < (1 << 23);
     const bool brick = d.brick != 0, rank = b.rank && d.occ_nwords != 0;
+    if (d.wide)
+        return u24 ? launch_eval_batch_t<K, kP, true, true, false, true>(b, stream, grid_out)
+                   : launch_eval_batch_t<K, kP, false, true, false, true>(b, stream, grid_out);
     if (u24) {
         if (brick) return rank ? launch_eval_batch_t<K, kP, true, true, true>(b, stream, grid_out)
                                : launch_eval_batch_t<K, kP, true, true, false>(b, stream, grid_out);

@@ -121,8 +121,16 @@ struct OccWord {
 // Every field the exact tiers of the lookup need (64-bit reference sizes,
 // the wrap-band constants).  Lives in device memory; the kernels read it
 // only when a wavefront leaves the fast tier.
+// Wide maps (block offsets past the reference's int32, for maps of more than
+// 2^31 voxels) keep, instead of the offset, each block's slot: offset /
+// sub_dim^3, or kNoSlot for an empty block.
+constexpr uint32_t kNoSlot = 0xffffffffu;
+
 struct DevMapFull {
     const int32_t* grid_occ;
+    const uint32_t* slots;            // wide maps: block slot or kNoSlot (grid_occ unused)
+    uint64_t sd3;                     // sub_dim^3
+    uint32_t wide, pad0;
     float res;
     uint32_t gneg_x, gneg_y, gneg_z;  // grid check result of an axis offset in (-2^23, -1]
     uint64_t dim_x, dim_y, dim_z;
@@ -136,6 +144,7 @@ struct DevMapFull {
 // Kernel-argument view of a map: what the fast tier reads every iteration.
 struct DevMap {
     const int32_t* grid_occ;   // block table: voxel offset of the block or -1
+    const uint32_t* slots;     // wide maps: block slot (offset / sub_dim^3) or kNoSlot
     const uint16_t* codes;     // per voxel: byte offset (4 * palette code) of its value in the palette
     const float* palette;      // palette_size values
     const SubEntry* sub_tab;   // sub-voxel threshold buckets
@@ -170,9 +179,9 @@ struct DevMap {
     const OccWord* occ_words;
     uint32_t occ_nwords;              // 0: use grid_occ; else a power of two (zero words past the blocks)
     uint32_t occ_log2;                // log2(occ_nwords): block up reads word bits [5, 5 + occ_log2) of up
-    uint32_t sentinel;                // element of codes holding the code of init_value
+    uint64_t sentinel;                // element of codes holding the code of init_value
     uint32_t sentinel_code;           // that byte offset
-    uint32_t pad1;
+    uint32_t wide;                    // 1: slots instead of grid_occ, 64-bit voxel and byte offsets
 };
 
 // Smallest u in [0, 2^23] with trunc(fl(fl(u)/res)) >= dim; host IEEE division.
@@ -283,6 +292,7 @@ struct tsl_map {
     int device = 0;
     tsl_map_coef coef{};
     tsl::DevBuf grid_occ, codes, palette, sub_tab, neg_tab, full, occ_words, axis_tab;
+    tsl::DevBuf slots;       // wide maps: uint32 block slots (grid_occ unused)
     tsl::DevMap dev{};
 };
 

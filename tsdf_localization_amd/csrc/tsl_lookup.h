@@ -41,12 +41,18 @@ __device__ __forceinline__ uint64_t lookup_index_general(const DevMapFull& m, fl
     const float spz = oz - cvt_u64_f32(gz) * 1.0f;
     const uint64_t up = gx + gy * m.up_dim_x + gz * m.up_dim_2;
     if (up >= m.grid_occ_size) return m.data_size;
-    const int32_t s = m.grid_occ[up];
+    int64_t s;
+    if (m.wide) {
+        const uint32_t slot = m.slots[up];
+        s = slot == kNoSlot ? -1 : (int64_t)((uint64_t)slot * m.sd3);
+    } else {
+        s = m.grid_occ[up];
+    }
     if (s < 0) return m.data_size;
     const uint64_t sx = cvt_f32_u64(spx / m.res);
     const uint64_t sy = cvt_f32_u64(spy / m.res);
     const uint64_t sz = cvt_f32_u64(spz / m.res);
-    return (uint64_t)(int64_t)s + sx + sy * m.sub_dim + sz * m.sub_dim_2;
+    return (uint64_t)s + sx + sy * m.sub_dim + sz * m.sub_dim_2;
 }
 
 // trunc(fl(a / res)) for a in [0, 1): bucket of a's bit pattern in the
@@ -101,9 +107,13 @@ __device__ __forceinline__ int32_t rank_offset(const OccWord* __restrict__ occ, 
     return set ? (int32_t)off : -1;
 }
 
-template <bool kRank>
-__device__ __forceinline__ int32_t block_offset(const DevMap& m, const OccWord* __restrict__ occ, uint32_t up)
+template <bool kRank, bool kWide>
+__device__ __forceinline__ int64_t block_offset(const DevMap& m, const OccWord* __restrict__ occ, uint32_t up)
 {
+    if (kWide) {
+        const uint32_t slot = m.slots[up];
+        return slot == kNoSlot ? -1 : (int64_t)((uint64_t)slot * m.sd3);
+    }
     if (kRank) return rank_offset(occ, m.occ_log2, up, m.sd3);
     return m.grid_occ[up];
 }
@@ -134,6 +144,18 @@ __device__ __forceinline__ uint32_t phys_from_linear(const DevMap& m, uint32_t x
 {
     const uint32_t slot = __umulhi(x, m.mag_sd3) >> m.sh_sd3;
     const uint32_t rem = x - slot * m.sd3;
+    const uint32_t lz = __umulhi(rem, m.mag_sd2) >> m.sh_sd2;
+    const uint32_t r2 = rem - lz * (uint32_t)m.sub_dim_2;
+    const uint32_t ly = __umulhi(r2, m.mag_sd) >> m.sh_sd;
+    const uint32_t lx = r2 - ly * (uint32_t)m.sub_dim;
+    return slot * m.sd3 + brick_offset(m, lx, ly, lz);
+}
+
+// The same for any 64-bit reference index (wide maps).
+__device__ __forceinline__ uint64_t phys_from_linear64(const DevMap& m, uint64_t x)
+{
+    const uint64_t slot = x / m.sd3;
+    const uint32_t rem = (uint32_t)(x - slot * m.sd3);
     const uint32_t lz = __umulhi(rem, m.mag_sd2) >> m.sh_sd2;
     const uint32_t r2 = rem - lz * (uint32_t)m.sub_dim_2;
     const uint32_t ly = __umulhi(r2, m.mag_sd) >> m.sh_sd;
@@ -182,6 +204,20 @@ __device__ __forceinline__ void fast_block(const DevMap& m, const OccWord* __res
     }
 }
 
+// Block part of a wide map: the block's byte offset in 64 bits from its slot.
+template <bool kU24>
+__device__ __forceinline__ void fast_block_wide(const DevMap& m, float ox, float oy, float oz, uint64_t& s,
+                                                bool& valid)
+{
+    const uint32_t ix = (uint32_t)ox, iy = (uint32_t)oy, iz = (uint32_t)oz;
+    const bool grid = (ix < m.gthr_x) & (iy < m.gthr_y) & (iz < m.gthr_z);
+    const uint32_t up = ix + mul_u<kU24>(iy, m.up_dim_x) + mul_u<kU24>(iz, m.up_dim_2);
+    uint32_t slot = kNoSlot;
+    if (grid & (up < m.occ)) slot = m.slots[up];
+    valid = slot != kNoSlot;
+    s = valid ? (uint64_t)slot * m.sd3b : 0ull;
+}
+
 // In-block part: the sum pc of the three per-axis in-block byte-offset
 // contributions and the in-block coordinates.  An axis whose coordinate
 // equals sub_dim contributes kCarryPoison, so s + pc >= carry_limit flags a
@@ -198,7 +234,7 @@ __device__ __forceinline__ void fast_inblock(const DevMap& m, const AxisEntry* _
 
 // Wrap tier for one lane (|o| < R on every axis): the reference's index,
 // data_size when rejected.  neg is the LDS copy of the wrap-band table.
-template <bool kU24, bool kRank>
+template <bool kU24, bool kRank, bool kWide>
 __device__ __forceinline__ uint64_t wrap_index(const DevMap& m, const SubEntry* __restrict__ tab,
                                                const uint64_t* __restrict__ neg, const OccWord* __restrict__ occ,
                                                float ox, float oy, float oz)
@@ -209,8 +245,8 @@ __device__ __forceinline__ uint64_t wrap_index(const DevMap& m, const SubEntry* 
                       (((uint32_t)iy < m.gthr_y) | (ny & ((m.gneg & 2u) != 0u))) &
                       (((uint32_t)iz < m.gthr_z) | (nz & ((m.gneg & 4u) != 0u)));
     const int32_t up = ix + mul_i<kU24>(iy, (int32_t)m.up_dim_x) + mul_i<kU24>(iz, (int32_t)m.up_dim_2);
-    int32_t s = -1;
-    if (grid & ((uint32_t)up < m.occ)) s = block_offset<kRank>(m, occ, (uint32_t)up);
+    int64_t s = -1;
+    if (grid & ((uint32_t)up < m.occ)) s = block_offset<kRank, kWide>(m, occ, (uint32_t)up);
     const float fx = ox - (float)ix, fy = oy - (float)iy, fz = oz - (float)iz;
     int32_t ex = (int32_t)sub_index(m.sub_base, m.sub_shift, tab, fabsf(fx));
     int32_t ey = (int32_t)sub_index(m.sub_base, m.sub_shift, tab, fabsf(fy));
@@ -220,7 +256,7 @@ __device__ __forceinline__ uint64_t wrap_index(const DevMap& m, const SubEntry* 
     ez = nz ? 0 : (fz < 0.0f ? -ez : ez);
     const int32_t small = ex + mul_i<kU24>(ey, m.sub_dim) + mul_i<kU24>(ez, m.sub_dim_2);
     const uint32_t mask = (nx ? 1u : 0u) | (ny ? 2u : 0u) | (nz ? 4u : 0u);
-    const uint64_t idx = (uint64_t)((int64_t)s + (int64_t)small) + neg[mask];
+    const uint64_t idx = (uint64_t)(s + (int64_t)small) + neg[mask];
     return s >= 0 ? idx : m.data_size;
 }
 

@@ -63,12 +63,17 @@ class CudaSubVoxelMap:
 
     Holds the MapCoef, the int32 block table and either one float per voxel
     (the reference's data_) or int16 TSDF millimetres plus the likelihood
-    parameters, from which the device map is built.
+    parameters, from which the device map is built.  A block table given as
+    int64 is the 64-bit form (maps past 2^31 voxels, int16 millimetres only,
+    tsl_map_create_tsdf_wide).
     """
 
     def __init__(self, coef, grid_occ, data=None, tsdf_mm=None, truncation_mm=None, sigma=0.1):
         self._coef = coef
-        self.grid_occ = np.ascontiguousarray(grid_occ, dtype=np.int32)
+        self.wide = np.asarray(grid_occ).dtype == np.int64
+        self.grid_occ = np.ascontiguousarray(grid_occ, dtype=np.int64 if self.wide else np.int32)
+        if self.wide and data is not None:
+            raise ValueError("the 64-bit map form takes int16 TSDF millimetres")
         self.data = None if data is None else np.ascontiguousarray(data, dtype=np.float32)
         self.tsdf_mm = None if tsdf_mm is None else np.ascontiguousarray(tsdf_mm, dtype=np.int16)
         self.truncation_mm = truncation_mm
@@ -125,6 +130,10 @@ class DeviceMap:
         if host_map.data is not None:
             check(lib.tsl_map_create(ctypes.byref(c), ptr(host_map.grid_occ), ptr(host_map.data),
                                      ctypes.byref(self.handle)))
+        elif host_map.wide:
+            check(lib.tsl_map_create_tsdf_wide(ctypes.byref(c), ptr(host_map.grid_occ), ptr(host_map.tsdf_mm),
+                                               int(host_map.truncation_mm), float(host_map.sigma),
+                                               ctypes.byref(self.handle)))
         else:
             check(lib.tsl_map_create_tsdf(ctypes.byref(c), ptr(host_map.grid_occ), ptr(host_map.tsdf_mm),
                                           int(host_map.truncation_mm), float(host_map.sigma),

@@ -347,13 +347,41 @@ class MixedWorkload:
 
 
 def mixed(seed=0x5EED0005, maps=4, tracking=9, tracking_particles=5_000, tracking_points=10_000,
-          global_particles=1_000_000, global_points=4_000, beams=64, azimuths=1024):
-    """Several building maps served side by side, each with `tracking`
-    robots (particle sets of tracking_particles around their poses, scans of
+          global_particles=1_000_000, global_points=4_000, beams=64, azimuths=1024, campus_tiles=(12, 11)):
+    """Several maps served side by side, each with `tracking` robots
+    (particle sets of tracking_particles around their poses, scans of
     tracking_points) and one global-localisation robot (global_particles
-    over the free space, a scan of global_points).  Maps are the
-    global-localisation building (51.2 x 51.2 x 6.4 m at 0.05 m) with a
-    different seed each."""
+    over the free space, a scan of global_points).  Each map is a campus of
+    campus_tiles building tiles (four seeded buildings per map; at 12 x 11
+    tiles ~7.3e9 int16 voxels, ~14.6 GB, block offsets past 2^31: the 64-bit
+    map form).  campus_tiles=None: each map is one global-localisation
+    building (51.2 x 51.2 x 6.4 m) instead."""
+    if campus_tiles is None:
+        return _mixed_buildings(seed, maps, tracking, tracking_particles, tracking_points, global_particles,
+                                global_points, beams, azimuths)
+    rng = np.random.default_rng(seed)
+    tf = tf_from_xyzrpy(**SCANNER_TF)
+    out_maps, out_tracking, out_global = [], [], []
+    for k in range(maps):
+        s = seed + 101 * k
+        cp = campus(s, tiles=campus_tiles)
+        out_maps.append(cp.map)
+        robots = []
+        for i in range(tracking):
+            pose, pts = campus_scan(cp, rng, tf, tracking_points, beams, azimuths)
+            parts = particles_normal(s + 7 * i, tracking_particles, pose,
+                                     [0.2, 0.2, 0.05, math.radians(5), math.radians(1), math.radians(5)])
+            robots.append(Robot(parts, pts, pose))
+        out_tracking.append(robots)
+        pose, pts = campus_scan(cp, rng, tf, global_points, beams, azimuths)
+        out_global.append(Robot(campus_particles(cp, s + 5, global_particles, math.radians(3.0)), pts, pose))
+    return MixedWorkload("mixed", out_maps, tf, out_tracking, out_global,
+                         {"maps": maps, "tracking_per_map": tracking, "campus_tiles": list(campus_tiles),
+                          "map_bytes": [int(m.dataSize()) * 2 for m in out_maps]})
+
+
+def _mixed_buildings(seed, maps, tracking, tracking_particles, tracking_points, global_particles, global_points,
+                     beams, azimuths):
     rng = np.random.default_rng(seed)
     tf = tf_from_xyzrpy(**SCANNER_TF)
     sx, sy = 51.2, 51.2
@@ -386,3 +414,117 @@ def mixed(seed=0x5EED0005, maps=4, tracking=9, tracking_particles=5_000, trackin
     return MixedWorkload("mixed", out_maps, tf, out_tracking, out_global,
                          {"maps": maps, "tracking_per_map": tracking,
                           "map_bytes": [int(m.dataSize()) * 2 for m in out_maps]})
+
+
+# ---------------------------------------------------------------------------
+# Campus maps: many building tiles side by side, past 2^31 voxels
+# ---------------------------------------------------------------------------
+
+TILE_BLOCKS = 52  # a building tile spans 52 one-metre blocks in x and y
+
+
+@dataclass
+class Campus:
+    map: CudaSubVoxelMap          # 64-bit form: int64 block offsets, int16 millimetres
+    tiles: tuple                  # (tiles in x, tiles in y)
+    tile_of: np.ndarray           # [ty, tx] -> base building index
+    scenes: list                  # the base buildings' scenes (for ray casts and free-space samples)
+
+
+def campus(seed, tiles=(12, 11), bases=4, resolution=0.05, truncation_mm=300, size=(51.2, 51.2, 6.4),
+           room_size=10.0, objects_per_room=4, ceiling_fraction=0.3, wall_h=3.0):
+    """A sparse TSDF map of tiles[0] x tiles[1] building tiles, each one of
+    `bases` seeded buildings (the global-localisation building layout),
+    placed every 52 m: block (bx, by, bz) of the campus is block
+    (bx mod 52, by mod 52, bz) of its tile's building, so its voxels are that
+    block's voxels.  Block offsets are assigned in linear block order as
+    setData does, past 2^31 voxels for large campuses (the 64-bit form).
+    At 12 x 11 tiles of the ~7,200-block building: ~950,000 blocks,
+    ~7.6e9 voxels, ~15 GB of int16."""
+    sx, sy, sz = size
+    rng = np.random.default_rng(seed)
+    base_maps, scenes = [], []
+    for b in range(bases):
+        sc = Scene()
+        _s.synth_scene_build_building(sc.h, seed + 1000 * b, sx, sy, wall_h, room_size, objects_per_room,
+                                      ceiling_fraction)
+        base_maps.append(sc.tsdf_map((0, 0, 0), (sx, sy, sz), resolution, truncation_mm, dense=False))
+        scenes.append(sc)
+    c0 = base_maps[0].coef()
+    assert c0.up_dim_x == TILE_BLOCKS and c0.up_dim_y == TILE_BLOCKS
+    sd3 = int(c0.sub_dim) ** 3
+    nx, ny = tiles
+    tile_of = rng.integers(0, bases, (ny, nx))
+    init = c0.init_value
+    c = coef_init((0, 0, 0), (TILE_BLOCKS * nx - 0.8, TILE_BLOCKS * ny - 0.8, sz), resolution, init)
+    UX, UY, UZ = int(c.up_dim_x), int(c.up_dim_y), int(c.up_dim_z)
+    assert (UX, UY, UZ) == (TILE_BLOCKS * nx, TILE_BLOCKS * ny, int(c0.up_dim_z))
+    # every campus block's block in the stack of the bases' blocks (-1: empty)
+    starts = np.cumsum([0] + [int(m.dataSize()) // sd3 for m in base_maps])
+    grids = np.stack([m.grid_occ for m in base_maps])            # [base, 52*52*UZ]
+    bz, by, bx = np.meshgrid(np.arange(UZ), np.arange(UY), np.arange(UX), indexing="ij")
+    tile = tile_of[by // TILE_BLOCKS, bx // TILE_BLOCKS].ravel()
+    local = ((bx % TILE_BLOCKS) + (by % TILE_BLOCKS) * TILE_BLOCKS + bz * TILE_BLOCKS * TILE_BLOCKS).ravel()
+    off = grids[tile, local].astype(np.int64)
+    alloc = off >= 0
+    src = (starts[tile] + off // sd3)[alloc]
+    grid64 = np.full(UX * UY * UZ, -1, np.int64)
+    grid64[alloc] = np.arange(int(alloc.sum()), dtype=np.int64) * sd3
+    stack = np.concatenate([m.tsdf_mm for m in base_maps]).reshape(-1, sd3)
+    mm = stack[src].reshape(-1)
+    c.data_size = mm.size
+    m = CudaSubVoxelMap(c, grid64, tsdf_mm=mm, truncation_mm=truncation_mm, sigma=base_maps[0].sigma)
+    return Campus(m, (nx, ny), tile_of, scenes)
+
+
+def campus_pose(cp, rng, clearance=0.8):
+    """A free pose somewhere on the campus: a tile, then a free spot of its
+    building; returns the campus pose and the tile's base scene and offset."""
+    nx, ny = cp.tiles
+    for _ in range(1000):
+        tx, ty = int(rng.integers(0, nx)), int(rng.integers(0, ny))
+        sc = cp.scenes[cp.tile_of[ty, tx]]
+        x, y = rng.uniform(2, 49.2), rng.uniform(2, 49.2)
+        if sc.sdf([[x, y, 1.0]])[0] > clearance:
+            break
+    ox, oy = TILE_BLOCKS * tx, TILE_BLOCKS * ty
+    return np.array([ox + x, oy + y, 1.0, 0.0, 0.0, rng.uniform(-math.pi, math.pi)]), sc, (ox, oy)
+
+
+def campus_particles(cp, seed, n, rp):
+    """n particles uniform over the campus's free space (a tile per particle,
+    then a free spot of its building), yaw uniform, roll/pitch within rp."""
+    rng = np.random.default_rng(seed)
+    nx, ny = cp.tiles
+    t = rng.integers(0, nx * ny, n)
+    out = np.empty((n, 7), np.float32)
+    for b, sc in enumerate(cp.scenes):
+        for ti in np.unique(t):
+            tx, ty = int(ti % nx), int(ti // nx)
+            if cp.tile_of[ty, tx] != b:
+                continue
+            sel = np.nonzero(t == ti)[0]
+            p = sc.particles_free(seed + int(ti), sel.size, (0.5, 0.5, 0.5), (50.7, 50.7, 2.0), 0.3, rp)
+            p[:, 0] += TILE_BLOCKS * tx
+            p[:, 1] += TILE_BLOCKS * ty
+            out[sel] = p
+    out[:, 6] = 1.0 / n
+    return out
+
+
+def campus_scan(cp, rng, tf, points, beams=64, azimuths=1024):
+    """A robot at a free campus pose and its scan (ray cast in its tile's
+    building, voxel-filtered at 0.05 m, `points` of them): (pose, points)."""
+    for _ in range(100):
+        pose, sc, (ox, oy) = campus_pose(cp, rng)
+        local = pose.copy()
+        local[0] -= ox
+        local[1] -= oy
+        raw, rings = sc.raycast(compose_pose(local, tf), beams, azimuths, -22.5, 22.5, 60.0, 0.01,
+                                int(rng.integers(0, 1 << 30)))
+        pts, rings = voxel_filter(raw, rings, 0.05, 1.0)
+        if pts.shape[0] >= points:
+            break
+    if pts.shape[0] > points:
+        pts = pts[np.sort(rng.choice(pts.shape[0], points, replace=False))]
+    return pose, np.ascontiguousarray(pts, np.float32)
