@@ -421,18 +421,20 @@ def bench_serving(args):
 
 
 def bench_mixed(args):
-    """Four building maps served side by side on one GPU (one server per
-    map), requests from a Poisson process split 9:1 between tracking robots
-    (5,000 particles x 10,000 points) and global-localisation robots
-    (1,000,000 particles x 4,000 points): p50/p99 latency per class under
-    that load, the saturation throughput of the served form (every request
-    submitted at once) and plain per-request launches of the same requests."""
+    """Four campus maps (12 x 11 building tiles, ~7.3e9 int16 voxels, ~14.6 GB
+    each, block offsets past 2^31) served side by side on one GPU (one
+    server per map), requests from a Poisson process split 9:1 between
+    tracking robots (5,000 particles x 10,000 points) and global-localisation
+    robots (1,000,000 particles x 4,000 points): p50/p99 latency per class
+    under that load, the saturation throughput of the served form (every
+    request submitted at once) and plain per-request launches of the same
+    requests."""
     import tsdf_localization_amd as tsl
     from tsdf_localization_amd import synth
     from tsdf_localization_amd.server import Server
 
     t0 = time.time()
-    w = synth.mixed()
+    w = synth.mixed(campus_tiles=None if args.mixed_small_maps else (12, 11))
     M = len(w.maps)
     T = len(w.tracking[0])
     log(f"workload mixed: {M} maps of {w.meta['map_bytes'][0] / 1e6:.0f} MB, {T} tracking + 1 global robot per map "
@@ -457,6 +459,7 @@ def bench_mixed(args):
         for k in range(M):
             srv = Server(evs[k], hbm_budget=args.hbm_budget_gb * (1 << 30), max_robots=T + 1, max_particles=gp,
                          max_batch=8, chunk_len=2048)
+            srv.set_slice(args.mixed_slice, args.mixed_reserve)
             trk = [srv.add_robot(r.particles, r.points.shape[0], w.tf) for r in w.tracking[k]]
             glb = srv.add_robot(w.global_robots[k].particles, w.global_robots[k].points.shape[0], w.tf)
             srvs.append(srv)
@@ -549,12 +552,14 @@ def bench_mixed(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (four seeded building maps, ray-cast scans, particles around each tracking robot and "
-                "over the free space for each global robot)",
+        "data": "synthetic (four seeded campus maps of building tiles, ray-cast scans, particles around each "
+                "tracking robot and over the free space for each global robot)",
         "config": {"workload": "mixed", "maps": M, "map_bytes_each": w.meta["map_bytes"][0],
-                   "map": "sparse 1024x1024x128 @0.05 m int16 each",
+                   "map": ("sparse 1024x1024x128 @0.05 m int16 each" if args.mixed_small_maps else
+                           "campus of 12x11 building tiles, 624x572x6.4 m @0.05 m int16 each (64-bit block offsets)"),
                    "classes": {"tracking": "5000 particles x 10000 points", "global": "1000000 x 4000"},
-                   "class_ratio": "9:1", "requests": n_req, "arrival_rate_per_s": rate},
+                   "class_ratio": "9:1", "requests": n_req, "arrival_rate_per_s": rate,
+                   "global_slice_pairs": args.mixed_slice, "global_reserved_slots_per_cu": args.mixed_reserve},
         "latency_ms_by_class": classes,
         "failed": int(bad),
         "plain_value": plain_rate,
@@ -571,6 +576,12 @@ def main():
     ap.add_argument("--workload", default="building", choices=["building", "room", "batched", "serving", "mixed"])
     ap.add_argument("--mixed-requests", type=int, default=400, help="mixed: number of requests")
     ap.add_argument("--mixed-rate", type=float, default=50.0, help="mixed: Poisson arrival rate (requests/s)")
+    ap.add_argument("--mixed-slice", type=int, default=1 << 27,
+                    help="mixed: particle-point pairs per launch of a global robot's evaluation (0 = one launch)")
+    ap.add_argument("--mixed-reserve", type=int, default=2,
+                    help="mixed: workgroup slots per CU a sliced global evaluation leaves to tracking requests")
+    ap.add_argument("--mixed-small-maps", action="store_true",
+                    help="mixed: one 51.2 m building per map instead of a 12 x 11-tile campus")
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--extra-rates", type=float, nargs="*", default=[1000.0, 2000.0],

@@ -557,6 +557,22 @@ int tsl_server_get_particles(tsl_server* s, uint32_t robot, tsl_particle* out, s
 
 int tsl_server_stats(tsl_server* s, uint64_t* used_bytes, uint64_t* ticks, uint64_t* requests);
 
+/* Robots of 65,536 particle slots and more (global localisation) are served
+ * on a stream of their own, apart from the others (every server of a device
+ * shares the two streams), and their evaluation is cut into launches of
+ * about `pairs` particle-point pairs on a grid that leaves reserved_per_cu
+ * workgroup slots per compute unit free, so other requests' kernels start
+ * within about one launch (default 2^27 pairs, 2 slots; pairs = 0: one
+ * launch per tick on the whole grid).  Results are the same bits either way.
+ * Takes effect for ticks started afterwards. */
+int tsl_server_set_slice(tsl_server* s, uint64_t pairs, uint32_t reserved_per_cu);
+
+/* Evaluation tile of the bulk class (bulk != 0) or of the others: k
+ * particles per wavefront (1, 2) and points per lane (1, 2, 4); default 2 and
+ * 1 for both.  Changes only the speed (a member's chunking is the server's
+ * chunk_len).  Takes effect for ticks started afterwards. */
+int tsl_server_set_tile(tsl_server* s, int bulk, int k, int points_per_lane);
+
 /* ---- .mcl snapshot files ------------------------------------------------ */
 
 /* MCLFile::write (src/util/mcl_file.cpp:14-64): points, their rings,

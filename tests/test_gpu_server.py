@@ -175,3 +175,118 @@ def test_server_tick_scan_preparation_paths_agree(tsl, room):
             assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32)), i
     finally:
         srv.close()
+
+
+def test_bulk_robot_sliced_equals_unsliced_and_tracking_interleaves(tsl):
+    """A robot of 70,000 particles (the bulk class: its own stream, its
+    evaluation cut into launches) gives the same bits with slices of 2^22
+    pairs as in one launch, and as the plain filter; a tracking robot's
+    request submitted right after it completes on the other stream."""
+    from tsdf_localization_amd import synth
+    from tsdf_localization_amd.server import Server
+    w = synth.building(num_particles=70_000, target_points=6000)
+    ev = tsl.TSDFEvaluator(w.map)
+    trk = w.particles[:800].copy()
+    trk[:, 6] = 1.0 / trk.shape[0]
+    motion = _motion(tsl, 7)
+    u0 = np.float32(0.29 / w.particles.shape[0])
+    outs = []
+    for slice_pairs in (1 << 22, 0):
+        srv = Server(ev, max_robots=4, max_particles=70_000, max_batch=4, chunk_len=2048)
+        try:
+            srv.set_slice(slice_pairs)
+            big = srv.add_robot(w.particles, w.points.shape[0], w.tf)
+            small = srv.add_robot(trk, w.points.shape[0], w.tf)
+            srv.submit(big, motion, w.points, u0, request_id=1)
+            srv.submit(small, _motion(tsl, 8), w.points[:2000], np.float32(0.5 / 800), request_id=2)
+            res = {r["id"]: r for r in srv.wait(2)}
+            assert res[1]["status"] == 0 and res[2]["status"] == 0
+            if slice_pairs:
+                # the tracking request ran on the other stream, not behind the bulk one
+                assert res[2]["done_ms"] <= res[1]["done_ms"]
+            outs.append((res[1]["pose"], srv.get_particles(big)))
+        finally:
+            srv.close()
+    assert outs[0][0] == outs[1][0]
+    assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
+    # the plain filter with the same chunk
+    s = tsl.Stream()
+    f = tsl.DeviceFilter(ev, w.particles.shape[0])
+    f.set_particles(w.particles)
+    f.set_tile(2, 2048, 1)
+    f.motion_async(motion, s)
+    f.set_scan(w.points, w.tf)
+    f.update_async(s)
+    pose, _ = f.sync(s)
+    f.resample_async(u0, s)
+    f.sync(s)
+    assert outs[0][0] == pose
+    assert np.array_equal(outs[0][1].view(np.uint32), f.get_particles().view(np.uint32))
+
+
+def test_remove_while_submitting_and_reading(tsl, room):
+    """Requests submitted from one thread and particles read from another
+    while the robot is removed: submits after the removal fail with "no
+    robot", reads never see a freed set, and the server then closes."""
+    import threading
+    import time
+    from tsdf_localization_amd.server import Server
+    ev = tsl.TSDFEvaluator(room.map)
+    srv = Server(ev, max_robots=4, max_particles=2000, max_batch=4)
+    rid = srv.add_robot(room.particles, room.points.shape[0], room.tf)
+    other = srv.add_robot(room.particles[:500], room.points.shape[0], room.tf)
+    stop = threading.Event()
+    errors = {"submit": 0, "read": 0, "read_ok": 0}
+
+    def submitter():
+        i = 0
+        while not stop.is_set():
+            try:
+                srv.submit(rid, _motion(tsl, i), room.points, np.float32(0.3 / 1000), request_id=i)
+            except tsl.TslError:
+                errors["submit"] += 1
+            i += 1
+            time.sleep(0.0005)
+
+    def reader():
+        while not stop.is_set():
+            try:
+                p = srv.get_particles(rid)
+                assert p.shape[1] == 7 and p.shape[0] > 0
+                errors["read_ok"] += 1
+            except tsl.TslError:
+                errors["read"] += 1
+            time.sleep(0.001)
+
+    ts = [threading.Thread(target=submitter), threading.Thread(target=reader)]
+    for t in ts:
+        t.start()
+    time.sleep(0.2)
+    removed = False
+    t_end = time.time() + 20.0
+    while time.time() < t_end:
+        srv.poll(1024, wait_ms=1)
+        try:
+            srv.remove_robot(rid)
+            removed = True
+            break
+        except tsl.TslError as e:
+            assert "pending" in str(e)
+    time.sleep(0.1)
+    stop.set()
+    for t in ts:
+        t.join(timeout=30)
+        assert not t.is_alive()
+    assert removed, "the robot was never without pending requests"
+    assert errors["submit"] > 0 and errors["read_ok"] > 0, errors
+    srv.submit(other, _motion(tsl, 1), room.points, np.float32(0.3 / 500), request_id=10**6)
+    res = []
+    for _ in range(400):
+        res += [r for r in srv.poll(1024, wait_ms=20) if r["id"] == 10**6]
+        if res:
+            break
+    assert res and res[0]["status"] == 0
+    closer = threading.Thread(target=srv.close)
+    closer.start()
+    closer.join(timeout=60)
+    assert not closer.is_alive(), "server did not close"

@@ -141,6 +141,8 @@ _SIGS = {
     "tsl_server_get_particles": (_I, [_P, ctypes.c_uint32, _P, _S, ctypes.POINTER(_S)]),
     "tsl_server_stats": (_I, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                               ctypes.POINTER(ctypes.c_uint64)]),
+    "tsl_server_set_slice": (_I, [_P, ctypes.c_uint64, ctypes.c_uint32]),
+    "tsl_server_set_tile": (_I, [_P, _I, _I, _I]),
     "tsl_mcl_write": (_I, [ctypes.c_char_p, _P, _P, _S, _P, _S, _P, _P]),
     "tsl_mcl_read": (_I, [ctypes.c_char_p, _P, _P, _S, _P, _S, _P, _P, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
     "tsl_filter_set_scan_raw": (_I, [_P, _P, _P, _S, _F, _P]),

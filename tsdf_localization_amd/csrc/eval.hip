@@ -682,22 +682,27 @@ __global__ __launch_bounds__(1024) void k_batch_plan(const Member* __restrict__ 
 // workgroups of XCD x (dealt round-robin by index) from the items x, x + 8,
 // ... through their XCD's counter, so all workgroups sweep the members
 // together, as plain per-filter launches would, and the tail balances
-// itself; per item the same wave body as k_eval.
+// itself; per item the same wave body as k_eval.  One launch covers the
+// slice [item0, item0 + count) of the plan's items (its claim counters
+// start at zero), so a long evaluation can be cut into launches that other
+// streams' kernels interleave with; every item writes only its own partial
+// weights, so the slicing changes no bit.
 template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false>
 __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member* __restrict__ members,
-                                                       BatchPlan* __restrict__ plan, float a_hit)
+                                                       BatchPlan* __restrict__ plan, float a_hit, uint32_t item0,
+                                                       uint32_t count)
 {
     const LdsTables t = stage_tables<kRank>(map, true, a_hit);
     float4* tile = t.tile;
     uint32_t* claim = t.scratch;
     const uint32_t n = plan->n;
-    const uint32_t total = plan->total;
+    const uint32_t total = min(plan->total, item0 + min(count, 0xffffffffu - item0));
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t* counter = &plan->next[(blockIdx.x % kPlanXcds) * 32];
     const uint32_t xcd = blockIdx.x % kPlanXcds;
     for (;;) {
         if (threadIdx.x == 0) {
-            const uint32_t item = atomicAdd(counter, 1u) * kPlanXcds + xcd;
+            const uint32_t item = item0 + atomicAdd(counter, 1u) * kPlanXcds + xcd;
             uint32_t lo = 0, hi = n;  // member: prefix[lo] <= item < prefix[lo + 1]
             if (item < total) {
                 while (hi - lo > 1) {
@@ -1164,6 +1169,11 @@ int launch_set_count(tsl_filter* f, uint32_t n, hipStream_t stream)
 namespace {
 
 // stream == nullptr && grid_out: report the persistent grid size only.
+__global__ void k_batch_claims_reset(BatchPlan* __restrict__ plan)
+{
+    if (threadIdx.x < kPlanXcds) plan->next[threadIdx.x * 32] = 0;
+}
+
 template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false>
 int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid_out)
 {
@@ -1184,9 +1194,18 @@ int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid
         *grid_out = (uint32_t)grid;
         return TSL_OK;
     }
-    hipLaunchKernelGGL((k_eval_batch<K, kP, kU24, kBrick, kRank, kWide>), dim3(b.grid), dim3(kBlock), lds, stream,
-                       b.map->dev, b.members, b.plan, b.a_hit);
-    TSL_HIP_CHECK(hipGetLastError());
+    const uint64_t slice = b.slice_items ? b.slice_items : (uint64_t)0xffffffffu;
+    const uint64_t end = b.slice_items ? std::min<uint64_t>(b.max_items, 0xffffffffull) : 1;
+    for (uint64_t i0 = 0; i0 < end; i0 += slice) {
+        if (i0) {
+            hipLaunchKernelGGL(k_batch_claims_reset, dim3(1), dim3(64), 0, stream, b.plan);
+            TSL_HIP_CHECK(hipGetLastError());
+        }
+        hipLaunchKernelGGL((k_eval_batch<K, kP, kU24, kBrick, kRank, kWide>), dim3(b.grid), dim3(kBlock), lds,
+                           stream, b.map->dev, b.members, b.plan, b.a_hit, (uint32_t)i0,
+                           (uint32_t)std::min<uint64_t>(slice, 0xffffffffu));
+        TSL_HIP_CHECK(hipGetLastError());
+    }
     return TSL_OK;
 }
 

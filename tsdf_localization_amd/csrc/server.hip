@@ -9,6 +9,18 @@
 // run are admitted into the next tick at once.  Robots are admitted against
 // an HBM budget.  Each robot's result is the same whether it runs alone or
 // in a mix (member-indexed kernels, counter-based motion noise).
+//
+// Robots come in two classes: "bulk" robots (particle sets of 65,536 slots
+// and more, e.g. global localisation over a whole map) and the rest
+// ("tracking").  Each class has its own stream and its own ticks, so a
+// tracking request never queues behind a bulk one; a bulk tick's evaluation
+// is cut into slices of a bounded number of particle-point pairs, launched
+// one after the other on a grid that leaves two workgroup slots per compute
+// unit free, so tracking kernels (of this server or of another server on
+// the same GPU) start within about a slice.  Slicing changes no result bit.
+// The two class streams are shared by every server of a device (one per
+// map, say): a process then uses two hardware queues for serving, so a
+// tracking stream never shares a queue with a bulk one.
 // No counterpart in the reference (one robot per ROS node, src/mcl_3d.cpp).
 #include "tsl_internal.h"
 
@@ -26,7 +38,9 @@ using namespace tsl;
 
 namespace {
 
-constexpr int kMaxInflight = 3;
+constexpr int kMaxInflight = 3;    // ticks in flight per class
+constexpr int kClasses = 2;        // 0 tracking, 1 bulk
+constexpr uint64_t kDefaultSlicePairs = 1ull << 27;  // ~1 ms of evaluation per bulk slice on an HBM-resident map
 constexpr uint32_t kSegmentedScans = 2;  // ticks with this many scans prepare them together (two device-wide sorts)
 // Robots with at least this many particle slots are evaluated in spatial
 // particle order (a global-localisation set spread over the whole map: the
@@ -38,8 +52,10 @@ struct Robot {
     tsl_filter* f = nullptr;
     uint32_t max_points = 0;
     uint64_t bytes = 0;
-    bool active = false;
-    bool busy = false;
+    bool active = false;    // accepts requests
+    bool busy = false;      // in a tick, or its particles being read
+    bool removing = false;  // being removed: neither active nor free for a new robot
+    bool bulk = false;      // evaluated in the bulk class
 };
 
 struct Pending {
@@ -66,7 +82,46 @@ struct Slot {  // one tick in flight
     hipEvent_t done = nullptr;
     double start_ms = 0.0;
     bool used = false;
+    int cls = 0;                         // robot class of the tick
 };
+
+// The class streams of a device, shared by its servers (reference counted).
+constexpr int kMaxDevices = 64;
+struct ClassStreams {
+    int refs = 0;
+    hipStream_t s[kClasses] = {nullptr, nullptr};
+};
+std::mutex g_streams_mu;
+ClassStreams g_streams[kMaxDevices];
+
+int acquire_streams(int device, hipStream_t out[kClasses])
+{
+    if (device < 0 || device >= kMaxDevices) return set_error(TSL_ERR_UNSUPPORTED, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_streams_mu);
+    ClassStreams& cs = g_streams[device];
+    if (cs.refs == 0) {
+        for (int c = 0; c < kClasses; ++c)
+            if (hipStreamCreateWithFlags(&cs.s[c], hipStreamNonBlocking) != hipSuccess) {
+                for (int k = 0; k < c; ++k) (void)hipStreamDestroy(cs.s[k]);
+                return set_error(TSL_ERR_HIP, "hipStreamCreate failed");
+            }
+    }
+    ++cs.refs;
+    for (int c = 0; c < kClasses; ++c) out[c] = cs.s[c];
+    return TSL_OK;
+}
+
+void release_streams(int device)
+{
+    std::lock_guard<std::mutex> lk(g_streams_mu);
+    ClassStreams& cs = g_streams[device];
+    if (cs.refs <= 0 || --cs.refs > 0) return;
+    for (int c = 0; c < kClasses; ++c) {
+        (void)hipStreamSynchronize(cs.s[c]);
+        (void)hipStreamDestroy(cs.s[c]);
+        cs.s[c] = nullptr;
+    }
+}
 
 __global__ void k_gather_states_s(const Member* __restrict__ members, uint32_t n, FilterState* __restrict__ out)
 {
@@ -82,7 +137,8 @@ struct tsl_server {
     uint64_t budget = 0, used = 0;
     uint32_t max_batch = 64;
     uint32_t chunk_len = 0;        // 0: each request's whole scan is one chunk
-    int K = 2;                     // particles per wavefront of the batched evaluation
+    int K[kClasses] = {2, 2};      // particles per wavefront of the batched evaluation, per class
+    int P[kClasses] = {1, 1};      // points per lane, per class
     uint64_t max_particles = 0;    // per robot; sizes the shared reduction grid
     std::atomic<uint32_t> max_points{0};  // largest scan any admitted robot may send
     std::vector<Robot> robots;     // fixed size (max_robots): entries never move
@@ -90,14 +146,17 @@ struct tsl_server {
     std::condition_variable cv_req, cv_done;
     std::deque<Pending> pending;
     std::deque<tsl_result> done;
-    std::deque<Slot*> inflight;
-    std::vector<Slot*> free_slots;
-    Slot slots[kMaxInflight];
-    hipStream_t stream = nullptr;
+    std::deque<Slot*> inflight[kClasses];
+    std::vector<Slot*> free_slots[kClasses];
+    Slot slots[kClasses * kMaxInflight];
+    hipStream_t stream[kClasses] = {nullptr, nullptr};
+    uint64_t slice_pairs = kDefaultSlicePairs;  // bulk evaluation pairs per launch (0: no slicing)
+    uint32_t reserved_per_cu = 2;  // workgroup slots per CU a sliced bulk evaluation leaves to the other class
     std::thread worker;
     std::atomic<bool> stop{false};
+    bool stopped = false;          // worker: stop requested and nothing left (worker thread only)
     std::chrono::steady_clock::time_point t0;
-    int per_cu = 0, cus = 0;
+    int per_cu[kClasses] = {0, 0}, cus = 0;
     int worker_error = TSL_OK;
     uint64_t ticks = 0, batched_requests = 0;
 
@@ -135,7 +194,7 @@ void complete(tsl_server* s, Slot* sl)
     sl->reqs.clear();
     sl->robots.clear();
     sl->used = false;
-    s->free_slots.push_back(sl);
+    s->free_slots[sl->cls].push_back(sl);
     s->cv_done.notify_all();
 }
 
@@ -167,7 +226,7 @@ int slot_reserve_scans(tsl_server* s, Slot& sl, size_t total)
 // Enqueue one tick for the selected requests (robots marked busy already).
 int dispatch(tsl_server* s, Slot* sl)
 {
-    hipStream_t st = s->stream;
+    hipStream_t st = s->stream[sl->cls];
     const uint32_t n = (uint32_t)sl->reqs.size();
     uint64_t max_items = 0;
     uint32_t max_cap = 0;
@@ -245,7 +304,7 @@ int dispatch(tsl_server* s, Slot* sl)
         h.cap_limit = (uint32_t)f->capacity;
         sl->u0_host[i] = pd.req.u0;
         sl->motions_host[i] = pd.req.motion;
-        max_items += (f->capacity + 4 * s->K - 1) / (4 * s->K) * chunks;
+        max_items += (f->capacity + 4 * s->K[sl->cls] - 1) / (4 * s->K[sl->cls]) * chunks;
         max_cap = std::max<uint32_t>(max_cap, (uint32_t)f->capacity);
     }
     TSL_HIP_CHECK(hipMemcpyAsync(sl->members.ptr, sl->members_host, sizeof(Member) * n, hipMemcpyHostToDevice, st));
@@ -257,17 +316,30 @@ int dispatch(tsl_server* s, Slot* sl)
     l.members = sl->members.as<Member>();
     l.plan = sl->plan.as<BatchPlan>();
     l.n = n;
-    l.K = s->K;
-    l.P = 1;
+    l.K = s->K[sl->cls];
+    l.P = s->P[sl->cls];
     l.rank = true;
     l.max_items = max_items;
     l.max_cap = max_cap;
     l.max_rs = max_cap;
-    uint64_t grid = (uint64_t)s->per_cu * (uint64_t)s->cus;
+    // a bulk tick leaves one workgroup slot per compute unit to the other
+    // stream and evaluates in slices of about slice_pairs pairs
+    const int pc = s->per_cu[sl->cls];
+    const int per_cu = (sl->cls == 1 && s->slice_pairs)
+                           ? std::max(1, pc - (int)std::min<uint32_t>(s->reserved_per_cu, (uint32_t)pc))
+                           : pc;
+    uint64_t grid = (uint64_t)per_cu * (uint64_t)s->cus;
     const uint64_t need = (max_items + 7) / 8 * 8;
     if (grid > need) grid = need;
     grid = std::max<uint64_t>(8, (grid + 7) / 8 * 8);
     l.grid = (uint32_t)grid;
+    if (sl->cls == 1 && s->slice_pairs) {
+        uint64_t pairs_per_item = 0;
+        for (uint32_t i = 0; i < n; ++i)
+            pairs_per_item =
+                std::max<uint64_t>(pairs_per_item, (uint64_t)4 * s->K[1] * sl->members_host[i].chunk_len);
+        l.slice_items = std::max<uint64_t>(grid, s->slice_pairs / std::max<uint64_t>(pairs_per_item, 1));
+    }
     TSL_TRY(launch_motion_members(l.members, sl->motions.as<tsl_motion>(), n, max_cap, st));
     for (uint32_t i = 0; i < n; ++i) {
         tsl_filter* f = s->robots[sl->robots[i]].f;
@@ -292,31 +364,37 @@ void worker_main(tsl_server* s)
     (void)hipSetDevice(s->device);
     while (true) {
         bool progressed = false;
-        // completions, oldest first
-        while (!s->inflight.empty()) {
-            Slot* sl = s->inflight.front();
-            const hipError_t q = hipEventQuery(sl->done);
-            if (q == hipErrorNotReady) break;
-            s->inflight.pop_front();
-            if (q != hipSuccess) {
-                std::lock_guard<std::mutex> lk(s->mu);
-                s->worker_error = set_error(TSL_ERR_HIP, "tick failed: %s", hipGetErrorString(q));
+        // completions, oldest first within each class's stream
+        for (int c = 0; c < kClasses; ++c) {
+            while (!s->inflight[c].empty()) {
+                Slot* sl = s->inflight[c].front();
+                const hipError_t q = hipEventQuery(sl->done);
+                if (q == hipErrorNotReady) break;
+                s->inflight[c].pop_front();
+                if (q != hipSuccess) {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    s->worker_error = set_error(TSL_ERR_HIP, "tick failed: %s", hipGetErrorString(q));
+                }
+                complete(s, sl);
+                progressed = true;
             }
-            complete(s, sl);
-            progressed = true;
         }
-        // admit the next tick
+        // admit the next tick of each class
+        for (int c = 0; c < kClasses; ++c) {
         Slot* sl = nullptr;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            if (s->stop && s->pending.empty() && s->inflight.empty()) break;
-            if (!s->free_slots.empty() && !s->pending.empty()) {
-                Slot* cand = s->free_slots.back();
+            if (c == 0 && s->stop && s->pending.empty() && s->inflight[0].empty() && s->inflight[1].empty()) {
+                s->stopped = true;
+                break;
+            }
+            if (!s->free_slots[c].empty() && !s->pending.empty()) {
+                Slot* cand = s->free_slots[c].back();
                 std::vector<uint8_t> taken(s->robots.size(), 0);
                 for (auto it = s->pending.begin(); it != s->pending.end() && cand->reqs.size() < s->max_batch;) {
                     const uint32_t r = it->req.robot;
                     Robot& rb = s->robots[r];
-                    if (rb.active && !rb.busy && !taken[r]) {
+                    if (rb.active && !rb.busy && !taken[r] && (rb.bulk ? 1 : 0) == c) {
                         taken[r] = 1;
                         rb.busy = true;
                         cand->robots.push_back(r);
@@ -327,7 +405,7 @@ void worker_main(tsl_server* s)
                     }
                 }
                 if (!cand->reqs.empty()) {
-                    s->free_slots.pop_back();
+                    s->free_slots[c].pop_back();
                     sl = cand;
                 }
             }
@@ -340,14 +418,16 @@ void worker_main(tsl_server* s)
                 s->worker_error = rc;
             }
             sl->used = true;
-            s->inflight.push_back(sl);
+            s->inflight[c].push_back(sl);
             s->ticks += 1;
             s->batched_requests += sl->reqs.size();
             progressed = true;
         }
+        }
+        if (s->stopped) break;
         if (!progressed) {
             std::unique_lock<std::mutex> lk(s->mu);
-            if (s->inflight.empty()) {
+            if (s->inflight[0].empty() && s->inflight[1].empty()) {
                 s->cv_req.wait_for(lk, std::chrono::milliseconds(2));
             } else {
                 lk.unlock();
@@ -378,6 +458,22 @@ int slot_init(tsl_server* s, Slot& sl)
     TSL_TRY(sl.scan_off.reserve(sizeof(uint32_t) * (n + 1)));
     TSL_TRY(sl.scan_dst.reserve(sizeof(ScanDest) * n));
     TSL_HIP_CHECK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    return TSL_OK;
+}
+
+// Workgroups per compute unit of the batched evaluation kernel a class's
+// tile selects on this map (its persistent grid is that times the CUs).
+int class_occupancy(tsl_server* s, int c)
+{
+    BatchLaunch probe;
+    probe.map = s->ev->map;
+    probe.K = s->K[c];
+    probe.P = s->P[c];
+    probe.rank = true;
+    probe.max_items = ~0ull >> 8;
+    uint32_t g = 0;
+    TSL_TRY(batch_grid(probe, &g));
+    s->per_cu[c] = (int)std::max<uint32_t>(1, g / (uint32_t)s->cus);
     return TSL_OK;
 }
 
@@ -420,30 +516,18 @@ int tsl_server_create(tsl_evaluator* ev, uint64_t hbm_budget_bytes, uint32_t max
     uint64_t map_bytes = 0;
     (void)tsl_map_device_bytes(ev->map, &map_bytes);
     s->used = map_bytes;
-    int rc = TSL_OK;
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
-        rc = set_error(TSL_ERR_HIP, "hipStreamCreate failed");
-    for (int i = 0; i < kMaxInflight && rc == TSL_OK; ++i) {
+    int rc = acquire_streams(s->device, s->stream);
+    if (rc != TSL_OK) s->stream[0] = s->stream[1] = nullptr;
+    for (int i = 0; i < kClasses * kMaxInflight && rc == TSL_OK; ++i) {
         rc = slot_init(s, s->slots[i]);
-        s->free_slots.push_back(&s->slots[i]);
+        s->slots[i].cls = i / kMaxInflight;
+        s->free_slots[i / kMaxInflight].push_back(&s->slots[i]);
     }
     if (rc == TSL_OK) {
-        // persistent grid of the batched evaluation: the occupancy of the
-        // kernel this map selects, on every compute unit
-        BatchLaunch probe;
-        probe.map = ev->map;
-        probe.K = s->K;
-        probe.P = 1;
-        probe.rank = true;
-        probe.max_items = ~0ull >> 8;
-        uint32_t g = 0;
-        rc = batch_grid(probe, &g);
-        if (rc == TSL_OK) {
-            int cus = 0;
-            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ev->device);
-            s->cus = cus > 0 ? cus : 1;
-            s->per_cu = (int)std::max<uint32_t>(1, g / (uint32_t)s->cus);
-        }
+        int cus = 0;
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ev->device);
+        s->cus = cus > 0 ? cus : 1;
+        for (int c = 0; c < kClasses && rc == TSL_OK; ++c) rc = class_occupancy(s, c);
     }
     if (rc != TSL_OK) {
         tsl_server_destroy(s);
@@ -463,7 +547,9 @@ int tsl_server_destroy(tsl_server* s)
         s->worker.join();
     }
     (void)hipSetDevice(s->device);
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    // the worker finished this server's ticks; the streams are shared
+    for (Slot& sl : s->slots)
+        if (sl.done) (void)hipEventSynchronize(sl.done);
     for (Robot& r : s->robots)
         if (r.f) tsl_filter_destroy(r.f);
     for (Slot& sl : s->slots) {
@@ -476,7 +562,7 @@ int tsl_server_destroy(tsl_server* s)
         if (sl.scan_host) (void)hipHostFree(sl.scan_host);
         if (sl.done) (void)hipEventDestroy(sl.done);
     }
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->stream[0]) release_streams(s->device);
     delete s;
     return TSL_OK;
 }
@@ -518,7 +604,8 @@ int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n,
     const uint64_t bytes = filter_bytes(f);
     std::lock_guard<std::mutex> lk(s->mu);
     uint32_t slot = 0;
-    while (slot < s->robots.size() && (s->robots[slot].active || s->robots[slot].busy)) ++slot;
+    while (slot < s->robots.size() && (s->robots[slot].active || s->robots[slot].busy || s->robots[slot].removing))
+        ++slot;
     if (slot == s->robots.size() || (s->budget && s->used + bytes > s->budget)) {
         tsl_filter_destroy(f);
         if (slot == s->robots.size())
@@ -533,6 +620,8 @@ int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n,
     r.bytes = bytes;
     r.active = true;
     r.busy = false;
+    r.removing = false;
+    r.bulk = f->capacity >= kSpatialOrderMin;
     if (max_points > s->max_points.load()) s->max_points.store(max_points);
     f->count_bound = f->capacity;
     f->count_known = false;
@@ -548,12 +637,18 @@ int tsl_server_remove_robot(tsl_server* s, uint32_t robot)
         return set_error(TSL_ERR_INVALID_ARGUMENT, "no robot %u", robot);
     for (const Pending& p : s->pending)
         if (p.req.robot == robot) return set_error(TSL_ERR_INVALID_ARGUMENT, "robot %u has pending requests", robot);
-    s->cv_done.wait(lk, [&] { return !s->robots[robot].busy; });
+    // no longer accepts requests (submit checks active under this lock), and
+    // its slot is not reused while the removal waits for its tick
     Robot& r = s->robots[robot];
     r.active = false;
+    r.removing = true;
+    s->cv_done.wait(lk, [&] { return !r.busy; });
     s->used -= r.bytes;
-    tsl_filter_destroy(r.f);
+    tsl_filter* f = r.f;
     r.f = nullptr;
+    r.removing = false;
+    lk.unlock();
+    tsl_filter_destroy(f);
     return TSL_OK;
 }
 
@@ -599,10 +694,46 @@ int tsl_server_get_particles(tsl_server* s, uint32_t robot, tsl_particle* out, s
     std::unique_lock<std::mutex> lk(s->mu);
     if (robot >= s->robots.size() || !s->robots[robot].active)
         return set_error(TSL_ERR_INVALID_ARGUMENT, "no robot %u", robot);
-    s->cv_done.wait(lk, [&] { return !s->robots[robot].busy; });
-    tsl_filter* f = s->robots[robot].f;
+    // held busy for the copy: no tick takes the robot and a removal waits
+    Robot& r = s->robots[robot];
+    s->cv_done.wait(lk, [&] { return !r.busy; });
+    if (!r.active) return set_error(TSL_ERR_INVALID_ARGUMENT, "no robot %u", robot);
+    r.busy = true;
+    tsl_filter* f = r.f;
     lk.unlock();
-    return tsl_filter_get_particles(f, out, capacity, n);
+    const int rc = tsl_filter_get_particles(f, out, capacity, n);
+    lk.lock();
+    r.busy = false;
+    s->cv_done.notify_all();
+    return rc;
+}
+
+int tsl_server_set_tile(tsl_server* s, int bulk, int k, int points_per_lane)
+{
+    if (!s) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL server");
+    if (k != 1 && k != 2) return set_error(TSL_ERR_INVALID_ARGUMENT, "K must be 1 or 2");
+    if (points_per_lane != 1 && points_per_lane != 2 && points_per_lane != 4)
+        return set_error(TSL_ERR_INVALID_ARGUMENT, "points per lane must be 1, 2 or 4");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int c = bulk ? 1 : 0;
+    const int k0 = s->K[c], p0 = s->P[c];
+    s->K[c] = k;
+    s->P[c] = points_per_lane;
+    const int rc = class_occupancy(s, c);
+    if (rc != TSL_OK) {
+        s->K[c] = k0;
+        s->P[c] = p0;
+    }
+    return rc;
+}
+
+int tsl_server_set_slice(tsl_server* s, uint64_t pairs, uint32_t reserved_per_cu)
+{
+    if (!s) return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->slice_pairs = pairs;
+    s->reserved_per_cu = reserved_per_cu;
+    return TSL_OK;
 }
 
 int tsl_server_stats(tsl_server* s, uint64_t* used_bytes, uint64_t* ticks, uint64_t* requests)

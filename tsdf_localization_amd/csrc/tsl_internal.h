@@ -439,6 +439,7 @@ struct BatchLaunch {
     uint32_t max_rs = 0;         // largest resampling output room
     uint32_t grid = 0;           // persistent evaluation grid (multiple of the XCD count)
     bool rank = false;           // block offsets from the LDS occupancy-rank table
+    uint64_t slice_items = 0;    // evaluation items per launch (0: one launch for all)
 };
 int launch_batch_update(const BatchLaunch& b, hipStream_t stream);
 int batch_grid(const BatchLaunch& b, uint32_t* grid);

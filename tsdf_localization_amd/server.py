@@ -66,6 +66,17 @@ class Server:
         check(lib.tsl_server_get_particles(self.handle, int(robot), ptr(out), out.shape[0], ctypes.byref(n)))
         return out[: n.value].copy()
 
+    def set_slice(self, pairs, reserved_per_cu=2):
+        """Particle-point pairs per launch of a bulk robot's evaluation (robots
+        of 65,536+ particle slots; 0 = one launch per tick) and the workgroup
+        slots per compute unit those launches leave to the other robots."""
+        check(lib.tsl_server_set_slice(self.handle, int(pairs), int(reserved_per_cu)))
+
+    def set_tile(self, bulk, k, points_per_lane):
+        """Evaluation tile (particles per wavefront, points per lane) of the
+        bulk class (robots of 65,536+ particle slots) or of the others."""
+        check(lib.tsl_server_set_tile(self.handle, int(bool(bulk)), int(k), int(points_per_lane)))
+
     def stats(self):
         used, ticks, reqs = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         check(lib.tsl_server_stats(self.handle, ctypes.byref(used), ctypes.byref(ticks), ctypes.byref(reqs)))
