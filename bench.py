@@ -568,18 +568,130 @@ def bench_mixed(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_campus(args):
+    """The global-localisation request of the mixed workload on its own: one
+    campus map (12 x 11 building tiles, 7.3e9 int16 voxels, 14.6 GB, 64-bit
+    block offsets: the map streams from HBM), 1,000,000 particles over its
+    free space, a 4,000-point scan.  One step = restore + pose + evaluation
+    + normalisation + resampling of the plain filter (k_eval); then the same
+    robot served (k_eval_batch in slices), one request after the other."""
+    import tsdf_localization_amd as tsl
+    from tsdf_localization_amd import synth
+    from tsdf_localization_amd.server import Server
+
+    t0 = time.time()
+    cp = synth.campus(0x5EED0005, tiles=(12, 11))
+    rng = np.random.default_rng(0x5EED0005)
+    tf = synth.tf_from_xyzrpy(**synth.SCANNER_TF)
+    _, pts = synth.campus_scan(cp, rng, tf, 4000)
+    parts = synth.campus_particles(cp, 0x5EED0005 + 5, args.particles or 1_000_000, np.radians(3.0))
+    P, N = parts.shape[0], pts.shape[0]
+    c = cp.map.coef()
+    log(f"workload campus: {c.data_size} voxels ({2 * c.data_size / 1e9:.2f} GB), P={P} N={N} "
+        f"(generated in {time.time() - t0:.1f}s)")
+    ev = tsl.TSDFEvaluator(cp.map)
+    f = tsl.DeviceFilter(ev, P)
+    f.set_particles(parts)
+    f.set_scan(pts, tf)
+    f.snapshot()
+    f.set_tile(args.tile_k, args.chunk, args.tile_p)
+    f.set_order(args.order)
+    stream = tsl.Stream()
+    u0s = [np.float32(rng.uniform(0.0, 1.0 / P)) for _ in range(args.warmup + args.steps)]
+
+    def step(i):
+        f.restore_async(stream)
+        f.update_async(stream)
+        f.resample_async(u0s[i], stream)
+
+    for i in range(args.warmup):
+        step(i)
+    f.sync(stream)
+    f.set_timing(True)
+    e0, e1 = tsl.Event(), tsl.Event()
+    e0.record(stream)
+    for i in range(args.steps):
+        step(args.warmup + i)
+    e1.record(stream)
+    stream.synchronize()
+    total_ms = e0.elapsed_ms(e1)
+    kern_ms, launches, pairs = f.eval_timing()
+    f.sync(stream)
+    f.set_timing(False)
+    ms_per_step = total_ms / args.steps
+    kern_avg = kern_ms / max(launches, 1)
+    evals = P * N
+    achieved = BYTES_PER_EVAL * (pairs / max(launches, 1)) / (kern_avg / 1000.0) / 1e9
+    log(f"plain: {ms_per_step:.2f} ms/step, {evals / ms_per_step * 1e3:.3e} evals/s; k_eval {kern_avg:.2f} ms "
+        f"({achieved:.0f} GB/s algorithmic)")
+    # served: the same request, one after the other; the robot is added
+    # afresh before each one so that every request sees the uniform particle
+    # set the plain steps restore (a resampled set collapses onto the best
+    # poses, whose lookups share cache lines)
+    srv = Server(ev, max_robots=2, max_particles=P, max_batch=4, chunk_len=2048)
+    srv.set_slice(args.mixed_slice, args.mixed_reserve)
+    if args.served_tile:
+        srv.set_tile(1, args.served_tile[0], args.served_tile[1])
+    motion = tsl.evaluator.motion_from_delta(0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, seed=1)
+    lat = []
+    for i in range(args.warmup + args.steps):
+        rid = srv.add_robot(parts, N, tf)
+        srv.submit(rid, motion, pts, u0s[i], request_id=i)
+        r = srv.wait(1)[0]
+        if r["status"] != 0:
+            raise RuntimeError(f"served campus request {i} failed: status {r['status']}")
+        srv.remove_robot(rid)
+        if i >= args.warmup:
+            lat.append(r["done_ms"] - r["submit_ms"])
+    srv.close()
+    served_ms = float(np.median(lat))
+    log(f"served: {served_ms:.2f} ms per request (median of {len(lat)}), {evals / served_ms * 1e3:.3e} evals/s")
+    traffic = _profile_value(CAMPUS_TRAFFIC_FILE, "traffic_bytes_per_launch",
+                             {"order": args.order, "k": args.tile_k, "p": args.tile_p, "chunk": args.chunk,
+                              "lookup": -1})
+    out = {
+        "metric": "particle*point evals/s", "value": evals / ms_per_step * 1e3, "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (campus of seeded building tiles, ray-cast scan, particles over the free space)",
+        "config": {"workload": "global localisation on a 14.6 GB campus map (the mixed workload's global class)",
+                   "particles": P, "points": N, "map_voxels": int(c.data_size),
+                   "map": "campus of 12x11 building tiles, 624x572x6.4 m @0.05 m int16 (64-bit block offsets)",
+                   "tile": {"particle_order": {0: "caller", 1: "spatial", -1: "auto"}[args.order],
+                            "particles_per_wavefront": args.tile_k, "points_per_lane": args.tile_p,
+                            "chunk_points": args.chunk or "auto"}},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_eval (64-bit map form)",
+                     "kernel_avg_ms": kern_avg, "bytes_per_eval": BYTES_PER_EVAL,
+                     "evals_per_launch": pairs // max(launches, 1)},
+        "served": {"ms_per_request": served_ms, "evals_per_s": evals / served_ms * 1e3,
+                   "slice_pairs": args.mixed_slice, "reserved_slots_per_cu": args.mixed_reserve,
+                   "tile": (None if not args.served_tile else
+                            {"particles_per_wavefront": args.served_tile[0],
+                             "points_per_lane": args.served_tile[1]})},
+    }
+    print(json.dumps(out), flush=True)
+
+
+CAMPUS_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_eval_campus_traffic.md")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="building", choices=["building", "room", "batched", "serving", "mixed"])
+    ap.add_argument("--workload", default="building",
+                    choices=["building", "room", "batched", "serving", "mixed", "campus"])
     ap.add_argument("--mixed-requests", type=int, default=400, help="mixed: number of requests")
     ap.add_argument("--mixed-rate", type=float, default=50.0, help="mixed: Poisson arrival rate (requests/s)")
     ap.add_argument("--mixed-slice", type=int, default=1 << 27,
                     help="mixed: particle-point pairs per launch of a global robot's evaluation (0 = one launch)")
     ap.add_argument("--mixed-reserve", type=int, default=2,
                     help="mixed: workgroup slots per CU a sliced global evaluation leaves to tracking requests")
+    ap.add_argument("--served-tile", type=int, nargs=2, default=None, metavar=("K", "P"),
+                    help="campus: tile of the served (bulk-class) evaluation, particles per wavefront and points "
+                         "per lane (default: the server's)")
     ap.add_argument("--mixed-small-maps", action="store_true",
                     help="mixed: one 51.2 m building per map instead of a 12 x 11-tile campus")
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
@@ -624,6 +736,9 @@ def main():
         return
     if args.workload == "mixed":
         bench_mixed(args)
+        return
+    if args.workload == "campus":
+        bench_campus(args)
         return
 
     import tsdf_localization_amd as tsl

@@ -105,7 +105,8 @@ int tsl_map_coef_init(float min_x, float min_y, float min_z, float max_x, float 
  * CudaSubVoxelMap (cuda_sub_voxel_map.h:141,166).  Replaces the map upload of
  * CudaEvaluator::CudaEvaluator (src/cuda/cuda_evaluator.cu:21-59).  Voxel
  * values are stored on the device as 16-bit codes into a value palette
- * (at most 4096 distinct values; TSL_ERR_UNSUPPORTED beyond). */
+ * when the map holds at most 4096 distinct values, else as the float values
+ * themselves (4 bytes per voxel); results are the same either way. */
 int tsl_map_create(const tsl_map_coef* coef, const int32_t* grid_occ, const float* data,
                    tsl_map** out);
 

@@ -207,3 +207,65 @@ def test_mixed_classes(tsl):
                     assert np.max(rel_err(weighted[:, 6], w64 / w64.sum())) <= WEIGHT_RTOL, (k, j)
         finally:
             srv.close()
+
+
+def test_batched_filters_graph_autotuned(tsl):
+    """The batched workload exactly as run: 64 filters, the tile the batch
+    autotuner picks, one step (restore + update + resample) captured in a
+    HIP graph and replayed three times with eager work between replays.
+    Every replay gives the same bits; four members equal the same filter
+    alone with that tile, update and resampling bit for bit; their weights
+    are within 1e-5 of the oracle."""
+    from tsdf_localization_amd import synth
+    w = synth.batched()
+    ev = tsl.TSDFEvaluator(w.map)
+    om = ob.OracleMap.from_host_map(w.map)
+    s = tsl.Stream()
+    filters = []
+    for r in w.robots:
+        f = tsl.DeviceFilter(ev, r.particles.shape[0])
+        f.set_particles(r.particles)
+        f.set_scan(r.points, w.tf)
+        f.snapshot()
+        filters.append(f)
+    b = tsl.Batch(ev, filters)
+    k, p, chunk, _ = b.autotune(s)
+    print(f"batched autotune: K={k} points/lane={p} chunk={chunk or 'auto'}", flush=True)
+    for f in filters:  # the autotuner left the members updated: back to the snapshots
+        f.restore_async(s)
+    s.synchronize()
+    b.prepare()
+    rng = np.random.default_rng(9)
+    u0 = np.array([np.float32(rng.uniform(0, 0.9 / r.particles.shape[0])) for r in w.robots], np.float32)
+    b.capture(s, restore=True)
+    replays = []
+    for rep in range(3):
+        b.launch(u0, s)
+        _, counts, status = b.sync(s)
+        assert set(status.tolist()) == {0}, rep
+        replays.append([f.get_particles() for f in filters])
+        f0 = tsl.DeviceFilter(ev, 64)  # eager work between replays
+        f0.set_particles(w.robots[0].particles[:64])
+    for rep in (1, 2):
+        for a, c in zip(replays[0], replays[rep]):
+            assert np.array_equal(a.view(np.uint32), c.view(np.uint32)), rep
+    sizes = [r.particles.shape[0] * r.points.shape[0] for r in w.robots]
+    picks = [int(np.argmin(sizes)), int(np.argmax(sizes)), 5, 40]
+    for i in picks:
+        r = w.robots[i]
+        n = r.points.shape[0]
+        eff_chunk = chunk or min(2048, (n + 63) // 64 * 64)
+        alone = tsl.DeviceFilter(ev, r.particles.shape[0])
+        alone.set_particles(r.particles)
+        alone.set_scan(r.points, w.tf)
+        alone.set_tile(k, eff_chunk, p)
+        alone.set_order(0)
+        alone.update_async(s)
+        alone.sync(s)
+        weighted = alone.get_particles()
+        _, w64 = om.evaluate(r.particles, r.points, w.tf)
+        print(f"batched member {i}: {r.particles.shape[0]} x {n}", flush=True)
+        assert np.max(rel_err(weighted[:, 6], w64 / w64.sum())) <= WEIGHT_RTOL, i
+        alone.resample_async(u0[i], s)
+        alone.sync(s)
+        assert np.array_equal(alone.get_particles().view(np.uint32), replays[0][i].view(np.uint32)), i

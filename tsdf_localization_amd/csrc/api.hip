@@ -223,17 +223,41 @@ static int validate_coef(const tsl_map_coef* c)
 }
 
 static int map_finish(tsl_map* m, const int32_t* grid_occ, const std::vector<float>& palette, uint16_t init_code,
-                      bool brick, uint64_t phys_size);
+                      bool brick, uint64_t phys_size, bool f32 = false);
+
+// Brick layout when every block offset is an aligned multiple of sd^3.
+static bool brick_layout(const tsl_map_coef& c, const int32_t* grid_occ)
+{
+    const uint64_t sd = c.sub_dim, sd3 = sd * sd * sd;
+    bool brick = sd >= 4 && sd % 4 == 0 && sd3 < (1ull << 30);
+    for (uint64_t b = 0; brick && b < c.grid_occ_size; ++b)
+        if (grid_occ[b] >= 0 && (uint64_t)grid_occ[b] % sd3 != 0) brick = false;
+    return brick;
+}
+
+// The voxels of the reference's linear order (slot*sd^3 + lx + ly*sd +
+// lz*sd^2) in 4x4x4 bricks: element slot*sd^3 + brick*64 + (lx&3) +
+// (ly&3)*4 + (lz&3)*16, the last block padded to a whole sd^3.
+extern "C++" template <typename T>
+static std::vector<T> to_bricks(const T* in, const tsl_map_coef& c)
+{
+    const uint64_t sd = c.sub_dim, sd3 = sd * sd * sd, B = sd / 4;
+    std::vector<T> phys((c.data_size + sd3 - 1) / sd3 * sd3, T{});
+    for (uint64_t idx = 0; idx < c.data_size; ++idx) {
+        const uint64_t slot = idx / sd3, rem = idx % sd3;
+        const uint64_t lz = rem / (sd * sd), ly = (rem / sd) % sd, lx = rem % sd;
+        const uint64_t off = ((lx >> 2) + (ly >> 2) * B + (lz >> 2) * B * B) * 64 + (lx & 3) + (ly & 3) * 4 +
+                             (lz & 3) * 16;
+        phys[slot * sd3 + off] = in[idx];
+    }
+    return phys;
+}
 
 static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uint16_t>& codes,
                       const std::vector<float>& palette, uint16_t init_code)
 {
     const tsl_map_coef& c = m->coef;
-    // brick layout when every block offset is an aligned multiple of sd^3
-    const uint64_t sd = c.sub_dim, sd3 = sd * sd * sd;
-    bool brick = sd >= 4 && sd % 4 == 0 && sd3 < (1ull << 30);
-    for (uint64_t b = 0; brick && b < c.grid_occ_size; ++b)
-        if (grid_occ[b] >= 0 && (uint64_t)grid_occ[b] % sd3 != 0) brick = false;
+    const bool brick = brick_layout(c, grid_occ);
     // device codes are byte offsets into the palette (code * 4 < 2^14), so a
     // lookup addresses the LDS palette without scaling
     std::vector<uint16_t> scaled(codes.size());
@@ -242,16 +266,8 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     std::vector<uint16_t> phys;
     uint64_t phys_size = c.data_size;
     if (brick && c.data_size) {
-        const uint64_t B = sd / 4;
-        phys_size = (c.data_size + sd3 - 1) / sd3 * sd3;
-        phys.assign(phys_size, 0);
-        for (uint64_t idx = 0; idx < c.data_size; ++idx) {
-            const uint64_t slot = idx / sd3, rem = idx % sd3;
-            const uint64_t lz = rem / (sd * sd), ly = (rem / sd) % sd, lx = rem % sd;
-            const uint64_t off = ((lx >> 2) + (ly >> 2) * B + (lz >> 2) * B * B) * 64 + (lx & 3) + (ly & 3) * 4 +
-                                 (lz & 3) * 16;
-            phys[slot * sd3 + off] = scaled[idx];
-        }
+        phys = to_bricks(scaled.data(), c);
+        phys_size = phys.size();
         upload = &phys;
     }
     // one sentinel element after the voxels holds the code of init_value:
@@ -273,11 +289,39 @@ static int map_upload(tsl_map* m, const int32_t* grid_occ, const std::vector<uin
     return map_finish(m, grid_occ, palette, init_code, brick, phys_size);
 }
 
+// Float form of a map with more distinct values than the palette holds:
+// the values themselves, 4 bytes per voxel, in the element order the codes
+// would have (bricked when the block offsets allow), init_value after them.
+static int map_upload_f32(tsl_map* m, const int32_t* grid_occ, const float* data)
+{
+    const tsl_map_coef& c = m->coef;
+    const bool brick = brick_layout(c, grid_occ);
+    std::vector<float> phys;
+    const float* upload = data;
+    uint64_t phys_size = c.data_size;
+    if (brick && c.data_size) {
+        phys = to_bricks(data, c);
+        phys_size = phys.size();
+        upload = phys.data();
+    }
+    TSL_TRY(m->grid_occ.reserve(sizeof(int32_t) * c.grid_occ_size));
+    TSL_TRY(m->codes.reserve(sizeof(float) * (phys_size + 1)));
+    TSL_TRY(m->palette.reserve(sizeof(float)));
+    TSL_HIP_CHECK(hipMemcpy(m->grid_occ.ptr, grid_occ, sizeof(int32_t) * c.grid_occ_size,
+                            hipMemcpyHostToDevice));
+    if (phys_size)
+        TSL_HIP_CHECK(hipMemcpy(m->codes.ptr, upload, sizeof(float) * phys_size, hipMemcpyHostToDevice));
+    TSL_HIP_CHECK(hipMemcpy(m->codes.as<float>() + phys_size, &c.init_value, sizeof(float),
+                            hipMemcpyHostToDevice));
+    return map_finish(m, grid_occ, std::vector<float>(), 0, brick, phys_size, true);
+}
+
 // The tables and the device views of an uploaded map (codes, palette and
 // block table already on the device).  grid_occ: the int32 block table on
-// the host, or NULL for a wide map (slot table on the device).
+// the host, or NULL for a wide map (slot table on the device).  f32: the
+// codes array holds the float values themselves.
 static int map_finish(tsl_map* m, const int32_t* grid_occ, const std::vector<float>& palette, uint16_t init_code,
-                      bool brick, uint64_t phys_size)
+                      bool brick, uint64_t phys_size, bool f32)
 {
     const tsl_map_coef& c = m->coef;
     const bool wide = grid_occ == nullptr;
@@ -347,6 +391,7 @@ static int map_finish(tsl_map* m, const int32_t* grid_occ, const std::vector<flo
     d.sd3b = (uint32_t)(2 * sd3);
     d.sentinel = phys_size;
     d.sentinel_code = init_code * 4u;
+    d.f32 = f32 ? 1u : 0u;
     {
         // occupancy-rank form of the block table
         // (rank < 2^17 and 2 sub_dim^3 < 2^24 keep the device product, a
@@ -368,7 +413,7 @@ static int map_finish(tsl_map* m, const int32_t* grid_occ, const std::vector<flo
             nwords *= 2;
             ++log2w;
         }
-        bool rank = !wide && nwords <= kMaxOccWords && 2 * sd3 < (1ull << 24);
+        bool rank = !wide && !f32 && nwords <= kMaxOccWords && 2 * sd3 < (1ull << 24);
         std::vector<OccWord> words;
         if (rank) {
             words.assign(nwords, OccWord{0u, 0u});
@@ -442,21 +487,39 @@ int tsl_map_create(const tsl_map_coef* coef, const int32_t* grid_occ, const floa
         return set_error(TSL_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     TSL_TRY(validate_coef(coef));
-    // value palette: distinct bit patterns in first-seen order
+    // value palette: distinct bit patterns in first-seen order; a map with
+    // more than kMaxPalette of them is kept in the float form
     std::vector<float> palette;
     std::vector<uint16_t> codes(coef->data_size);
     std::unordered_map<uint32_t, uint16_t> seen;
-    for (uint64_t i = 0; i < coef->data_size; ++i) {
+    bool f32 = false;
+    for (uint64_t i = 0; i < coef->data_size && !f32; ++i) {
         uint32_t bits;
         std::memcpy(&bits, &data[i], 4);
         auto it = seen.find(bits);
         if (it == seen.end()) {
-            if (palette.size() >= kMaxPalette)
-                return set_error(TSL_ERR_UNSUPPORTED, "map holds more than %u distinct voxel values", kMaxPalette);
+            if (palette.size() >= kMaxPalette) {
+                f32 = true;
+                break;
+            }
             it = seen.emplace(bits, (uint16_t)palette.size()).first;
             palette.push_back(data[i]);
         }
         codes[i] = it->second;
+    }
+    if (f32) {
+        std::vector<uint16_t>().swap(codes);
+        tsl_map* m = new (std::nothrow) tsl_map();
+        if (!m) return set_error(TSL_ERR_OUT_OF_MEMORY, "out of host memory");
+        (void)hipGetDevice(&m->device);
+        m->coef = *coef;
+        const int rc = map_upload_f32(m, grid_occ, data);
+        if (rc != TSL_OK) {
+            delete m;
+            return rc;
+        }
+        *out = m;
+        return TSL_OK;
     }
     uint16_t init_code;
     {

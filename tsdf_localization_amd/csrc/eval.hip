@@ -173,7 +173,9 @@ inline size_t lds_bytes(const DevMap& d, bool rank)
 // kMode 0 writes the chunk's partial weight of particle p0 + k to
 // partial_row[p0 + k].  kWide: the map's block table holds slots and its
 // voxel and byte offsets need 64 bits (bricked layout, no rank table).
-template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false>
+// kF32: the float form (the voxel array holds the values themselves, read at
+// twice the code byte offsets; no palette, no rank table).
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
 __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t, const float* __restrict__ L,
                                           uint32_t P, uint32_t p0, bool active, const float4* __restrict__ pts,
                                           uint32_t N, uint32_t j0, uint32_t j1, float a_hit,
@@ -306,6 +308,21 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
                 }
             }
             }
+            if constexpr (kF32) {
+                // float form: the value itself at twice the code's byte
+                // offset; rejected pairs take init_value without a load
+#pragma unroll
+                for (int u = 0; u < kP; ++u) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        float val = map.init;
+                        if (ok[u][k])
+                            val = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(map.codes) +
+                                                                  2 * (uint64_t)addr[u][k]);
+                        v[u][k] = kTrace ? val : a_hit * val;
+                    }
+                }
+            } else {
             // rejected pairs take the code of init_value without a load
             uint32_t code[kP][K];
 #pragma unroll
@@ -323,6 +340,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
             for (int u = 0; u < kP; ++u) {
 #pragma unroll
                 for (int k = 0; k < K; ++k) v[u][k] = palette_at(lut, code[u][k]);
+            }
             }
 #pragma unroll
             for (int u = 0; u < kP; ++u) {
@@ -357,7 +375,13 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
                             v = palette_at(lut, map.codes[phys_from_linear64(map, full)]);
                         } else {
                             const uint32_t x = (uint32_t)full;
-                            v = palette_at(lut, map.codes[kBrick ? phys_from_linear(map, x) : x]);
+                            const uint32_t e = kBrick ? phys_from_linear(map, x) : x;
+                            if constexpr (kF32) {
+                                const float val = reinterpret_cast<const float*>(map.codes)[e];
+                                v = kTrace ? val : a_hit * val;
+                            } else {
+                                v = palette_at(lut, map.codes[e]);
+                            }
                         }
                     }
                     if (kMode == 1) {
@@ -385,7 +409,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
     }
 }
 
-template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false>
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
 __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __restrict__ L,
                                                  const uint32_t* __restrict__ count,
                                                  const float4* __restrict__ pts, uint32_t N,
@@ -409,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
     if (g * kWaves * K >= P) return;  // the whole workgroup is idle (uniform across its waves)
     const uint32_t j0 = c * chunk_len;
     const uint32_t j1 = min(N, j0 + chunk_len);
-    eval_wave<K, kP, kMode, kU24, kBrick, kRank, kWide>(map, t, L, P, p0, p0 < P, pts, N, j0, j1, a_hit,
+    eval_wave<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>(map, t, L, P, p0, p0 < P, pts, N, j0, j1, a_hit,
                                                         kMode == 0 ? partial + (size_t)c * stride : nullptr,
                                                         trace_out, perm, t.tile);
 }
@@ -687,7 +711,7 @@ __global__ __launch_bounds__(1024) void k_batch_plan(const Member* __restrict__ 
 // start at zero), so a long evaluation can be cut into launches that other
 // streams' kernels interleave with; every item writes only its own partial
 // weights, so the slicing changes no bit.
-template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false>
+template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
 __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member* __restrict__ members,
                                                        BatchPlan* __restrict__ plan, float a_hit, uint32_t item0,
                                                        uint32_t count)
@@ -725,7 +749,8 @@ __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member*
         const uint32_t p0 = (g * kWaves + wave) * K;
         const uint32_t j0 = c * m.chunk_len;
         const uint32_t j1 = min(m.N, j0 + m.chunk_len);
-        eval_wave<K, kP, 0, kU24, kBrick, kRank, kWide>(map, t, m.L, P, p0, p0 < P, m.points, m.N, j0, j1, a_hit,
+        eval_wave<K, kP, 0, kU24, kBrick, kRank, kWide, kF32>(map, t, m.L, P, p0, p0 < P, m.points, m.N, j0, j1,
+                                                              a_hit,
                                                         m.partial + (size_t)c * m.stride, nullptr, nullptr, tile);
     }
 }
@@ -814,14 +839,15 @@ bool palette_at_zero(const void* kernel)
         if (!tsl_lds0_) return set_error(TSL_ERR_UNSUPPORTED, "evaluation kernel declares static LDS"); \
     } while (0)
 
-template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false>
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
 int launch_eval_t(dim3 grid, size_t lds, hipStream_t stream, const tsl_filter* f, const Tile& tile, size_t N,
                   float* partial, void* trace_out)
 {
-    TSL_CHECK_PALETTE_AT_ZERO((&k_eval<K, kP, kMode, kU24, kBrick, kRank, kWide>));
+    TSL_CHECK_PALETTE_AT_ZERO((&k_eval<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>));
     const tsl_map* map = f->ev->map;
     FilterState* st = f->state.as<FilterState>();
-    hipLaunchKernelGGL((k_eval<K, kP, kMode, kU24, kBrick, kRank, kWide>), grid, dim3(kBlock), lds, stream, map->dev,
+    hipLaunchKernelGGL((k_eval<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>), grid, dim3(kBlock), lds, stream,
+                       map->dev,
                        f->L.as<float>(), &st->count, f->points.as<float4>(), (uint32_t)N, tile.chunk_len,
                        tile.chunks, f->ev->a_hit, partial, (uint32_t)f->capacity, trace_out,
                        f->perm.as<uint32_t>());
@@ -837,6 +863,12 @@ int launch_eval_flags(bool u24, bool brick, bool rank, dim3 grid, size_t lds, hi
     if (f->ev->map->dev.wide) {  // bricked, slot table, no rank table
         if (u24) return launch_eval_t<K, kP, kMode, true, true, false, true>(TSL_EVAL_ARGS);
         return launch_eval_t<K, kP, kMode, false, true, false, true>(TSL_EVAL_ARGS);
+    }
+    if (f->ev->map->dev.f32) {  // float form: int32 block table, no rank table
+        if (u24) return brick ? launch_eval_t<K, kP, kMode, true, true, false, false, true>(TSL_EVAL_ARGS)
+                              : launch_eval_t<K, kP, kMode, true, false, false, false, true>(TSL_EVAL_ARGS);
+        return brick ? launch_eval_t<K, kP, kMode, false, true, false, false, true>(TSL_EVAL_ARGS)
+                     : launch_eval_t<K, kP, kMode, false, false, false, false, true>(TSL_EVAL_ARGS);
     }
     if (u24) {
         if (brick) {
@@ -1174,15 +1206,16 @@ __global__ void k_batch_claims_reset(BatchPlan* __restrict__ plan)
     if (threadIdx.x < kPlanXcds) plan->next[threadIdx.x * 32] = 0;
 }
 
-template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false>
+template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
 int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid_out)
 {
-    TSL_CHECK_PALETTE_AT_ZERO((&k_eval_batch<K, kP, kU24, kBrick, kRank, kWide>));
+    TSL_CHECK_PALETTE_AT_ZERO((&k_eval_batch<K, kP, kU24, kBrick, kRank, kWide, kF32>));
     const size_t lds = lds_bytes(b.map->dev, kRank);
     if (grid_out) {
         int per_cu = 0;
         TSL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&k_eval_batch<K, kP, kU24, kBrick, kRank, kWide>), kBlock, lds));
+            &per_cu, reinterpret_cast<const void*>(&k_eval_batch<K, kP, kU24, kBrick, kRank, kWide, kF32>), kBlock,
+            lds));
         if (per_cu < 1) per_cu = 1;
         int cus = 0;
         TSL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b.map->device));
@@ -1201,7 +1234,7 @@ int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid
             hipLaunchKernelGGL(k_batch_claims_reset, dim3(1), dim3(64), 0, stream, b.plan);
             TSL_HIP_CHECK(hipGetLastError());
         }
-        hipLaunchKernelGGL((k_eval_batch<K, kP, kU24, kBrick, kRank, kWide>), dim3(b.grid), dim3(kBlock), lds,
+        hipLaunchKernelGGL((k_eval_batch<K, kP, kU24, kBrick, kRank, kWide, kF32>), dim3(b.grid), dim3(kBlock), lds,
                            stream, b.map->dev, b.members, b.plan, b.a_hit, (uint32_t)i0,
                            (uint32_t)std::min<uint64_t>(slice, 0xffffffffu));
         TSL_HIP_CHECK(hipGetLastError());
@@ -1218,6 +1251,12 @@ int launch_eval_batch_flags(const BatchLaunch& b, hipStream_t stream, uint32_t* 
     if (d.wide)
         return u24 ? launch_eval_batch_t<K, kP, true, true, false, true>(b, stream, grid_out)
                    : launch_eval_batch_t<K, kP, false, true, false, true>(b, stream, grid_out);
+    if (d.f32) {  // float form: int32 block table, no rank table
+        if (u24) return brick ? launch_eval_batch_t<K, kP, true, true, false, false, true>(b, stream, grid_out)
+                              : launch_eval_batch_t<K, kP, true, false, false, false, true>(b, stream, grid_out);
+        return brick ? launch_eval_batch_t<K, kP, false, true, false, false, true>(b, stream, grid_out)
+                     : launch_eval_batch_t<K, kP, false, false, false, false, true>(b, stream, grid_out);
+    }
     if (u24) {
         if (brick) return rank ? launch_eval_batch_t<K, kP, true, true, true>(b, stream, grid_out)
                                : launch_eval_batch_t<K, kP, true, true, false>(b, stream, grid_out);

@@ -145,7 +145,8 @@ struct DevMapFull {
 struct DevMap {
     const int32_t* grid_occ;   // block table: voxel offset of the block or -1
     const uint32_t* slots;     // wide maps: block slot (offset / sub_dim^3) or kNoSlot
-    const uint16_t* codes;     // per voxel: byte offset (4 * palette code) of its value in the palette
+    const uint16_t* codes;     // per voxel: byte offset (4 * palette code) of its value in the palette;
+                               // float form (f32): the float values themselves, at twice the byte offsets
     const float* palette;      // palette_size values
     const SubEntry* sub_tab;   // sub-voxel threshold buckets
     const uint64_t* neg_tab;   // index contribution of the wrap-band axes, by axis bit mask (8)
@@ -182,6 +183,7 @@ struct DevMap {
     uint64_t sentinel;                // element of codes holding the code of init_value
     uint32_t sentinel_code;           // that byte offset
     uint32_t wide;                    // 1: slots instead of grid_occ, 64-bit voxel and byte offsets
+    uint32_t f32;                     // 1: float form (more distinct values than the palette holds)
 };
 
 // Smallest u in [0, 2^23] with trunc(fl(fl(u)/res)) >= dim; host IEEE division.
