@@ -211,6 +211,14 @@ int tsl_trig(int mode, const float* x, size_t n, float* s, float* c, int on_devi
  * multiple of stride.  No counterpart in the reference. */
 int tsl_check_trig_libm(int mode, uint32_t stride, uint64_t* mismatches, uint64_t* checked);
 
+/* Host self-check of the inverse trigonometry of the motion model's
+ * getAngleFromMat (src/util/util.cpp:86-111): the host build of its asinf
+ * and atanf against this process's C library over every non-NaN float
+ * whose bit pattern is a multiple of stride, and of its atan2f on
+ * atan2_samples seeded pseudo-random argument pairs.  No counterpart in
+ * the reference. */
+int tsl_check_invtrig_libm(uint32_t stride, uint64_t atan2_samples, uint64_t* mismatches, uint64_t* checked);
+
 /* Evaluator::evaluate(particles, points, tf_matrix)
  * (cuda_evaluator.h:102; src/cuda/cuda_evaluator.cu:118-428;
  * CPU semantics src/evaluation/tsdf_evaluator.cpp:78-245).

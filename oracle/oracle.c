@@ -768,10 +768,13 @@ ORC_EXPORT void orc_cumsum(const float* weights, size_t M, double* out)
  * (Steele, Lea, Flood, OOPSLA 2014) mixes (seed, particle, draw, half), 53 bits
  * give a uniform in (0, 1), Box-Muller's cosine branch a standard normal, and
  * draw = z * stddev + mean as libstdc++'s normal_distribution forms it.
- * Parity: the transcendental functions are glibc's here and the device
- * math library's there, so results agree within a few float ulps; pinned by
- * closed forms (pure translation, pure yaw, rotation composition) in the
- * tests. */
+ * Parity: every float function (sinf/cosf of the particle angles, and
+ * asinf/atan2f/cosf in getAngleFromMat) is the C library's here and its
+ * restatement on the device, bit for bit; the double sin/cos of the draws and
+ * the log/sqrt/cos of Box-Muller are each side's math library (rounded to
+ * float before they meet the pose, which hides almost every last-bit
+ * difference).  Pinned by closed forms (pure translation, pure yaw, rotation
+ * composition) in the tests. */
 /* ------------------------------------------------------------------------ */
 static uint64_t orc_splitmix64(uint64_t x)
 {
@@ -843,22 +846,46 @@ ORC_EXPORT void orc_motion(const orc_particle* in, size_t n, const double mean[6
             tf[4 * r + 3] = tp[4 * r] * tf_odom[3] + tp[4 * r + 1] * tf_odom[7] + tp[4 * r + 2] * tf_odom[11] +
                             tp[4 * r + 3];
         }
-        /* getAngleFromMat (util.cpp:86-111) */
+        /* getAngleFromMat (util.cpp:86-111).  Its arguments are floats: in
+         * the C++ reference with libstdc++, fabs, atan2, asin and cos bind to
+         * the float functions (fabsf, atan2f, asinf, cosf) and the divisions
+         * are float; mode 2 keeps the C binding (double functions). */
         float roll, pitch, yaw;
-        if (fabs(tf[8]) >= 1) {
-            yaw = 0;
-            double delta = atan2(tf[9], tf[10]);
-            if (tf[8] < 0) {
-                pitch = M_PI / 2.0;
-                roll = delta;
+        if (trig_mode == 2) {
+            if (fabs(tf[8]) >= 1) {
+                yaw = 0;
+                double delta = atan2(tf[9], tf[10]);
+                if (tf[8] < 0) {
+                    pitch = M_PI / 2.0;
+                    roll = delta;
+                } else {
+                    pitch = -M_PI / 2.0;
+                    roll = delta;
+                }
             } else {
-                pitch = -M_PI / 2.0;
-                roll = delta;
+                pitch = -asin(tf[8]);
+                roll = atan2(tf[9] / cos(pitch), tf[10] / cos(pitch));
+                yaw = atan2(tf[4] / cos(pitch), tf[0] / cos(pitch));
             }
         } else {
-            pitch = -asin(tf[8]);
-            roll = atan2(tf[9] / cos(pitch), tf[10] / cos(pitch));
-            yaw = atan2(tf[4] / cos(pitch), tf[0] / cos(pitch));
+            volatile float m8 = tf[8], m9 = tf[9], m10 = tf[10];
+            if (fabsf(m8) >= 1) {
+                yaw = 0;
+                float delta = atan2f(m9, m10);
+                if (tf[8] < 0) {
+                    pitch = M_PI / 2.0;
+                    roll = delta;
+                } else {
+                    pitch = -M_PI / 2.0;
+                    roll = delta;
+                }
+            } else {
+                pitch = -asinf(m8);
+                volatile float pv = pitch;
+                float cp = cosf(pv);
+                roll = atan2f(tf[9] / cp, tf[10] / cp);
+                yaw = atan2f(tf[4] / cp, tf[0] / cp);
+            }
         }
         out[i].first[0] = last[0] + tf[3];
         out[i].first[1] = last[1] + tf[7];

@@ -128,3 +128,30 @@ def test_binding_mismatch_global_localisation():
           f"({bad / pairs:.2e}), max weight rel diff {worst:.2e}")
     assert bad / pairs <= 1e-6
     assert worst > 1e-5
+
+
+def test_motion_inverse_trig_every_float():
+    """The motion model's getAngleFromMat (src/util/util.cpp:86-111) takes
+    floats, so with libstdc++ its asin/atan2/cos are asinf/atan2f/cosf.  The
+    library's restatements of glibc's asinf and atanf equal this process's on
+    every non-NaN float, its atan2f on 2e7 seeded random pairs (half of them
+    with exponents in [2^-27, 2^28])."""
+    bad, n = tsl.evaluator.check_invtrig_libm(1, 20_000_000)
+    assert n == 2 * 4278190082 + 20_000_000
+    assert bad == 0
+
+
+def test_motion_angle_binding_matters():
+    """The float binding (the reference's) and the C binding of
+    getAngleFromMat give different angles for a large share of particles
+    after one motion step of the room's particle set, so the motion model
+    has to follow the float binding too."""
+    w = synth.room()
+    mean = [0.4, 0.05, 0.01, 0.01, -0.02, 0.15]
+    sd = [0.02, 0.02, 0.01, 0.01, 0.01, 0.01]
+    a = ob.motion(w.particles, mean, sd, 1234, trig_mode=ob.TRIG_LIBM)
+    b = ob.motion(w.particles, mean, sd, 1234, trig_mode=ob.TRIG_DOUBLE)
+    differ = np.mean(np.any(a[:, 3:6] != b[:, 3:6], axis=1))
+    print(f"particles whose angles differ between the bindings: {differ:.3f}")
+    assert differ > 0.05
+    assert np.max(np.abs(a[:, 3:6] - b[:, 3:6])) < 1e-5

@@ -110,6 +110,8 @@ _SIGS = {
     "tsl_trig": (_I, [_I, _P, _S, _P, _P, _I]),
     "tsl_check_trig_libm": (_I, [_I, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
                                  ctypes.POINTER(ctypes.c_uint64)]),
+    "tsl_check_invtrig_libm": (_I, [ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                    ctypes.POINTER(ctypes.c_uint64)]),
     "tsl_evaluate_cloud": (_I, [_P, _P, _S, _P, _P, _S, _P, ctypes.POINTER(Pose)]),
     "tsl_filter_set_scan_cloud": (_I, [_P, _P, _P, _S, _P]),
     "tsl_resample_systematic": (_I, [_P, _S, _F, _P, _S, _P, ctypes.POINTER(_S)]),

@@ -1141,6 +1141,62 @@ int tsl_check_trig_libm(int mode, uint32_t stride, uint64_t* mismatches, uint64_
     return TSL_OK;
 }
 
+int tsl_check_invtrig_libm(uint32_t stride, uint64_t atan2_samples, uint64_t* mismatches, uint64_t* checked)
+{
+    if (!mismatches || !checked || stride == 0) return set_error(TSL_ERR_INVALID_ARGUMENT, "bad argument");
+    const unsigned nt = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    std::vector<uint64_t> bad(nt, 0), seen(nt, 0);
+    std::vector<std::thread> th;
+    const uint64_t span = 0x100000000ull, steps = (span + stride - 1) / stride;
+    auto same = [](float a, float b) { return std::memcmp(&a, &b, 4) == 0 || (a != a && b != b); };
+    for (unsigned t = 0; t < nt; ++t) {
+        th.emplace_back([&, t]() {
+            const uint64_t per = (steps + nt - 1) / nt;
+            const uint64_t k0 = per * t, k1 = std::min(per * (t + 1), steps);
+            uint64_t b = 0, n = 0;
+            for (uint64_t k = k0; k < k1; ++k) {
+                const uint32_t u = (uint32_t)(k * stride);
+                float x;
+                std::memcpy(&x, &u, 4);
+                if (x != x) continue;
+                volatile float xv = x;
+                if (!same(asinf_glibc(x), ::asinf(xv))) ++b;
+                if (!same(atanf_glibc(x), ::atanf(xv))) ++b;
+                n += 2;
+            }
+            // atan2f: random bit patterns, every other pair with exponents
+            // drawn from [2^-27, 2^28]
+            const uint64_t per2 = (atan2_samples + nt - 1) / nt;
+            const uint64_t j0 = per2 * t, j1 = std::min(per2 * (t + 1), atan2_samples);
+            for (uint64_t j = j0; j < j1; ++j) {
+                const uint64_t h = splitmix64(0x5EEDA7A2ull ^ splitmix64(j));
+                uint32_t a = (uint32_t)h, c = (uint32_t)(h >> 32);
+                if (j & 1) {
+                    a = (a & 0x807fffffu) | ((100u + (a >> 23) % 56u) << 23);
+                    c = (c & 0x807fffffu) | ((100u + (c >> 23) % 56u) << 23);
+                }
+                float y, x;
+                std::memcpy(&y, &a, 4);
+                std::memcpy(&x, &c, 4);
+                volatile float yv = y, xv = x;
+                if (!same(atan2f_glibc(y, x), ::atan2f(yv, xv))) ++b;
+                ++n;
+            }
+            bad[t] = b;
+            seen[t] = n;
+        });
+    }
+    for (auto& x : th) x.join();
+    uint64_t b = 0, n = 0;
+    for (unsigned t = 0; t < nt; ++t) {
+        b += bad[t];
+        n += seen[t];
+    }
+    *mismatches = b;
+    *checked = n;
+    return TSL_OK;
+}
+
 int tsl_evaluator_destroy(tsl_evaluator* ev)
 {
     if (!ev) return TSL_OK;

@@ -3,9 +3,11 @@
 // Everything on the sensor-update path here gives bit-identical results on
 // the host (x86-64, gcc or clang) and on gfx950: IEEE float/double +,-,*,/
 // only, no FMA contraction (all sources are compiled with -ffp-contract=off),
-// no library math calls.  The motion model at the end is the exception: its
-// double sin, cos, asin, atan2, log and sqrt come from each side's math
-// library, so host and device agree to within a few float ulps.
+// no library math calls.  The motion model at the end is the exception: the
+// double sin and cos of its noise draws and the log, sqrt and cos that make
+// the draws come from each side's math library (their results are rounded
+// to float before they meet the pose, which hides almost every last-bit
+// difference); its float functions are the restatements below.
 #pragma once
 
 #include <math.h>
@@ -256,6 +258,153 @@ TSL_HD void sincosf_mode(int mode, float x, float* s, float* c)
 }
 
 // ---------------------------------------------------------------------------
+// asinf, atanf and atan2f as glibc 2.35 computes them on x86-64
+// (sysdeps/ieee754/flt-32/e_asinf.c, s_atanf.c, e_atan2f.c): the float
+// versions of Sun's fdlibm algorithms (fdlibm 5.3, 1993), asinf with the
+// degree-4 polynomial of Moshier's Cephes library.  The reference's
+// getAngleFromMat (src/util/util.cpp:86-111) applies asin, atan2 and cos to
+// floats; with libstdc++ those are the float functions.  Float arithmetic,
+// no contraction; checked bit for bit against the host's glibc (asinf and
+// atanf on every float).
+// ---------------------------------------------------------------------------
+TSL_HD uint32_t f32_bits(float x)
+{
+    union { float f; uint32_t u; } v;
+    v.f = x;
+    return v.u;
+}
+
+TSL_HD float f32_from_bits(uint32_t u)
+{
+    union { uint32_t u; float f; } v;
+    v.u = u;
+    return v.f;
+}
+
+TSL_HD float asinf_glibc(float x)
+{
+    const float pio2_hi = 1.57079637050628662109375f, pio2_lo = -4.37113900018624283e-8f,
+                pio4_hi = 0.785398185253143310546875f;
+    const float p0 = 1.666675248e-1f, p1 = 7.495297643e-2f, p2 = 4.547037598e-2f, p3 = 2.417951451e-2f,
+                p4 = 4.216630880e-2f;
+    const uint32_t hx = f32_bits(x), ix = hx & 0x7fffffffu;
+    if (ix == 0x3f800000u) return x * pio2_hi + x * pio2_lo;  // +-pi/2
+    if (ix > 0x3f800000u) return (x - x) / (x - x);           // |x| > 1 or NaN: NaN
+    if (ix < 0x3f000000u) {                                   // |x| < 0.5
+        if (ix < 0x32000000u) return x;                       // |x| < 2^-27
+        const float t = x * x;
+        const float w = t * (p0 + t * (p1 + t * (p2 + t * (p3 + t * p4))));
+        return x + x * w;
+    }
+    // 0.5 <= |x| < 1
+    float w = 1.0f - fabsf(x);
+    float t = w * 0.5f;
+    float p = t * (p0 + t * (p1 + t * (p2 + t * (p3 + t * p4))));
+    const float s = sqrtf(t);
+    if (ix >= 0x3f79999au) {  // |x| > 0.975
+        t = pio2_hi - (2.0f * (s + s * p) - pio2_lo);
+    } else {
+        w = f32_from_bits(f32_bits(s) & 0xfffff000u);
+        const float c = (t - w * w) / (s + w);
+        const float r = p;
+        p = 2.0f * s * r - (pio2_lo - 2.0f * c);
+        const float q = pio4_hi - 2.0f * w;
+        t = pio4_hi - (p - q);
+    }
+    return (int32_t)hx > 0 ? t : -t;
+}
+
+TSL_HD float atanf_glibc(float x)
+{
+    const float atanhi[4] = {4.6364760399e-01f, 7.8539812565e-01f, 9.8279368877e-01f, 1.5707962513e+00f};
+    const float atanlo[4] = {5.0121582440e-09f, 3.7748947079e-08f, 3.4473217170e-08f, 7.5497894159e-08f};
+    const float aT[11] = {3.3333334327e-01f,  -2.0000000298e-01f, 1.4285714924e-01f, -1.1111110449e-01f,
+                          9.0908870101e-02f,  -7.6918758452e-02f, 6.6610731184e-02f, -5.8335702866e-02f,
+                          4.9768779427e-02f,  -3.6531571299e-02f, 1.6285819933e-02f};
+    const uint32_t hx = f32_bits(x), ix = hx & 0x7fffffffu;
+    int id;
+    if (ix >= 0x4c000000u) {  // |x| >= 2^25
+        if (ix > 0x7f800000u) return x + x;  // NaN
+        return (int32_t)hx > 0 ? atanhi[3] + atanlo[3] : -atanhi[3] - atanlo[3];
+    }
+    if (ix < 0x3ee00000u) {  // |x| < 0.4375
+        if (ix < 0x31000000u) return x;  // |x| < 2^-29
+        id = -1;
+    } else {
+        x = fabsf(x);
+        if (ix < 0x3f980000u) {    // |x| < 1.1875
+            if (ix < 0x3f300000u) {  // 7/16 <= |x| < 11/16
+                id = 0;
+                x = (2.0f * x - 1.0f) / (2.0f + x);
+            } else {  // 11/16 <= |x| < 19/16
+                id = 1;
+                x = (x - 1.0f) / (x + 1.0f);
+            }
+        } else if (ix < 0x401c0000u) {  // |x| < 2.4375
+            id = 2;
+            x = (x - 1.5f) / (1.0f + 1.5f * x);
+        } else {  // 2.4375 <= |x| < 2^25
+            id = 3;
+            x = -1.0f / x;
+        }
+    }
+    const float z = x * x;
+    const float w = z * z;
+    const float s1 = z * (aT[0] + w * (aT[2] + w * (aT[4] + w * (aT[6] + w * (aT[8] + w * aT[10])))));
+    const float s2 = w * (aT[1] + w * (aT[3] + w * (aT[5] + w * (aT[7] + w * aT[9]))));
+    if (id < 0) return x - x * (s1 + s2);
+    const float r = atanhi[id] - ((x * (s1 + s2) - atanlo[id]) - x);
+    return (int32_t)hx < 0 ? -r : r;
+}
+
+TSL_HD float atan2f_glibc(float y, float x)
+{
+    const float tiny = 1.0e-30f, pi_o_4 = 7.8539818525e-01f, pi_o_2 = 1.5707963705e+00f, pi = 3.1415927410e+00f,
+                pi_lo = -8.7422776573e-08f;
+    const uint32_t hx = f32_bits(x), hy = f32_bits(y);
+    const uint32_t ix = hx & 0x7fffffffu, iy = hy & 0x7fffffffu;
+    if (ix > 0x7f800000u || iy > 0x7f800000u) return x + y;  // NaN
+    if (hx == 0x3f800000u) return atanf_glibc(y);            // x = 1
+    const int m = (int)((hy >> 31) & 1u) | (int)((hx >> 30) & 2u);  // 2 sign(x) + sign(y)
+    if (iy == 0) {
+        if (m < 2) return y;  // atan(+-0, +anything) = +-0
+        return m == 2 ? pi + tiny : -pi - tiny;
+    }
+    if (ix == 0) return (int32_t)hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;
+    if (ix == 0x7f800000u) {
+        if (iy == 0x7f800000u) {
+            switch (m) {
+            case 0: return pi_o_4 + tiny;
+            case 1: return -pi_o_4 - tiny;
+            case 2: return 3.0f * pi_o_4 + tiny;
+            default: return -3.0f * pi_o_4 - tiny;
+            }
+        }
+        switch (m) {
+        case 0: return 0.0f;
+        case 1: return -0.0f;
+        case 2: return pi + tiny;
+        default: return -pi - tiny;
+        }
+    }
+    if (iy == 0x7f800000u) return (int32_t)hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;
+    const int k = ((int32_t)iy - (int32_t)ix) >> 23;
+    float z;
+    if (k > 60)
+        z = pi_o_2 + 0.5f * pi_lo;  // |y/x| > 2^60
+    else if ((int32_t)hx < 0 && k < -60)
+        z = 0.0f;  // |y|/x < -2^60
+    else
+        z = atanf_glibc(fabsf(y / x));
+    switch (m) {
+    case 0: return z;
+    case 1: return f32_from_bits(f32_bits(z) ^ 0x80000000u);
+    case 2: return pi - (z - pi_lo);
+    default: return (z - pi_lo) - pi;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Pose matrix of a particle composed with the scanner->base transform
 // (tsdf_evaluator.cpp:102-145): L = [R | t] * T, R = Rz(yaw) Ry(pitch)
 // Rx(roll), left-to-right float evaluation exactly as the reference writes it.
@@ -366,18 +515,33 @@ TSL_HD void apply_motion(const float last[6], const double d[6], float out[6], i
         t[4 * r + 2] = p[4 * r] * o[2] + p[4 * r + 1] * o[6] + p[4 * r + 2] * o[10];
         t[4 * r + 3] = p[4 * r] * o[3] + p[4 * r + 1] * o[7] + p[4 * r + 2] * o[11] + p[4 * r + 3];
     }
-    // getAngleFromMat on the 3x4 part (entries 0, 4, 8, 9, 10 of the 4x4)
+    // getAngleFromMat on the 3x4 part (entries 0, 4, 8, 9, 10 of the 4x4):
+    // float arguments, so asin, atan2 and cos bind to the float functions
+    // (glibc's asinf/atan2f, cosf of the trig mode), the divisions are float;
+    // in the double mode, the C library's double functions
     float roll, pitch, yaw;
-    if (fabs((double)t[8]) >= 1.0) {
+    if (trig == kTrigDouble) {
+        if (fabs((double)t[8]) >= 1.0) {
+            yaw = 0.0f;
+            const double delta = atan2((double)t[9], (double)t[10]);
+            pitch = t[8] < 0.0f ? (float)(3.141592653589793 / 2.0) : (float)(-3.141592653589793 / 2.0);
+            roll = (float)delta;
+        } else {
+            pitch = (float)(-asin((double)t[8]));
+            const double cpd = cos((double)pitch);
+            roll = (float)atan2((double)t[9] / cpd, (double)t[10] / cpd);
+            yaw = (float)atan2((double)t[4] / cpd, (double)t[0] / cpd);
+        }
+    } else if (fabsf(t[8]) >= 1.0f) {
         yaw = 0.0f;
-        const double delta = atan2((double)t[9], (double)t[10]);
+        roll = atan2f_glibc(t[9], t[10]);
         pitch = t[8] < 0.0f ? (float)(3.141592653589793 / 2.0) : (float)(-3.141592653589793 / 2.0);
-        roll = (float)delta;
     } else {
-        pitch = (float)(-asin((double)t[8]));
-        const double cpd = cos((double)pitch);
-        roll = (float)atan2((double)t[9] / cpd, (double)t[10] / cpd);
-        yaw = (float)atan2((double)t[4] / cpd, (double)t[0] / cpd);
+        pitch = -asinf_glibc(t[8]);
+        float sp, cp;
+        sincosf_mode(trig, pitch, &sp, &cp);
+        roll = atan2f_glibc(t[9] / cp, t[10] / cp);
+        yaw = atan2f_glibc(t[4] / cp, t[0] / cp);
     }
     out[0] = last[0] + t[3];
     out[1] = last[1] + t[7];

@@ -268,6 +268,17 @@ def check_trig_libm(mode, stride):
     return bad.value, n.value
 
 
+def check_invtrig_libm(stride, atan2_samples):
+    """Host build of the motion model's asinf and atanf against this
+    process's C library over every non-NaN float whose bit pattern is a
+    multiple of stride, and of its atan2f on atan2_samples seeded random
+    pairs: (mismatches, checked)."""
+    bad = ctypes.c_uint64()
+    n = ctypes.c_uint64()
+    check(lib.tsl_check_invtrig_libm(int(stride), int(atan2_samples), ctypes.byref(bad), ctypes.byref(n)))
+    return bad.value, n.value
+
+
 def resample_systematic(particles, u0):
     """Device systematic resampling; returns (new_particles, source_indices)."""
     p = _particles(particles)
