@@ -541,8 +541,10 @@ typedef struct tsl_result {
  * scans of the largest admitted robot -- are not charged); max_robots
  * resident at once; max_particles per robot;
  * max_batch requests per tick; chunk_len scan points per evaluation chunk
- * (0 = each request's whole scan).  A request's result depends on its own
- * chunking only, never on the tick it ran in. */
+ * of robots with fewer than 65,536 particle slots (0 = each request's whole
+ * scan); robots with more (the bulk class) are always evaluated a whole
+ * scan per chunk.  A request's result depends on its own chunking only,
+ * never on the tick it ran in. */
 int tsl_server_create(tsl_evaluator* ev, uint64_t hbm_budget_bytes, uint32_t max_robots, uint64_t max_particles,
                       uint32_t max_batch, uint32_t chunk_len, tsl_server** out);
 int tsl_server_destroy(tsl_server* s);
@@ -577,9 +579,10 @@ int tsl_server_stats(tsl_server* s, uint64_t* used_bytes, uint64_t* ticks, uint6
 int tsl_server_set_slice(tsl_server* s, uint64_t pairs, uint32_t reserved_per_cu);
 
 /* Evaluation tile of the bulk class (bulk != 0) or of the others: k
- * particles per wavefront (1, 2) and points per lane (1, 2, 4); default 2 and
- * 1 for both.  Changes only the speed (a member's chunking is the server's
- * chunk_len).  Takes effect for ticks started afterwards. */
+ * particles per wavefront (1, 2) and points per lane (1, 2, 4); default 1
+ * and 2 for the bulk class, 2 and 1 for the others.  Changes only the speed
+ * (a member's chunking does not depend on it).  Takes effect for ticks
+ * started afterwards. */
 int tsl_server_set_tile(tsl_server* s, int bulk, int k, int points_per_lane);
 
 /* ---- .mcl snapshot files ------------------------------------------------ */

@@ -186,7 +186,9 @@ def test_mixed_classes(tsl):
                 assert res[j]["status"] == 0
                 f = tsl.DeviceFilter(ev, r.particles.shape[0])
                 f.set_particles(r.particles)
-                f.set_tile(2, 2048, 1)
+                # the server's chunking: 2048 points for the tracking robot,
+                # the whole scan for the bulk (global) one
+                f.set_tile(2, 2048 if j == 0 else (r.points.shape[0] + 63) // 64 * 64, 1)
                 f.set_order(0)
                 f.motion_async(motions[j], s)
                 s.synchronize()

@@ -209,11 +209,11 @@ def test_bulk_robot_sliced_equals_unsliced_and_tracking_interleaves(tsl):
             srv.close()
     assert outs[0][0] == outs[1][0]
     assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
-    # the plain filter with the same chunk
+    # the plain filter with the same chunk (a bulk robot's: the whole scan)
     s = tsl.Stream()
     f = tsl.DeviceFilter(ev, w.particles.shape[0])
     f.set_particles(w.particles)
-    f.set_tile(2, 2048, 1)
+    f.set_tile(2, (w.points.shape[0] + 63) // 64 * 64, 1)
     f.motion_async(motion, s)
     f.set_scan(w.points, w.tf)
     f.update_async(s)

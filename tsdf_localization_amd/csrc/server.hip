@@ -137,8 +137,8 @@ struct tsl_server {
     uint64_t budget = 0, used = 0;
     uint32_t max_batch = 64;
     uint32_t chunk_len = 0;        // 0: each request's whole scan is one chunk
-    int K[kClasses] = {2, 2};      // particles per wavefront of the batched evaluation, per class
-    int P[kClasses] = {1, 1};      // points per lane, per class
+    int K[kClasses] = {2, 1};      // particles per wavefront of the batched evaluation, per class
+    int P[kClasses] = {1, 2};      // points per lane, per class
     uint64_t max_particles = 0;    // per robot; sizes the shared reduction grid
     std::atomic<uint32_t> max_points{0};  // largest scan any admitted robot may send
     std::vector<Robot> robots;     // fixed size (max_robots): entries never move
@@ -278,7 +278,11 @@ int dispatch(tsl_server* s, Slot* sl)
         const Pending& pd = sl->reqs[i];
         const size_t np = pd.points.size();
         f->num_points = np;
-        const uint32_t chunk = s->chunk_len ? s->chunk_len : (uint32_t)std::max<size_t>(64, (np + 63) / 64 * 64);
+        // tracking robots: the server's chunk length; bulk robots: the whole
+        // scan per item (their particles alone give enough items, and one
+        // chunk stages each pose matrix and scan tile once)
+        const uint32_t chunk = (s->chunk_len && !rb.bulk) ? s->chunk_len
+                                                          : (uint32_t)std::max<size_t>(64, (np + 63) / 64 * 64);
         const uint32_t chunks = (uint32_t)std::max<size_t>(1, (np + chunk - 1) / chunk);
         Member& h = sl->members_host[i];
         std::memset(&h, 0, sizeof(h));
@@ -585,7 +589,8 @@ int tsl_server_add_robot(tsl_server* s, const tsl_particle* particles, size_t n,
     // everything a tick can touch, reserved now: ticks never allocate
     if (rc == TSL_OK) rc = filter_reserve_particles(f, cap);
     if (rc == TSL_OK) rc = filter_reserve_scan(f, max_points);
-    const size_t chunks = s->chunk_len ? ((size_t)max_points + s->chunk_len - 1) / s->chunk_len : 1;
+    const bool bulk = f->capacity >= kSpatialOrderMin;
+    const size_t chunks = (s->chunk_len && !bulk) ? ((size_t)max_points + s->chunk_len - 1) / s->chunk_len : 1;
     if (rc == TSL_OK) rc = f->partial.reserve(sizeof(float) * std::max<size_t>(chunks, 1) * f->capacity);
     if (rc == TSL_OK) rc = f->ubuf.reserve(sizeof(float) * f->capacity);
     if (rc == TSL_OK) rc = f->plan.reserve(sizeof(RsPlan));
