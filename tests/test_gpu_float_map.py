@@ -154,3 +154,70 @@ def test_palette_and_float_forms_agree(tsl, room_f32):
     wp = ev_p.evaluate_raw(parts, room.points, room.tf)
     wf = ev_f.evaluate_raw(parts, room.points, room.tf)
     assert np.array_equal(wp.view(np.uint32), wf.view(np.uint32))
+
+
+def test_float_form_batch_and_server(tsl, room_f32):
+    """The batched and served forms on a float-form map (their own kernel
+    instances): every batch member and every served robot gives the bits of
+    the same update run alone with the same chunk length."""
+    from tsdf_localization_amd.server import Server
+    room, data = room_f32
+    m = tsl.CudaSubVoxelMap(room.map.coef(), room.map.grid_occ, data=data)
+    ev = tsl.TSDFEvaluator(m)
+    rng = np.random.default_rng(9)
+    chunk = 320
+    mem = []
+    for P, N in [(300, 900), (1000, 2000), (37, 64)]:
+        parts = room.particles[rng.choice(room.particles.shape[0], P, replace=False)].copy()
+        parts[:, 6] = 1.0 / P
+        mem.append((parts, room.points[rng.permutation(room.points.shape[0])[:N]]))
+    u0 = [np.float32(0.41 / p.shape[0]) for p, _ in mem]
+    s = tsl.Stream()
+
+    def alone(parts, pts, u, motion=None, ch=chunk):
+        f = tsl.DeviceFilter(ev, parts.shape[0])
+        f.set_particles(parts)
+        f.set_tile(2, ch, 1)
+        f.set_order(0)
+        if motion is not None:
+            f.motion_async(motion, s)
+        f.set_scan(pts, room.tf)
+        f.update_async(s)
+        pose, _ = f.sync(s)
+        f.resample_async(u, s)
+        f.sync(s)
+        return pose, f.get_particles()
+
+    # batch
+    for k, pp in [(1, 2), (2, 1)]:
+        filters = []
+        for parts, pts in mem:
+            f = tsl.DeviceFilter(ev, parts.shape[0])
+            f.set_particles(parts)
+            f.set_scan(pts, room.tf)
+            filters.append(f)
+        b = tsl.Batch(ev, filters)
+        b.set_tile(k, chunk, pp)
+        b.update_async(s)
+        b.sync(s)
+        b.resample_async(np.array(u0, np.float32), s)
+        b.sync(s)
+        for i, (parts, pts) in enumerate(mem):
+            _, ref = alone(parts, pts, u0[i])
+            assert np.array_equal(filters[i].get_particles().view(np.uint32), ref.view(np.uint32)), (k, pp, i)
+    # server
+    srv = Server(ev, max_robots=4, max_particles=2000, max_batch=4, chunk_len=chunk)
+    try:
+        ids = [srv.add_robot(p, 3000, room.tf) for p, _ in mem]
+        motions = [tsl.evaluator.motion_from_delta(0.05, 0.01, 0.0, 0.0, 0.0, 0.02, 0.01, 0.005, seed=300 + i)
+                   for i in range(len(mem))]
+        for i, (p, pts) in enumerate(mem):
+            srv.submit(ids[i], motions[i], pts, u0[i], request_id=i)
+        res = {r["id"]: r for r in srv.wait(len(mem))}
+        for i, (parts, pts) in enumerate(mem):
+            assert res[i]["status"] == 0
+            pose, ref = alone(parts, pts, u0[i], motions[i])
+            assert res[i]["pose"] == pose, i
+            assert np.array_equal(srv.get_particles(ids[i]).view(np.uint32), ref.view(np.uint32)), i
+    finally:
+        srv.close()
