@@ -36,6 +36,10 @@ HIP_FLAGS = [
 
 HIP_SOURCES = ["api.hip", "eval.hip", "resample.hip", "tables.hip", "scan.hip", "batch.hip", "motion.hip", "mcl.hip", "server.hip", "chunks.hip",
                "noalloc.hip", "probe.hip"]
+# eval_tiles.hip is compiled once per evaluation tile shape (particles per
+# wavefront, points per lane), so that the kernel instantiations build in
+# parallel
+TILE_SHAPES = [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2), (2, 4)]
 SYNTH_SOURCES = ["synth.cpp"]
 
 LIBTSL = os.path.join(HERE, "libtsl.so")
@@ -53,16 +57,23 @@ def _run(cmd, verbose):
         print(out, flush=True)
 
 
-def build_tsl(verbose=False, jobs=4):
+def build_tsl(verbose=False, jobs=None):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
     procs = []
+    for k, p in TILE_SHAPES:
+        obj = os.path.join(objdir, f"eval_tiles_k{k}_p{p}.o")
+        procs.append([HIPCC, *HIP_FLAGS, f"-DTSL_TILE_K={k}", f"-DTSL_TILE_P={p}", "-c",
+                      os.path.join(CSRC, "eval_tiles.hip"), "-o", obj])
+        objs.append(obj)
     for src in HIP_SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         objs.append(obj)
         procs.append(cmd)
+    if jobs is None:
+        jobs = max(1, min(8, os.cpu_count() or 1))
     # compile in parallel
     from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(max_workers=jobs) as ex:
