@@ -320,6 +320,8 @@ def bench_serving(args):
     def fresh_server():
         srv = Server(ev, hbm_budget=args.hbm_budget_gb * (1 << 30), max_robots=R, max_particles=20_000,
                      max_batch=args.max_batch, chunk_len=args.chunk or 2048)
+        if args.served_tile:
+            srv.set_tile(0, args.served_tile[0], args.served_tile[1])
         ids = [srv.add_robot(r.particles, 20_000, w.tf) for r in w.robots]
         return srv, ids
 
@@ -691,8 +693,8 @@ def main():
     ap.add_argument("--mixed-reserve", type=int, default=2,
                     help="mixed: workgroup slots per CU a sliced global evaluation leaves to tracking requests")
     ap.add_argument("--served-tile", type=int, nargs=2, default=None, metavar=("K", "P"),
-                    help="campus: tile of the served (bulk-class) evaluation, particles per wavefront and points "
-                         "per lane (default: the server's)")
+                    help="campus, serving: tile of the served evaluation (campus: the bulk class; serving: the "
+                         "tracking class), particles per wavefront and points per lane (default: the server's)")
     ap.add_argument("--served-chunk", type=int, default=2048,
                     help="campus: scan points per item of the served evaluation (0 = the whole scan)")
     ap.add_argument("--mixed-small-maps", action="store_true",
