@@ -858,8 +858,9 @@ def main():
 
     probe = gather_probe() if not args.no_probe else None
 
-    # the north star's forms beside the plain update: batched filters and
-    # the serving loop, each against plain per-filter / per-request launches
+    # the headline metric's other forms beside the plain update: batched
+    # filters and the serving loop, each against plain per-filter /
+    # per-request launches
     headline = {}
     if not args.no_headline and w.name == "building":
         hb = argparse.Namespace(**vars(args))
