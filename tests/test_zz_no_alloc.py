@@ -114,3 +114,26 @@ def test_noalloc_empty_scan_and_capacity(tsl, room):
     assert rc == 0
     want = ev.evaluate_raw(room.particles, room.points, room.tf)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_noalloc_other_map_forms(tsl, room):
+    """The float form (a map with more distinct values than the palette) and
+    the 64-bit form (int64 block table): the allocation-free entry equals
+    tsl_evaluate_raw bit for bit on each, two calls in a row, the second
+    smaller."""
+    rng = np.random.default_rng(17)
+    data = ob.float_data(room.map)
+    data = (data * (np.float32(1.0) + rng.uniform(-1e-3, 1e-3, data.shape).astype(np.float32))).astype(np.float32)
+    maps = {
+        "float": tsl.CudaSubVoxelMap(room.map.coef(), room.map.grid_occ, data=data),
+        "wide": tsl.CudaSubVoxelMap(room.map.coef(), room.map.grid_occ.astype(np.int64), tsdf_mm=room.map.tsdf_mm,
+                                    truncation_mm=room.map.truncation_mm, sigma=room.map.sigma),
+    }
+    for name, m in maps.items():
+        ev = tsl.TSDFEvaluator(m)
+        for parts, pts in ((room.particles, room.points), (room.particles[:300], room.points[:700])):
+            print(f"noalloc {name} map P={parts.shape[0]} N={pts.shape[0]}", flush=True)
+            rc, got = _noalloc(ev, parts, pts, room.tf)
+            assert rc == 0, tsl._lib.lib.tsl_last_error()
+            want = ev.evaluate_raw(parts, pts, room.tf)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), name
