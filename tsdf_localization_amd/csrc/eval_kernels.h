@@ -165,12 +165,15 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 
     // The workgroup's waves walk the same chunk: its points are staged in
     // LDS one tile at a time (every wave helps and waits, active or not).
+    // tb, te, base and active are wavefront-uniform; readfirstlane says so to
+    // the compiler, which then keeps the point loop's control in scalar
+    // registers instead of running it as a divergent loop on the VALU
     for (uint32_t tb = j0; tb < j1; tb += kTile) {
-    const uint32_t te = min(j1, tb + kTile);
+    const uint32_t te = __builtin_amdgcn_readfirstlane(min(j1, tb + kTile));
     __syncthreads();  // the previous tile is consumed
     for (uint32_t i = threadIdx.x; i < te - tb; i += blockDim.x) tile[i] = pts[tb + i];
     __syncthreads();
-    if (!active) continue;
+    if (!__builtin_amdgcn_readfirstlane(active ? 1u : 0u)) continue;
     // one iteration: 64*kP consecutive points from base; kFull: all of
     // them lie before te (no per-lane tail checks)
     auto step = [&](uint32_t base, auto full_tag) {
@@ -357,7 +360,7 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
             }
         }
     };
-    uint32_t base = tb;
+    uint32_t base = __builtin_amdgcn_readfirstlane(tb);
     for (; base + 64u * kP <= te; base += 64u * kP) step(base, std::true_type{});
     if (base < te) step(base, std::false_type{});
     }
