@@ -136,8 +136,19 @@ __device__ __forceinline__ void reduce_slice(const Member& m, uint32_t P, uint32
     double w = 0.0;
     if (b < nb && i < P) {
         const uint32_t slot = m.inv ? m.inv[i] : i;
+        // the chunk partials in chunk order, four loads in flight at a time
+        const float* part = m.partial + slot;
         double acc = 0.0;
-        for (uint32_t c = 0; c < m.chunks; ++c) acc += (double)m.partial[(size_t)c * m.stride + slot];
+        uint32_t c = 0;
+        for (; c + 4 <= m.chunks; c += 4) {
+            const float v0 = part[(size_t)c * m.stride], v1 = part[(size_t)(c + 1) * m.stride],
+                        v2 = part[(size_t)(c + 2) * m.stride], v3 = part[(size_t)(c + 3) * m.stride];
+            acc += (double)v0;
+            acc += (double)v1;
+            acc += (double)v2;
+            acc += (double)v3;
+        }
+        for (; c < m.chunks; ++c) acc += (double)part[(size_t)c * m.stride];
         const float wf = (float)acc;
         m.raw[i] = wf;
         w = (double)wf;

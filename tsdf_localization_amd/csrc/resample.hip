@@ -288,7 +288,10 @@ __global__ void k_rs_u(const Member* __restrict__ members)
 // serial repairs of s and U from their first mismatches, then the number
 // emitted J = #{j : U_j < s_{M-1}} (U is non-decreasing).
 // Run by the first wavefront (threads 0..63) of a workgroup.
-__device__ void count_member(const Member& mb, uint32_t scan_blocks)
+// kSearch false (the one-workgroup form): stop after the capacity check and
+// leave s_{M-1} in *s_final_out (thread 0); the caller counts J in parallel.
+template <bool kSearch = true>
+__device__ void count_member(const Member& mb, uint32_t scan_blocks, double* s_final_out = nullptr)
 {
     RsPlan* p = mb.plan;
     const uint32_t cap = p->cap;
@@ -337,6 +340,10 @@ __device__ void count_member(const Member& mb, uint32_t scan_blocks)
     if ((double)U[cap - 1] < s_final) {  // weights sum beyond 1 + 1e-5
         st->status = kStCapacity;
         p->cap = 0;
+        return;
+    }
+    if constexpr (!kSearch) {
+        *s_final_out = s_final;
         return;
     }
     uint32_t lo = 0, hi = cap;  // first j with U_j >= s_final
@@ -393,24 +400,66 @@ __global__ void k_rs_copy_back(const Member* __restrict__ members)
 // The whole resampling of one member in one workgroup (kRsThreads
 // threads), for sets up to kRsSmall candidates: the phases above in
 // order, separated by workgroup barriers instead of launches, with the
-// same functions (a single scan span) and therefore the same bits.
+// same functions (a single scan span) and therefore the same bits.  The
+// run table and the running sum are searched in LDS copies and J is counted
+// in parallel (U is non-decreasing, so #{j : U_j < s_final} is the binary
+// search's answer): the same values and comparisons without the chains of
+// dependent global loads that dominate a launch of this size.
 __global__ __launch_bounds__(kRsThreads) void k_rs_small(Member* __restrict__ members, float u0_arg, int copy_back)
 {
+    __shared__ USeg seg_sh[kMaxSeg];
+    __shared__ double s_sh[kRsSmall];
+    __shared__ double s_final_sh;
+    __shared__ uint32_t count_sh[kRsThreads / 64];
     const Member& mb = members[blockIdx.y];
     const RsPlan* p = mb.plan;
     if (threadIdx.x == 0) plan_member(mb, u0_arg);
     __syncthreads();
     const uint32_t cap = p->cap;
     if (cap == 0) return;
-    scan_span(mb, 0, 1);
-    for (uint32_t j = threadIdx.x; j < cap; j += kRsThreads) u_step(mb, p, j);
+    const uint32_t nseg = p->nseg, M = p->M;
+    for (uint32_t k = threadIdx.x; k < nseg; k += kRsThreads) seg_sh[k] = p->seg[k];
+    scan_span(mb, 0, 1);  // its barriers also publish seg_sh
     __syncthreads();
-    if (threadIdx.x < 64) count_member(mb, 1);
+    // phase B (u_step) on the LDS run table
+    for (uint32_t j = threadIdx.x; j < cap; j += kRsThreads) {
+        const float u = u_candidate(seg_sh, nseg, j);
+        mb.ubuf[j] = u;
+        const float expect = j == 0 ? p->u0 : (float)((double)u_candidate(seg_sh, nseg, j - 1) + p->inv_m);
+        if (u != expect) atomicMin(&mb.st->first_bad, j);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) count_member<false>(mb, 1, &s_final_sh);
     __syncthreads();
     // re-read past this workgroup's cached copy: thread 0 may have cleared it
     if (__hip_atomic_load(&mb.plan->cap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
-    const uint32_t J = mb.st->resample_count;
-    for (uint32_t j = threadIdx.x; j < J; j += kRsThreads) assign_step(mb, p, j);
+    // J = #{j : U_j < s_final}, U as repaired
+    const double s_final = s_final_sh;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t below = 0;
+    for (uint32_t j0 = 0; j0 < cap; j0 += kRsThreads) {
+        const uint32_t j = j0 + threadIdx.x;
+        below += (uint32_t)__popcll(__ballot(j < cap && (double)mb.ubuf[j] < s_final));
+    }
+    if (lane == 0) count_sh[wave] = below;
+    for (uint32_t m = threadIdx.x; m < M && M <= kRsSmall; m += kRsThreads) s_sh[m] = mb.cumsum[m];
+    __syncthreads();
+    uint32_t J = 0;
+    for (int w = 0; w < kRsThreads / 64; ++w) J += count_sh[w];
+    if (threadIdx.x == 0) mb.st->resample_count = J;
+    // phase D (assign_step) on the LDS running sum
+    const double* s = M <= kRsSmall ? s_sh : mb.cumsum;
+    for (uint32_t j = threadIdx.x; j < J; j += kRsThreads) {
+        const double u = (double)mb.ubuf[j];
+        uint32_t lo = 0, hi = M;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s[mid] > u) hi = mid; else lo = mid + 1;
+        }
+        const uint32_t m = lo < M ? lo : M - 1;
+        if (mb.idx_out) mb.idx_out[j] = (int32_t)m;
+        mb.alt[j] = mb.particles[m];
+    }
     if (!copy_back) return;
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < J; j += kRsThreads) mb.particles[j] = mb.alt[j];
