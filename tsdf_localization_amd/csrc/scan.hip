@@ -19,9 +19,17 @@ namespace tsl {
 
 namespace {
 
-// particle-order buckets: kBucketBits bits per axis (x, y, yaw)
-constexpr uint32_t kBucketBits = 5;
-constexpr uint32_t kBuckets = 1u << (3 * kBucketBits);
+// particle-order buckets: kXyBits bits for each of x and y, kYawBits for
+// the heading
+#ifndef TSL_ORDER_XY_BITS
+#define TSL_ORDER_XY_BITS 6
+#endif
+#ifndef TSL_ORDER_YAW_BITS
+#define TSL_ORDER_YAW_BITS 5
+#endif
+constexpr uint32_t kXyBits = TSL_ORDER_XY_BITS;
+constexpr uint32_t kYawBits = TSL_ORDER_YAW_BITS;
+constexpr uint32_t kBuckets = 1u << (2 * kXyBits + kYawBits);
 
 __device__ __forceinline__ uint64_t spread3(uint64_t v)
 {
@@ -86,16 +94,22 @@ __device__ __forceinline__ uint32_t bucket_coord(float v, float inv_cell, uint32
     return (uint32_t)c;
 }
 
-// bits 3k, 3k+1, 3k+2 of the bucket = bit k of x, y, yaw
+// Morton key: from the lowest bit up, bit k of x, of y (k < kXyBits) and of
+// the heading sector (k < kYawBits)
 __device__ __forceinline__ uint32_t bucket_of(const tsl_particle& p, const BucketGrid& g)
 {
-    const uint32_t x = bucket_coord(p.first[0] - g.min_x, g.inv_x, 1u << kBucketBits);
-    const uint32_t y = bucket_coord(p.first[1] - g.min_y, g.inv_y, 1u << kBucketBits);
-    const uint32_t a = bucket_coord(p.first[5] + 3.14159265f, g.inv_yaw, 1u << kBucketBits);
-    uint32_t key = 0;
+    const uint32_t x = bucket_coord(p.first[0] - g.min_x, g.inv_x, 1u << kXyBits);
+    const uint32_t y = bucket_coord(p.first[1] - g.min_y, g.inv_y, 1u << kXyBits);
+    const uint32_t a = bucket_coord(p.first[5] + 3.14159265f, g.inv_yaw, 1u << kYawBits);
+    uint32_t key = 0, pos = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kBucketBits; ++k)
-        key |= (((x >> k) & 1u) << (3 * k)) | (((y >> k) & 1u) << (3 * k + 1)) | (((a >> k) & 1u) << (3 * k + 2));
+    for (uint32_t k = 0; k < (kXyBits > kYawBits ? kXyBits : kYawBits); ++k) {
+        if (k < kXyBits) {
+            key |= ((x >> k) & 1u) << pos++;
+            key |= ((y >> k) & 1u) << pos++;
+        }
+        if (k < kYawBits) key |= ((a >> k) & 1u) << pos++;
+    }
     return key;
 }
 
@@ -139,37 +153,6 @@ __global__ void k_bucket_count(const tsl_particle* __restrict__ in, const uint32
         bucket[i] = k;
     }
     (void)wave_bucket_add(hist, k, active);
-}
-
-// One workgroup: hist becomes the exclusive prefix sum of the counts.
-constexpr int kBucketThreads = 1024;
-constexpr uint32_t kPerThread = kBuckets / kBucketThreads;
-__global__ __launch_bounds__(kBucketThreads) void k_bucket_scan(uint32_t* __restrict__ hist)
-{
-    __shared__ uint32_t wsum[kBucketThreads / 64];
-    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    uint32_t v[kPerThread];
-    uint32_t run = 0;
-#pragma unroll
-    for (uint32_t e = 0; e < kPerThread; ++e) {
-        v[e] = hist[t * kPerThread + e];
-        run += v[e];
-    }
-    uint32_t incl = run;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t o = __shfl_up(incl, off, 64);
-        if ((int)lane >= off) incl += o;
-    }
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    uint32_t base = incl - run;
-    for (uint32_t w = 0; w < wave; ++w) base += wsum[w];
-#pragma unroll
-    for (uint32_t e = 0; e < kPerThread; ++e) {
-        hist[t * kPerThread + e] = base;
-        base += v[e];
-    }
 }
 
 __global__ void k_bucket_scatter(const uint32_t* __restrict__ count, const uint32_t* __restrict__ bucket,
@@ -382,10 +365,31 @@ int launch_reduce_scan(tsl_filter* f, int mode, const tsl_point* raw, const void
     return TSL_OK;
 }
 
+// Layout of f->pkeys: bucket counts, bucket offsets (their exclusive scan),
+// the bucket of each particle, the scan's temporary storage; each part
+// 256-byte aligned.
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+static int bucket_scan_temp_bytes(size_t* bytes)
+{
+    static size_t t = 0;
+    static bool known = false;
+    if (!known) {
+        TSL_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                       (int)kBuckets, (hipStream_t)0));
+        known = true;
+    }
+    *bytes = t;
+    return TSL_OK;
+}
+
 int reserve_particle_order(tsl_filter* f)
 {
     const size_t n = f->count_bound;
-    TSL_TRY(f->pkeys.reserve(sizeof(uint32_t) * ((size_t)kBuckets + n)));
+    size_t temp = 0;
+    TSL_TRY(bucket_scan_temp_bytes(&temp));
+    TSL_TRY(f->pkeys.reserve(2 * round256(sizeof(uint32_t) * kBuckets) + round256(sizeof(uint32_t) * n) +
+                             round256(temp)));
     TSL_TRY(f->porder.reserve(sizeof(uint32_t) * n));
     return f->pinv.reserve(sizeof(uint32_t) * n);
 }
@@ -395,27 +399,35 @@ int launch_particle_order(tsl_filter* f, hipStream_t stream)
     const uint32_t n = (uint32_t)f->count_bound;
     if (n == 0) return TSL_OK;
     TSL_TRY(reserve_particle_order(f));
-    uint32_t* hist = f->pkeys.as<uint32_t>();
-    uint32_t* bucket = hist + kBuckets;
+    size_t temp = 0;
+    TSL_TRY(bucket_scan_temp_bytes(&temp));
+    char* at = f->pkeys.as<char>();
+    uint32_t* hist = reinterpret_cast<uint32_t*>(at);
+    at += round256(sizeof(uint32_t) * kBuckets);
+    uint32_t* offsets = reinterpret_cast<uint32_t*>(at);
+    at += round256(sizeof(uint32_t) * kBuckets);
+    uint32_t* bucket = reinterpret_cast<uint32_t*>(at);
+    at += round256(sizeof(uint32_t) * n);
+    void* scan_temp = at;
     const uint32_t* count = &f->state.as<FilterState>()->count;
     // x and y cells spanning the map's extent (at least 0.5 m), yaw sectors
     // spanning [-pi, pi)
     const tsl_map_coef& c = f->ev->map->coef;
-    const float cells = (float)(1u << kBucketBits);
+    const float cells = (float)(1u << kXyBits);
     BucketGrid g;
     g.min_x = c.min_x;
     g.min_y = c.min_y;
     g.inv_x = 1.0f / std::max(0.5f, (c.max_x - c.min_x) / cells);
     g.inv_y = 1.0f / std::max(0.5f, (c.max_y - c.min_y) / cells);
-    g.inv_yaw = cells / 6.2831853f;
+    g.inv_yaw = (float)(1u << kYawBits) / 6.2831853f;
     hipLaunchKernelGGL(k_bucket_zero, dim3(kBuckets / 256), dim3(256), 0, stream, hist);
     TSL_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_bucket_count, dim3((n + 255) / 256), dim3(256), 0, stream, f->particles.as<tsl_particle>(),
                        count, g, hist, bucket);
     TSL_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(kBucketThreads), 0, stream, hist);
-    TSL_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_bucket_scatter, dim3((n + 255) / 256), dim3(256), 0, stream, count, bucket, hist,
+    size_t tb = temp;
+    TSL_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_temp, tb, hist, offsets, (int)kBuckets, stream));
+    hipLaunchKernelGGL(k_bucket_scatter, dim3((n + 255) / 256), dim3(256), 0, stream, count, bucket, offsets,
                        f->porder.as<uint32_t>());
     TSL_HIP_CHECK(hipGetLastError());
     return TSL_OK;
