@@ -430,7 +430,9 @@ def bench_mixed(args):
     robots (1,000,000 particles x 4,000 points): p50/p99 latency per class
     under that load, the saturation throughput of the served form (every
     request submitted at once) and plain per-request launches of the same
-    requests."""
+    requests.  Every robot keeps its particle set between its requests, so
+    a global robot's particles concentrate as its requests resample them and
+    its later requests gather from fewer distinct map lines."""
     import tsdf_localization_amd as tsl
     from tsdf_localization_amd import synth
     from tsdf_localization_amd.server import Server
@@ -686,7 +688,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="building",
                     choices=["building", "room", "batched", "serving", "mixed", "campus"])
-    ap.add_argument("--mixed-requests", type=int, default=400, help="mixed: number of requests")
+    ap.add_argument("--mixed-requests", type=int, default=2000,
+                    help="mixed: number of requests (2000: p99 of each class over more than 100 samples)")
     ap.add_argument("--mixed-rate", type=float, default=50.0, help="mixed: Poisson arrival rate (requests/s)")
     ap.add_argument("--mixed-slice", type=int, default=1 << 27,
                     help="mixed: particle-point pairs per launch of a global robot's evaluation (0 = one launch)")
