@@ -198,10 +198,13 @@ def copy_calibration(mib, reps):
 def bench_batched(args):
     """64 independent filters on one shared map (particle counts log-uniform
     in 500-50k, scans of 2k-30k points): one step = every filter restored,
-    updated and resampled.  Timed three ways on one stream: plain per-filter
-    launches (the drop-in path, one filter after the other), the batched
-    launch sequence with the tile the batch autotuner picks, and the batched
-    step replayed from a HIP graph."""
+    updated and resampled.  Timed four ways: plain per-filter launches of
+    device-resident filters on one stream (one filter after the other), the
+    batched launch sequence with the tile the batch autotuner picks, the
+    batched step replayed from a HIP graph, and the drop-in as the
+    reference's node calls it, filter after filter on host arrays
+    (Evaluator::evaluate: particles and scan copied in, weights copied back;
+    then the systematic resampler on the host particle array)."""
     import tsdf_localization_amd as tsl
     from tsdf_localization_amd import synth
 
@@ -264,6 +267,24 @@ def bench_batched(args):
     log(f"batched graph K={k} points/lane={pp} chunk={chunk or 'auto'}: {graph_ms:.3f} ms/step, "
         f"{evals / graph_ms * 1e3:.3e} evals/s ({plain_ms / graph_ms:.2f}x plain); "
         f"member status {sorted(set(status.tolist()))}")
+
+    # the drop-in on host arrays, filter after filter; each filter keeps its
+    # own copy of its particles (evaluate overwrites only the weights, the
+    # resampled set is dropped, as the device step restores its snapshot)
+    host_parts = [r.particles.copy() for r in w.robots]
+
+    def dropin_step(i):
+        for r, parts, u0 in zip(w.robots, host_parts, draws[i]):
+            ev.evaluate(parts, r.points, w.tf)
+            tsl.evaluator.resample_systematic(parts, u0)
+    for i in range(args.warmup):
+        dropin_step(i)
+    t_d = time.perf_counter()
+    for i in range(args.steps):
+        dropin_step(args.warmup + i)
+    dropin_ms = (time.perf_counter() - t_d) * 1e3 / args.steps
+    log(f"drop-in per filter on host arrays: {dropin_ms:.3f} ms/step, {evals / dropin_ms * 1e3:.3e} evals/s "
+        f"(batched graph {dropin_ms / graph_ms:.2f}x)")
     out = {
         "metric": "particle*point evals/s",
         "value": evals / graph_ms * 1e3,
@@ -286,6 +307,9 @@ def bench_batched(args):
         "plain_value": evals / plain_ms * 1e3,
         "batched_eager_ms_per_step": eager_ms,
         "speedup_vs_plain": plain_ms / graph_ms,
+        "plain_dropin_ms_per_step": dropin_ms,
+        "plain_dropin_value": evals / dropin_ms * 1e3,
+        "speedup_vs_plain_dropin": dropin_ms / graph_ms,
     }
     return out
 
@@ -295,7 +319,11 @@ def bench_serving(args):
     5k-20k point scan + draw) from a Poisson process of 200 requests/s:
     p50/p99 latency (submission to result) under that load, and the
     saturation throughput of the served form (every request submitted at
-    once) against plain per-request launches of the same requests."""
+    once) against plain per-request launches of the same requests, two
+    ways: device-resident filters (motion, update and resampling on the
+    device), and the drop-in as the reference's node calls it on host arrays
+    (Evaluator::evaluate, then the systematic resampler; the motion step,
+    host code in that node, is left out of this leg)."""
     import tsdf_localization_amd as tsl
     from tsdf_localization_amd import synth
     from tsdf_localization_amd.server import Server
@@ -398,6 +426,21 @@ def bench_serving(args):
     plain_s = time.perf_counter() - t2
     plain_rate = sum(evals) / plain_s
     log(f"plain per-request: {plain_s * 1e3:.1f} ms, {plain_rate:.3e} evals/s; served/plain {served_rate / plain_rate:.2f}x")
+
+    # 4. the drop-in per request on host arrays; each robot's copy of its
+    # particles is evaluated (only its weights change) and resampled into a
+    # new array that is dropped, so every request evaluates the particle set
+    # the served form starts from
+    host_parts = {r: w.robots[r].particles.copy() for r in set(robots_of.tolist())}
+    t3 = time.perf_counter()
+    for i in range(n_req):
+        r = robots_of[i]
+        ev.evaluate(host_parts[r], w.robots[r].points, w.tf)
+        tsl.evaluator.resample_systematic(host_parts[r], u0[i])
+    dropin_s = time.perf_counter() - t3
+    dropin_rate = sum(evals) / dropin_s
+    log(f"drop-in per request on host arrays: {dropin_s * 1e3:.1f} ms, {dropin_rate:.3e} evals/s; "
+        f"served/drop-in {served_rate / dropin_rate:.2f}x")
     out = {
         "metric": "particle*point evals/s",
         "value": served_rate,
@@ -418,6 +461,8 @@ def bench_serving(args):
         "latency_ms_by_rate": by_rate,
         "plain_value": plain_rate,
         "speedup_vs_plain": served_rate / plain_rate,
+        "plain_dropin_value": dropin_rate,
+        "speedup_vs_plain_dropin": served_rate / dropin_rate,
     }
     return out
 
@@ -878,11 +923,14 @@ def main():
         hb.steps, hb.warmup = args.headline_steps, 2
         b = bench_batched(hb)
         headline["batched"] = {k2: b[k2] for k2 in ("value", "ms_per_step", "plain_value", "plain_ms_per_step",
-                                                    "speedup_vs_plain", "batched_eager_ms_per_step")}
+                                                    "speedup_vs_plain", "batched_eager_ms_per_step",
+                                                    "plain_dropin_value", "plain_dropin_ms_per_step",
+                                                    "speedup_vs_plain_dropin")}
         headline["batched"]["config"] = b["config"]
         s = bench_serving(argparse.Namespace(**dict(vars(args), extra_rates=[1000.0])))
         headline["serving"] = {k2: s[k2] for k2 in ("value", "plain_value", "speedup_vs_plain", "latency_ms",
-                                                    "latency_ms_by_rate")}
+                                                    "latency_ms_by_rate", "plain_dropin_value",
+                                                    "speedup_vs_plain_dropin")}
         headline["serving"]["config"] = s["config"]
 
     out = {
