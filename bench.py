@@ -892,16 +892,22 @@ def main():
     # reference's CudaEvaluator (PCIe-inclusive; reported beside the value)
     dropin = None
     if not args.no_dropin:
-        reps = 3
+        # median of 10 calls after a warm-up: pageable host copies vary
+        # from call to call with the host's memory traffic
+        reps = 10
         copies = [w.particles.copy() for _ in range(reps + 1)]
         ev.evaluate(copies[0], w.points, w.tf)
-        t0 = time.perf_counter()
+        times = []
         for i in range(reps):
+            t0 = time.perf_counter()
             ev.evaluate(copies[i + 1], w.points, w.tf)
-        dropin_ms = (time.perf_counter() - t0) * 1000.0 / reps
+            times.append((time.perf_counter() - t0) * 1000.0)
+        dropin_ms = float(np.median(times))
         dropin = {"ms_per_update": dropin_ms, "evals_per_s": evals / dropin_ms * 1e3,
+                  "ms_min_max": [float(min(times)), float(max(times))], "calls": reps,
                   "form": "tsl_evaluate on host arrays: copy in, pose + evaluate + normalise, copy weights out"}
-        log(f"drop-in evaluate on host arrays: {dropin_ms:.3f} ms/update, {evals / dropin_ms * 1e3:.3e} evals/s")
+        log(f"drop-in evaluate on host arrays: {dropin_ms:.3f} ms/update (median of {reps}; "
+            f"{min(times):.3f}-{max(times):.3f}), {evals / dropin_ms * 1e3:.3e} evals/s")
 
     cpu = None
     if not args.no_cpu:

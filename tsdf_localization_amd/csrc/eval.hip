@@ -83,57 +83,90 @@ __global__ void k_pose_m(const Member* __restrict__ members)
 
 // Reduction partials of a member: blocks[0, nb) the weight sums of the nb
 // workgroup-sized slices, blocks[nb + q * nb + b] the nine pose sums of
-// slice b.
+// slice b.  The member kernels keep them in m.blocks (device memory, read by
+// later workgroups and launches); k_finish_small keeps them in LDS.
 
 // The slice functions below run as groups of kBlock threads: kG groups per
 // workgroup (1 in the member kernels, 4 in k_finish_small), group g =
 // threadIdx.x / kBlock working on its own slice; every thread of the
-// workgroup calls them (their barriers are workgroup-wide).  A group sum is
-// formed in one fixed order whatever kG is (the fixed butterfly of each
-// wave, then the group's waves in order from 0.0), so a slice gives the
-// same bits in either kernel.
-template <int kG>
-__device__ __forceinline__ double group_sum_d(double v, double* sh /* kWaves * kG */)
+// workgroup calls them (their barriers are workgroup-wide, and every lane
+// of every wavefront is active there).  A group sum is formed in one fixed
+// order whatever kG is (the row sums below, then the group's 16-lane rows
+// in order from 0.0), so a slice gives the same bits in either kernel.
+
+// v from the lane that DPP control kCtrl selects (both 32-bit halves).
+template <int kCtrl>
+__device__ __forceinline__ double dpp_d(double v)
 {
-    v = wave_sum_d(v);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) sh[wave] = v;
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// The sum of v over each 16-lane row, in every lane of the row, without the
+// LDS: the xor butterfly with partners 1 and 2 (quad permutations), then 4
+// and 8 through the half-row and row mirrors.  Addition commutes, so after
+// the first two steps the four lanes of a quad hold the same value and
+// after the third the eight lanes of a half-row do; the mirrors therefore
+// pair every lane with a lane that holds exactly what its xor-4 (xor-8)
+// partner holds.
+__device__ __forceinline__ double row_sum_d(double v)
+{
+    v += dpp_d<0xB1>(v);   // quad_perm [1, 0, 3, 2]: lane ^ 1
+    v += dpp_d<0x4E>(v);   // quad_perm [2, 3, 0, 1]: lane ^ 2
+    v += dpp_d<0x141>(v);  // row_half_mirror: lane 7 - l of its eight
+    v += dpp_d<0x140>(v);  // row_mirror: lane 15 - l of its row
+    return v;
+}
+
+constexpr int kRows = kWaves * 4;  // 16-lane rows per group
+
+template <int kG>
+__device__ __forceinline__ double group_sum_d(double v, double* sh /* kRows * kG */)
+{
+    v = row_sum_d(v);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if ((lane & 15) == 0) sh[wave * 4 + (lane >> 4)] = v;
     __syncthreads();
     double r = 0.0;
     if ((threadIdx.x & (kBlock - 1)) == 0)
-        for (int w = 0; w < kWaves; ++w) r += sh[wave + w];
+        for (int k = 0; k < kRows; ++k) r += sh[wave * 4 + k];
     __syncthreads();
     return r;  // in the group's thread 0
 }
 
 template <int kG>
-__device__ __forceinline__ double group_sum9(const double (&v)[9], double* sh /* kWaves * kG * 9 */)
+__device__ __forceinline__ double group_sum9(const double (&v)[9], double* sh /* kRows * kG * 9 */)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double w[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const double w = wave_sum_d(v[k]);
-        if (lane == 0) sh[wave * 9 + k] = w;
+    for (int k = 0; k < 9; ++k) w[k] = row_sum_d(v[k]);
+    if ((lane & 15) == 0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sh[(wave * 4 + (lane >> 4)) * 9 + k] = w[k];
     }
     __syncthreads();
     double r = 0.0;
     const uint32_t lt = threadIdx.x & (kBlock - 1);
     if (lt < 9)
-        for (int w = 0; w < kWaves; ++w) r += sh[(wave + w) * 9 + lt];
+        for (int k = 0; k < kRows; ++k) r += sh[(wave * 4 + k) * 9 + lt];
     __syncthreads();
     return r;  // sum k in the group's thread k < 9
 }
 
 // raw[p] = sum over chunks (ascending) of partial[c][p], in double, stored as
-// float; the partial sum of raw in double over slice b (particles
-// [b kBlock, (b+1) kBlock)) into blocks[b] (a slice past nb contributes and
-// writes nothing).
+// float (and returned); the partial sum of raw in double over slice b
+// (particles [b kBlock, (b+1) kBlock)) into blocks[b] (a slice past nb
+// contributes and writes nothing).
 template <int kG>
-__device__ __forceinline__ void reduce_slice(const Member& m, uint32_t P, uint32_t nb, uint32_t b, double* sh)
+__device__ __forceinline__ float reduce_slice(const Member& m, uint32_t P, uint32_t nb, uint32_t b, double* sh,
+                                              double* blocks)
 {
     const uint32_t lt = threadIdx.x & (kBlock - 1);
     const uint32_t i = b * kBlock + lt;
     double w = 0.0;
+    float wf = 0.0f;
     if (b < nb && i < P) {
         const uint32_t slot = m.inv ? m.inv[i] : i;
         // the chunk partials in chunk order, four loads in flight at a time
@@ -149,19 +182,20 @@ __device__ __forceinline__ void reduce_slice(const Member& m, uint32_t P, uint32
             acc += (double)v3;
         }
         for (; c < m.chunks; ++c) acc += (double)part[(size_t)c * m.stride];
-        const float wf = (float)acc;
+        wf = (float)acc;
         m.raw[i] = wf;
         w = (double)wf;
     }
     const double t = group_sum_d<kG>(w, sh);
-    if (lt == 0 && b < nb) m.blocks[b] = t;
+    if ((lt == 0) & (b < nb)) blocks[b] = t;
+    return wf;
 }
 
 __global__ __launch_bounds__(kBlock) void k_reduce_m(const Member* __restrict__ members)
 {
     const Member& m = members[blockIdx.y];
-    __shared__ double sh[kWaves];
-    reduce_slice<1>(m, m.st->count, gridDim.x, blockIdx.x, sh);
+    __shared__ double sh[kRows];
+    reduce_slice<1>(m, m.st->count, gridDim.x, blockIdx.x, sh, m.blocks);
 }
 
 // sum = total of the nb raw-weight partials (thread k of a group adds
@@ -171,11 +205,11 @@ __global__ __launch_bounds__(kBlock) void k_reduce_m(const Member* __restrict__ 
 // as a float) by thread 0 of a publishing workgroup.
 template <int kG>
 __device__ __forceinline__ float weight_total(const Member& m, uint32_t nb, bool publish, double* sh,
-                                              double* total_sh /* kG */)
+                                              double* total_sh /* kG */, const double* blocks)
 {
     const uint32_t lt = threadIdx.x & (kBlock - 1);
     double t = 0.0;
-    for (uint32_t k = lt; k < nb; k += kBlock) t += m.blocks[k];
+    for (uint32_t k = lt; k < nb; k += kBlock) t += blocks[k];
     const double total = group_sum_d<kG>(t, sh);
     if (lt == 0) total_sh[threadIdx.x / kBlock] = total;
     if (threadIdx.x == 0 && publish) {
@@ -186,14 +220,33 @@ __device__ __forceinline__ float weight_total(const Member& m, uint32_t nb, bool
     return (float)total_sh[threadIdx.x / kBlock];
 }
 
+// The pose of particle i of slice b (zeros outside the member), read ahead
+// of the weight so that its load overlaps other work.
+struct SlicePose {
+    float f[6];
+};
+
+__device__ __forceinline__ SlicePose slice_pose(const Member& m, uint32_t P, uint32_t nb, uint32_t b)
+{
+    const uint32_t i = b * kBlock + (threadIdx.x & (kBlock - 1));
+    SlicePose sp;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sp.f[k] = 0.0f;
+    if (b < nb && i < P) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sp.f[k] = m.particles[i].first[k];
+    }
+    return sp;
+}
+
 // Slice b: w_i = raw_i / sum written to particles[i].second (the
 // reference's particles[index].second /= weight_sum, float division) and
-// over raw_i, and
-// the slice's partials of the nine weighted pose sums
-// (tsdf_evaluator.cpp:198-217) in double into blocks[nb + q nb + b].
+// over raw_i, and the slice's partials of the nine weighted pose sums
+// (tsdf_evaluator.cpp:198-217) in double into blocks[nb + q nb + b].  raw_i
+// and the pose come from the caller.
 template <int kG>
 __device__ __forceinline__ void normalize_slice(const Member& m, uint32_t P, uint32_t nb, uint32_t b, float sum,
-                                                double* sh)
+                                                double* sh, double* blocks, float raw_i, const SlicePose& sp)
 {
     const uint32_t lt = threadIdx.x & (kBlock - 1);
     const uint32_t i = b * kBlock + lt;
@@ -201,36 +254,35 @@ __device__ __forceinline__ void normalize_slice(const Member& m, uint32_t P, uin
 #pragma unroll
     for (int k = 0; k < 9; ++k) q[k] = 0.0;
     if (b < nb && i < P && sum != 0.0f) {
-        tsl_particle* particles = m.particles;
-        const float w = m.raw[i] / sum;
-        particles[i].second = w;
+        const float w = raw_i / sum;
+        m.particles[i].second = w;
         m.raw[i] = w;  // a contiguous copy of the normalised weights for host readers
         const double wd = (double)w;
-        q[0] = (double)particles[i].first[0] * wd;
-        q[1] = (double)particles[i].first[1] * wd;
-        q[2] = (double)particles[i].first[2] * wd;
+        q[0] = (double)sp.f[0] * wd;
+        q[1] = (double)sp.f[1] * wd;
+        q[2] = (double)sp.f[2] * wd;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float s, c;
-            sincosf_mode(m.trig, particles[i].first[3 + a], &s, &c);
+            sincosf_mode(m.trig, sp.f[3 + a], &s, &c);
             q[3 + 2 * a] = (double)s * wd;
             q[4 + 2 * a] = (double)c * wd;
         }
     }
     const double t = group_sum9<kG>(q, sh);
-    if (lt < 9 && b < nb) m.blocks[nb + (size_t)lt * nb + b] = t;
+    if ((lt < 9) & (b < nb)) blocks[nb + (size_t)lt * nb + b] = t;
 }
 
 // The nine pose sums from nb slice partials into stats[1..9] (every group
 // forms them; group 0 stores).
 template <int kG>
-__device__ __forceinline__ void pose_sums(const Member& m, uint32_t nb, double* sh)
+__device__ __forceinline__ void pose_sums(const Member& m, uint32_t nb, double* sh, const double* blocks)
 {
     const uint32_t lt = threadIdx.x & (kBlock - 1);
     double t[9];
 #pragma unroll
     for (int qq = 0; qq < 9; ++qq) t[qq] = 0.0;
-    const double* part = m.blocks + nb;
+    const double* part = blocks + nb;
     for (uint32_t k = lt; k < nb; k += kBlock) {
         double v[9];
 #pragma unroll
@@ -252,34 +304,52 @@ __device__ __forceinline__ void pose_sums(const Member& m, uint32_t nb, double* 
 __global__ __launch_bounds__(kBlock) void k_normalize_m(const Member* __restrict__ members)
 {
     const Member& m = members[blockIdx.y];
-    __shared__ double sh[kWaves * 9];
+    __shared__ double sh[kRows * 9];
     __shared__ double total_sh[1];
     const uint32_t nb = gridDim.x;
-    const float sum = weight_total<1>(m, nb, blockIdx.x == 0, sh, total_sh);
-    normalize_slice<1>(m, m.st->count, nb, blockIdx.x, sum, sh);
+    const uint32_t P = m.st->count;
+    const uint32_t b = blockIdx.x, i = b * kBlock + threadIdx.x;
+    const SlicePose sp = slice_pose(m, P, nb, b);
+    const float raw_i = i < P ? m.raw[i] : 0.0f;
+    const float sum = weight_total<1>(m, nb, blockIdx.x == 0, sh, total_sh, m.blocks);
+    normalize_slice<1>(m, P, nb, b, sum, sh, m.blocks, raw_i, sp);
 }
 
 __global__ __launch_bounds__(kBlock) void k_pose_sums_m(const Member* __restrict__ members, uint32_t nb)
 {
-    __shared__ double sh[kWaves * 9];
-    pose_sums<1>(members[blockIdx.y], nb, sh);
+    __shared__ double sh[kRows * 9];
+    pose_sums<1>(members[blockIdx.y], nb, sh, members[blockIdx.y].blocks);
 }
 
 constexpr int kFinishGroups = 4;
+constexpr int kFinishRounds = (kFinishSmallSlices + kFinishGroups - 1) / kFinishGroups;
 __global__ __launch_bounds__(kBlock * kFinishGroups) void k_finish_small(const Member* __restrict__ members,
                                                                            uint32_t nb)
 {
     const Member& m = members[blockIdx.y];
-    __shared__ double sh[kWaves * kFinishGroups * 9];
+    __shared__ double sh[kRows * kFinishGroups * 9];
     __shared__ double total_sh[kFinishGroups];
+    __shared__ double blocks[kFinishSmallSlices * 10];  // the slice partials, in LDS
     const uint32_t P = m.st->count;
     const uint32_t g = threadIdx.x / kBlock;
-    for (uint32_t b0 = 0; b0 < nb; b0 += kFinishGroups) reduce_slice<kFinishGroups>(m, P, nb, b0 + g, sh);
+    // every slice this thread normalises: pose read ahead, raw weight kept
+    SlicePose sp[kFinishRounds];
+    float raw[kFinishRounds];
+#pragma unroll
+    for (int r = 0; r < kFinishRounds; ++r) sp[r] = slice_pose(m, P, nb, r * kFinishGroups + g);
+#pragma unroll
+    for (int r = 0; r < kFinishRounds; ++r) {
+        raw[r] = 0.0f;
+        if ((uint32_t)(r * kFinishGroups) < nb) raw[r] = reduce_slice<kFinishGroups>(m, P, nb, r * kFinishGroups + g, sh, blocks);
+    }
     __syncthreads();  // the slice partials are written
-    const float sum = weight_total<kFinishGroups>(m, nb, true, sh, total_sh);
-    for (uint32_t b0 = 0; b0 < nb; b0 += kFinishGroups) normalize_slice<kFinishGroups>(m, P, nb, b0 + g, sum, sh);
+    const float sum = weight_total<kFinishGroups>(m, nb, true, sh, total_sh, blocks);
+#pragma unroll
+    for (int r = 0; r < kFinishRounds; ++r)
+        if ((uint32_t)(r * kFinishGroups) < nb)
+            normalize_slice<kFinishGroups>(m, P, nb, r * kFinishGroups + g, sum, sh, blocks, raw[r], sp[r]);
     __syncthreads();  // the pose partials are written
-    pose_sums<kFinishGroups>(m, nb, sh);
+    pose_sums<kFinishGroups>(m, nb, sh, blocks);
 }
 
 // Restore every member from its snapshot.

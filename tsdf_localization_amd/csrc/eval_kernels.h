@@ -34,13 +34,6 @@ __device__ __forceinline__ float wave_sum(float v)
     return v;
 }
 
-__device__ __forceinline__ double wave_sum_d(double v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 template <bool kU24, bool kRank, bool kWide>
 __device__ __forceinline__ uint64_t slow_index(const DevMap& map, const SubEntry* __restrict__ tab,
                                                const uint64_t* __restrict__ neg, const OccWord* __restrict__ occ,
