@@ -89,7 +89,7 @@ __host__ __device__ uint64_t cap_walk(uint64_t M, float u0, double inv_m, USeg* 
 // Plan (one thread per member): M from the device count, cap, run table.
 // A member whose update failed (status set) or that holds no particles is
 // skipped (cap = 0) and keeps its particles.
-__device__ void plan_member(const Member& m, float u0_arg)
+__device__ void plan_member(const Member& m, float u0_arg, USeg* seg_out = nullptr)
 {
     FilterState* st = m.st;
     RsPlan* p = m.plan;
@@ -108,7 +108,7 @@ __device__ void plan_member(const Member& m, float u0_arg)
     }
     const double inv_m = 1.0 / (double)M;
     uint32_t nseg = 0;
-    const uint64_t cap = cap_walk(M, u0, inv_m, p->seg, &nseg);
+    const uint64_t cap = cap_walk(M, u0, inv_m, seg_out ? seg_out : p->seg, &nseg);
     if (cap == 0) {
         st->status = kStNoAdvance;
         return;
@@ -398,72 +398,175 @@ __global__ void k_rs_copy_back(const Member* __restrict__ members)
 }
 
 // The whole resampling of one member in one workgroup (kRsThreads
-// threads), for sets up to kRsSmall candidates: the phases above in
-// order, separated by workgroup barriers instead of launches, with the
-// same functions (a single scan span) and therefore the same bits.  The
-// run table and the running sum are searched in LDS copies and J is counted
-// in parallel (U is non-decreasing, so #{j : U_j < s_final} is the binary
-// search's answer): the same values and comparisons without the chains of
-// dependent global loads that dominate a launch of this size.
+// threads), for sets of up to kRsSmall particles and candidates: the phases
+// above in order, separated by workgroup barriers instead of launches, on
+// LDS copies of the run table, the weights, the running sum and U (the
+// device-memory copies are not written; nothing reads them after a
+// resampling).  Candidates are formed and checked as in the member kernels
+// and repaired serially from the first mismatch, so s and U are the
+// sequential recurrences and every output is the same; the particles move
+// through registers instead of the alternate buffer when they replace the
+// resident set.
+constexpr int kRsSmallPer = (int)(kRsSmall / kRsThreads);  // particles and candidates per thread
+static_assert(kRsSmall % kRsThreads == 0, "");
+
 __global__ __launch_bounds__(kRsThreads) void k_rs_small(Member* __restrict__ members, float u0_arg, int copy_back)
 {
     __shared__ USeg seg_sh[kMaxSeg];
     __shared__ double s_sh[kRsSmall];
-    __shared__ double s_final_sh;
+    __shared__ float w_sh[kRsSmall];
+    __shared__ float u_sh[kRsSmall];
+    __shared__ double wave_tot[kRsThreads / 64];
     __shared__ uint32_t count_sh[kRsThreads / 64];
+    __shared__ uint32_t s_bad_sh, u_bad_sh, go_sh, cap_sh, nseg_sh;
+    __shared__ double inv_m_sh;
+    __shared__ float u0_sh;
     const Member& mb = members[blockIdx.y];
-    const RsPlan* p = mb.plan;
-    if (threadIdx.x == 0) plan_member(mb, u0_arg);
-    __syncthreads();
-    const uint32_t cap = p->cap;
-    if (cap == 0) return;
-    const uint32_t nseg = p->nseg, M = p->M;
-    for (uint32_t k = threadIdx.x; k < nseg; k += kRsThreads) seg_sh[k] = p->seg[k];
-    scan_span(mb, 0, 1);  // its barriers also publish seg_sh
-    __syncthreads();
-    // phase B (u_step) on the LDS run table
-    for (uint32_t j = threadIdx.x; j < cap; j += kRsThreads) {
-        const float u = u_candidate(seg_sh, nseg, j);
-        mb.ubuf[j] = u;
-        const float expect = j == 0 ? p->u0 : (float)((double)u_candidate(seg_sh, nseg, j - 1) + p->inv_m);
-        if (u != expect) atomicMin(&mb.st->first_bad, j);
+    RsPlan* p = mb.plan;
+    FilterState* st = mb.st;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the weights are read while thread 0 plans the draw
+    const uint32_t M = st->count;
+    float w[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const uint32_t m = 2 * tid + e;
+        w[e] = (m < M && M <= kRsSmall) ? mb.particles[m].second : 0.0f;
+    }
+    if (tid == 0) {
+        s_bad_sh = 0xffffffffu;
+        u_bad_sh = 0xffffffffu;
+        plan_member(mb, u0_arg, seg_sh);
+        uint32_t go = p->cap != 0;
+        if (go && (M > kRsSmall || p->cap > kRsSmall)) {  // the launcher keeps both within kRsSmall
+            st->status = kStCapacity;
+            p->cap = 0;
+            go = 0;
+        }
+        go_sh = go;
+        cap_sh = p->cap;
+        nseg_sh = p->nseg;
+        inv_m_sh = p->inv_m;
+        u0_sh = p->u0;
     }
     __syncthreads();
-    if (threadIdx.x < 64) count_member<false>(mb, 1, &s_final_sh);
+    if (!go_sh) return;
+    const uint32_t cap = cap_sh, nseg = nseg_sh;
+    const double inv_m = inv_m_sh;
+    const float u0 = u0_sh;
+    // running-sum candidates (one span, as k_rs_scan), checked against
+    // s_m == s_{m-1} + w_m
+    const double d0 = (double)w[0], d1 = (double)w[1];
+    const double run = d0 + d1;
+    const double incl = wave_incl_scan(run, (int)lane);
+    if (lane == 63) wave_tot[wave] = incl;
+    w_sh[2 * tid] = w[0];
+    w_sh[2 * tid + 1] = w[1];
     __syncthreads();
-    // re-read past this workgroup's cached copy: thread 0 may have cleared it
-    if (__hip_atomic_load(&mb.plan->cap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
-    // J = #{j : U_j < s_final}, U as repaired
-    const double s_final = s_final_sh;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double woff = 0.0;
+    for (uint32_t k = 0; k < wave; ++k) woff += wave_tot[k];
+    const double excl = woff + (incl - run);
+    s_sh[2 * tid] = excl + d0;
+    s_sh[2 * tid + 1] = excl + run;
+    // U candidates from the run table, checked against one step of the
+    // recurrence
+    for (uint32_t j = tid; j < cap; j += kRsThreads) {
+        const float u = u_candidate(seg_sh, nseg, j);
+        u_sh[j] = u;
+        const float expect = j == 0 ? u0 : (float)((double)u_candidate(seg_sh, nseg, j - 1) + inv_m);
+        if (u != expect) atomicMin(&u_bad_sh, j);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const uint32_t m = 2 * tid + e;
+        if (m < M) {
+            const double prev = m == 0 ? 0.0 : s_sh[m - 1];
+            if (s_sh[m] != prev + (double)w_sh[m]) atomicMin(&s_bad_sh, m);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // serial repairs from the first mismatches, then the capacity check
+        const uint32_t bad = s_bad_sh;
+        if (bad < M) {
+            double acc = bad == 0 ? 0.0 : s_sh[bad - 1];
+            for (uint32_t m = bad; m < M; ++m) {
+                acc += (double)w_sh[m];
+                s_sh[m] = acc;
+            }
+        }
+        const uint32_t fb = u_bad_sh;
+        if (fb != 0xffffffffu) {
+            float u = fb == 0 ? u0 : (float)((double)u_sh[fb - 1] + inv_m);
+            for (uint32_t j = fb; j < cap; ++j) {
+                u_sh[j] = u;
+                u = (float)((double)u + inv_m);
+            }
+        }
+        st->first_bad = 0xffffffffu;
+        uint32_t go = 1;
+        if ((double)u_sh[cap - 1] < s_sh[M - 1]) {  // weights sum beyond 1 + 1e-5
+            st->status = kStCapacity;
+            p->cap = 0;
+            go = 0;
+        }
+        go_sh = go;
+    }
+    __syncthreads();
+    if (!go_sh) return;
+    // J = #{j : U_j < s_final} (U is non-decreasing)
+    const double s_final = s_sh[M - 1];
     uint32_t below = 0;
     for (uint32_t j0 = 0; j0 < cap; j0 += kRsThreads) {
-        const uint32_t j = j0 + threadIdx.x;
-        below += (uint32_t)__popcll(__ballot(j < cap && (double)mb.ubuf[j] < s_final));
+        const uint32_t j = j0 + tid;
+        below += (uint32_t)__popcll(__ballot(j < cap && (double)u_sh[j] < s_final));
     }
     if (lane == 0) count_sh[wave] = below;
-    for (uint32_t m = threadIdx.x; m < M && M <= kRsSmall; m += kRsThreads) s_sh[m] = mb.cumsum[m];
     __syncthreads();
     uint32_t J = 0;
-    for (int w = 0; w < kRsThreads / 64; ++w) J += count_sh[w];
-    if (threadIdx.x == 0) mb.st->resample_count = J;
-    // phase D (assign_step) on the LDS running sum
-    const double* s = M <= kRsSmall ? s_sh : mb.cumsum;
-    for (uint32_t j = threadIdx.x; j < J; j += kRsThreads) {
-        const double u = (double)mb.ubuf[j];
-        uint32_t lo = 0, hi = M;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s[mid] > u) hi = mid; else lo = mid + 1;
+    for (int k = 0; k < kRsThreads / 64; ++k) J += count_sh[k];
+    // output j is particle min{m : s_m > U_j}, held in registers (its
+    // seven floats)
+    float out[kRsSmallPer][7];
+#pragma unroll
+    for (int r = 0; r < kRsSmallPer; ++r) {
+        const uint32_t j = tid + (uint32_t)r * kRsThreads;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) out[r][k] = 0.0f;
+        if (j < J) {
+            const double u = (double)u_sh[j];
+            uint32_t lo = 0, hi = M;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_sh[mid] > u) hi = mid; else lo = mid + 1;
+            }
+            const uint32_t m = lo < M ? lo : M - 1;
+            if (mb.idx_out) mb.idx_out[j] = (int32_t)m;
+            const tsl_particle& src = mb.particles[m];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) out[r][k] = src.first[k];
+            out[r][6] = src.second;
+            if (!copy_back) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) mb.alt[j].first[k] = out[r][k];
+                mb.alt[j].second = out[r][6];
+            }
         }
-        const uint32_t m = lo < M ? lo : M - 1;
-        if (mb.idx_out) mb.idx_out[j] = (int32_t)m;
-        mb.alt[j] = mb.particles[m];
     }
+    if (tid == 0) st->resample_count = J;
     if (!copy_back) return;
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < J; j += kRsThreads) mb.particles[j] = mb.alt[j];
-    if (threadIdx.x == 0) mb.st->count = J;
+    __syncthreads();  // every source particle is read
+#pragma unroll
+    for (int r = 0; r < kRsSmallPer; ++r) {
+        const uint32_t j = tid + (uint32_t)r * kRsThreads;
+        if (j < J) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) mb.particles[j].first[k] = out[r][k];
+            mb.particles[j].second = out[r][6];
+        }
+    }
+    if (tid == 0) st->count = J;
 }
 
 }  // namespace
