@@ -759,20 +759,31 @@ def main():
     ap.add_argument("--graph-step", action="store_true", help="also time the step replayed from a HIP graph")
     ap.add_argument("--cpu-sample", type=int, default=10000)
     ap.add_argument("--cpu-repeats", type=int, default=10)
-    ap.add_argument("--autotune", action="store_true", help="pick the tile with tsl_filter_autotune")
-    ap.add_argument("--tile-k", type=int, default=PINNED_TILE["k"], help="particles per wavefront (1, 2)")
-    ap.add_argument("--tile-p", type=int, default=PINNED_TILE["p"], help="points per lane per iteration (1, 2, 4)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the tile with tsl_filter_autotune (the room's default when no tile flag is given)")
+    ap.add_argument("--no-autotune", action="store_true", help="room: the pinned tile instead of the autotuner's")
+    ap.add_argument("--tile-k", type=int, default=None, help="particles per wavefront (1, 2)")
+    ap.add_argument("--tile-p", type=int, default=None, help="points per lane per iteration (1, 2, 4)")
     ap.add_argument("--order", type=int, default=None,
                     help="particle order: 0 caller, 1 spatial, -1 auto (default: the pinned tile's on the "
                          "building and campus workloads, auto on the room)")
-    ap.add_argument("--lookup", type=int, default=PINNED_TILE["lookup"], help="block lookup: 0 table, 1 LDS rank, -1 auto")
-    ap.add_argument("--chunk", type=int, default=PINNED_TILE["chunk"], help="scan points per wavefront (0 = automatic)")
+    ap.add_argument("--lookup", type=int, default=None, help="block lookup: 0 table, 1 LDS rank, -1 auto")
+    ap.add_argument("--chunk", type=int, default=None, help="scan points per wavefront (0 = automatic)")
     ap.add_argument("--no-headline", action="store_true", help="skip the batched and serving forms in the default run")
     ap.add_argument("--headline-steps", type=int, default=10, help="batched steps timed in the default run")
     ap.add_argument("--no-probe", action="store_true", help="skip the gather probe")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run plain copies of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
+    # the room's update is a chain of small latency-bound kernels whose best
+    # tile differs from the global-localisation workload's pinned one: the
+    # library's autotuner picks it unless a tile flag is given
+    tile_flags = (args.tile_k, args.tile_p, args.lookup, args.chunk, args.order)
+    if args.workload == "room" and not args.no_autotune and all(v is None for v in tile_flags):
+        args.autotune = True
+    for name, key in (("tile_k", "k"), ("tile_p", "p"), ("lookup", "lookup"), ("chunk", "chunk")):
+        if getattr(args, name) is None:
+            setattr(args, name, PINNED_TILE[key])
     if args.order is None:
         # the room's 1,000 particles gain nothing from the spatial order's
         # sort kernels; the automatic choice takes the caller's order there
@@ -821,7 +832,7 @@ def main():
     lookup = args.lookup
     if args.autotune:
         order, k, pp, chunk, tms = f.autotune(stream, reps=3)
-        lookup = -1
+        lookup = None  # the autotuner sets the block lookup too
         picked_by = "tsl_filter_autotune"
         log(f"autotuned: order={order} K={k} points/lane={pp} chunk={chunk or 'auto'} "
             f"({tms:.3f} ms per update for evaluation and particle order)")
@@ -960,7 +971,7 @@ def main():
                    "step": "restore + pose + evaluate + normalise + resample",
                    "tile": {"particle_order": {0: "caller", 1: "spatial", -1: "auto"}[order],
                             "particles_per_wavefront": k, "points_per_lane": pp, "chunk_points": chunk or "auto",
-                            "block_lookup": {0: "table", 1: "LDS rank", -1: "auto"}[lookup],
+                            "block_lookup": {0: "table", 1: "LDS rank", -1: "auto", None: "autotuned"}[lookup],
                             "picked_by": picked_by, "profile_key": tile_key(tile)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
