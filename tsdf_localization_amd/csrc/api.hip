@@ -833,6 +833,7 @@ int tsl_filter_destroy(tsl_filter* f)
     (void)hipSetDevice(f->device);
     (void)hipDeviceSynchronize();
     if (f->host_state) (void)hipHostFree(f->host_state);
+    if (f->weights_host) (void)hipHostFree(f->weights_host);
     if (f->motion_host) (void)hipHostFree(f->motion_host);
     if (f->motion_copied) (void)hipEventDestroy(static_cast<hipEvent_t>(f->motion_copied));
     if (f->ev_start) (void)hipEventDestroy(static_cast<hipEvent_t>(f->ev_start));
@@ -1212,21 +1213,56 @@ static int scratch_filter(tsl_evaluator* ev, size_t P, tsl_filter** f)
     return TSL_OK;
 }
 
-// The evaluation of a host particle set against the scan already on the
-// evaluator's scratch filter: weights overwritten with the normalised
-// weights (cuda_evaluator.cu:405-408), pose returned.
+// The evaluation of a host particle set on the evaluator's scratch filter,
+// against a host scan (points != nullptr) or the scan already on it:
+// weights overwritten with the normalised weights (cuda_evaluator.cu:
+// 405-408), pose returned.  The copies in, the scan preparation, the update
+// and the read-back of the state and of the normalised weights (kept
+// contiguous by the normalisation) are queued back to back on the null
+// stream with one synchronisation at the end; the calls before it leave no
+// work in flight on this filter, and a buffer that grows is freed by
+// hipFree, which waits for the device.
 static int evaluate_filter_scan(tsl_evaluator* ev, tsl_filter* f, tsl_particle* particles, size_t P,
-                                tsl_pose* out_pose)
+                                tsl_pose* out_pose, const tsl_point* points = nullptr, size_t N = 0,
+                                const float* tf = nullptr)
 {
     (void)ev;
     if (P == 0) return set_error(TSL_ERR_NO_VALID_PARTICLE, "No particle is valid!");
-    TSL_TRY(tsl_filter_set_particles(f, particles, P));
-    TSL_TRY(tsl_filter_update_async(f, nullptr));
-    int rc = tsl_filter_sync(f, nullptr, out_pose, nullptr);
-    if (rc != TSL_OK) return rc;
-    // the normalised weights, also kept contiguous by the normalisation
-    std::vector<float> w(P);
-    TSL_HIP_CHECK(hipMemcpy(w.data(), f->raw.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));
+    if (P > 0x7fffffffull) return set_error(TSL_ERR_UNSUPPORTED, "too many particles");
+    if (N > 0x7fffffffull) return set_error(TSL_ERR_UNSUPPORTED, "too many points");
+    TSL_HIP_CHECK(hipSetDevice(f->device));
+    hipStream_t s = nullptr;
+    f->member_on_stream = true;  // the descriptor is written in stream order, by a kernel
+    if (f->weights_host_cap < P) {
+        if (f->weights_host) (void)hipHostFree(f->weights_host);
+        f->weights_host = nullptr;
+        f->weights_host_cap = 0;
+        if (hipHostMalloc(&f->weights_host, sizeof(float) * P, hipHostMallocDefault) != hipSuccess)
+            return set_error(TSL_ERR_OUT_OF_MEMORY, "hipHostMalloc failed");
+        f->weights_host_cap = P;
+    }
+    if (points) {
+        TSL_TRY(filter_reserve_points(f, N));
+        TSL_HIP_CHECK(hipMemcpyAsync(f->points_in.ptr, points, sizeof(tsl_point) * N, hipMemcpyHostToDevice, s));
+        f->num_points = N;
+        std::memcpy(f->tf, tf, sizeof(float) * 16);
+        TSL_TRY(launch_prep_points(f, s));
+    }
+    f->count_bound = 0;
+    TSL_TRY(filter_reserve_particles(f, P));
+    TSL_HIP_CHECK(hipMemcpyAsync(f->particles.ptr, particles, sizeof(tsl_particle) * P, hipMemcpyHostToDevice, s));
+    f->count_bound = P;
+    f->count_known = true;
+    TSL_TRY(launch_set_count(f, (uint32_t)P, s));
+    TSL_TRY(launch_update(f, s));
+    FilterState* hs = host_state(f);
+    TSL_HIP_CHECK(hipMemcpyAsync(hs, f->state.ptr, sizeof(FilterState), hipMemcpyDeviceToHost, s));
+    TSL_HIP_CHECK(hipMemcpyAsync(f->weights_host, f->raw.ptr, sizeof(float) * P, hipMemcpyDeviceToHost, s));
+    TSL_HIP_CHECK(hipStreamSynchronize(s));
+    f->count_bound = hs->count;
+    pose_from_stats(hs->stats, out_pose);
+    TSL_TRY(status_error(hs->status));
+    const float* w = static_cast<const float*>(f->weights_host);
     for (size_t i = 0; i < P; ++i) particles[i].second = w[i];
     return TSL_OK;
 }
@@ -1243,8 +1279,7 @@ int tsl_evaluate(tsl_evaluator* ev, tsl_particle* particles, size_t P, const tsl
     }
     tsl_filter* f;
     TSL_TRY(scratch_filter(ev, P, &f));
-    TSL_TRY(tsl_filter_set_scan(f, points, N, tf));
-    return evaluate_filter_scan(ev, f, particles, P, out_pose);
+    return evaluate_filter_scan(ev, f, particles, P, out_pose, points, N, tf);
 }
 
 int tsl_evaluate_raw(tsl_evaluator* ev, const tsl_particle* particles, size_t P, const tsl_point* points,

@@ -359,6 +359,8 @@ struct tsl_filter {
     bool member_dirty = true;
     bool member_on_stream = false;  // write the descriptor with a kernel on the launch stream, every time
     void* host_state = nullptr;  // pinned FilterState mirror
+    void* weights_host = nullptr;  // pinned read-back of the normalised weights (drop-in evaluation)
+    size_t weights_host_cap = 0;   // floats it holds
 };
 
 namespace tsl {

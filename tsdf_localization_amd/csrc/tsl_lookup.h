@@ -128,8 +128,8 @@ __device__ __forceinline__ bool fast_ok(float ox, float oy, float oz, uint32_t r
 // Wrap tier: |o| < R on every axis.
 __device__ __forceinline__ bool wrap_ok(float ox, float oy, float oz, uint32_t range_bits)
 {
-    return ((__float_as_uint(ox) & 0x7fffffffu) < range_bits) & ((__float_as_uint(oy) & 0x7fffffffu) < range_bits) &
-           ((__float_as_uint(oz) & 0x7fffffffu) < range_bits);
+    return max(max(__float_as_uint(ox) & 0x7fffffffu, __float_as_uint(oy) & 0x7fffffffu),
+               __float_as_uint(oz) & 0x7fffffffu) < range_bits;
 }
 
 // Offset of local voxel (lx, ly, lz), each < sub_dim, inside a bricked block:
