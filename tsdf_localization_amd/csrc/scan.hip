@@ -123,23 +123,31 @@ __global__ void k_bucket_zero(uint32_t* __restrict__ hist)
 // distinct key of the wavefront (the particles of a tracking filter share a
 // few buckets, and per-lane atomics on one counter serialise); returns the
 // counter's old value plus the lane's rank among the lanes with its key.
-// Every lane of the wavefront calls it.
+// The lanes are grouped by key first (ballots only), then every group's
+// first lane adds its group's size in one atomic instruction for all
+// groups, so the wavefront waits for one atomic round trip however many
+// keys it holds.  Every lane of the wavefront calls it.
 __device__ __forceinline__ uint32_t wave_bucket_add(uint32_t* __restrict__ hist, uint32_t key, bool active)
 {
     const uint32_t lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
     uint64_t pending = __ballot(active);
-    uint32_t pos = 0;
+    uint32_t leader_of = lane, rank = 0, group = 0;
     while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lk = __shfl(key, leader, 64);
+        const int leader = __ffsll((unsigned long long)pending) - 1;  // wavefront-uniform
+        const uint32_t lk = __builtin_amdgcn_readlane(key, leader);
         const uint64_t same = __ballot(active && key == lk);
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&hist[lk], (uint32_t)__popcll(same));
-        base = __shfl(base, leader, 64);
-        if (active && key == lk) pos = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        if (active && key == lk) {
+            leader_of = (uint32_t)leader;
+            rank = (uint32_t)__popcll(same & below);
+            group = (uint32_t)__popcll(same);
+        }
         pending &= ~same;
     }
-    return pos;
+    uint32_t base = 0;
+    if (active && leader_of == lane) base = atomicAdd(&hist[key], group);
+    base = __shfl(base, (int)leader_of, 64);
+    return base + rank;
 }
 
 __global__ void k_bucket_count(const tsl_particle* __restrict__ in, const uint32_t* __restrict__ count,
