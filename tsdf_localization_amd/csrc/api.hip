@@ -427,12 +427,14 @@ static int map_finish(tsl_map* m, const int32_t* grid_occ, const std::vector<flo
             }
         }
         d.occ_nwords = 0;
+        d.occ_used = 0;
         d.occ_words = nullptr;
         if (rank) {
             TSL_TRY(m->occ_words.reserve(sizeof(OccWord) * nwords));
             TSL_HIP_CHECK(hipMemcpy(m->occ_words.ptr, words.data(), sizeof(OccWord) * nwords, hipMemcpyHostToDevice));
             d.occ_words = m->occ_words.as<OccWord>();
             d.occ_nwords = (uint32_t)nwords;
+            d.occ_used = (uint32_t)((span + 31) / 32);
             d.occ_log2 = log2w;
         }
     }

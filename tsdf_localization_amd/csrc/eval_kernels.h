@@ -24,7 +24,11 @@ namespace {
 constexpr int kBlock = 256;           // threads per workgroup (4 wavefronts)
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kXcds = 8;         // accelerator complex dies of the MI355X (one L2 each)
-constexpr uint32_t kTile = 512;       // scan points staged in LDS per step (8 KB), a multiple of 64 * 4
+#ifndef TSL_TILE_POINTS
+#define TSL_TILE_POINTS 512
+#endif
+constexpr uint32_t kTile = TSL_TILE_POINTS;  // scan points staged in LDS per step (8 KB), a multiple of 64 * 4
+static_assert(kTile % 256 == 0, "the point tile holds whole iterations of 64 lanes x 4 points");
 constexpr uint32_t kFinishSmallSlices = 8;  // filters of up to 8 * kBlock particles finish in one workgroup
 
 __device__ __forceinline__ float wave_sum(float v)
@@ -80,7 +84,7 @@ __device__ __forceinline__ LdsTables stage_tables(const DevMap& map, bool scaled
     float* lut = reinterpret_cast<float*>(smem_raw);
     AxisEntry* atab = reinterpret_cast<AxisEntry*>(smem_raw + (map.palette_size + 3) / 4);
     OccWord* occ = reinterpret_cast<OccWord*>(atab + map.axis_n);
-    uint64_t* neg = reinterpret_cast<uint64_t*>(occ + (kRank ? map.occ_nwords : 0u));
+    uint64_t* neg = reinterpret_cast<uint64_t*>(occ + (kRank ? map.occ_used : 0u));
     SubEntry* tab = reinterpret_cast<SubEntry*>(neg + 8);
     // the point tile and the workgroup words follow, 16-byte aligned (the
     // kernels declare no static LDS, so the palette sits at address 0)
@@ -96,7 +100,7 @@ __device__ __forceinline__ LdsTables stage_tables(const DevMap& map, bool scaled
     if (kRank) {
         const uint2* src = reinterpret_cast<const uint2*>(map.occ_words);
         uint2* dst = reinterpret_cast<uint2*>(occ);
-        for (uint32_t i = threadIdx.x; i < map.occ_nwords; i += blockDim.x) dst[i] = src[i];
+        for (uint32_t i = threadIdx.x; i < map.occ_used; i += blockDim.x) dst[i] = src[i];
     }
     if (threadIdx.x < 8) neg[threadIdx.x] = map.neg_tab[threadIdx.x];
     for (uint32_t i = threadIdx.x; i < map.sub_entries; i += blockDim.x) tab[i] = map.sub_tab[i];
@@ -106,13 +110,17 @@ __device__ __forceinline__ LdsTables stage_tables(const DevMap& map, bool scaled
     return LdsTables{atab, occ, neg, tab, lut, tile, scratch};
 }
 
-// LDS bytes stage_tables needs for a map.
+// LDS bytes stage_tables needs for a map.  The rank table's used words are
+// staged; the allocation still reaches the end of the whole power-of-two
+// table, so that the word a discarded lane addresses lies inside it.
 inline size_t lds_bytes(const DevMap& d, bool rank)
 {
-    const size_t tables = 16 * ((d.palette_size + 3) / 4) + sizeof(AxisEntry) * d.axis_n +
-                          (rank ? sizeof(OccWord) * d.occ_nwords : 0) + 8 * sizeof(uint64_t) +
+    const size_t head = 16 * ((d.palette_size + 3) / 4) + sizeof(AxisEntry) * d.axis_n;
+    const size_t tables = head + (rank ? sizeof(OccWord) * d.occ_used : 0) + 8 * sizeof(uint64_t) +
                           sizeof(SubEntry) * d.sub_entries;
-    return (tables + 15) / 16 * 16 + sizeof(float4) * kTile + 16;
+    const size_t total = (tables + 15) / 16 * 16 + sizeof(float4) * kTile + 16;
+    const size_t reach = rank ? head + sizeof(OccWord) * d.occ_nwords : 0;
+    return total > reach ? total : reach;
 }
 
 // One wavefront: K particles (pose matrices wavefront-uniform) x the points
@@ -367,13 +375,27 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
     }
 }
 
-template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
-__global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __restrict__ L,
-                                                 const uint32_t* __restrict__ count,
-                                                 const float4* __restrict__ pts, uint32_t N,
-                                                 uint32_t chunk_len, uint32_t chunks, float a_hit,
-                                                 float* __restrict__ partial, uint32_t stride,
-                                                 void* __restrict__ trace_out, const uint32_t* __restrict__ perm)
+// Instances with one particle per wavefront and integer voxel codes are
+// held to eight wavefronts per SIMD: the compiler then fits them in 80
+// scalar registers (parking a few wavefront-uniform values in vector-register
+// lanes), and with only the rank table's used words in LDS eight
+// workgroups fit on a CU; left to itself it takes 106 scalar registers,
+// which allow seven.  On the global-localisation workload: k_eval 4.12 ->
+// 3.84 ms.  Two particles per wavefront (24 matrix entries in scalar
+// registers) and the float form spill to memory under that limit and run
+// slower, so they keep the compiler's choice.
+template <int K, bool kF32>
+constexpr bool kEightWaves = K == 1 && !kF32;
+
+#define TSL_EVAL_PARAMS                                                                                    \
+    DevMap map, const float* __restrict__ L, const uint32_t* __restrict__ count,                           \
+        const float4* __restrict__ pts, uint32_t N, uint32_t chunk_len, uint32_t chunks, float a_hit,      \
+        float* __restrict__ partial, uint32_t stride, void* __restrict__ trace_out,                        \
+        const uint32_t* __restrict__ perm
+#define TSL_EVAL_PARAM_NAMES map, L, count, pts, N, chunk_len, chunks, a_hit, partial, stride, trace_out, perm
+
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide, bool kF32>
+__device__ __forceinline__ void eval_items(TSL_EVAL_PARAMS)
 {
     const LdsTables t = stage_tables<kRank>(map, kMode == 0, a_hit);
     const uint32_t P = *count;
@@ -396,6 +418,29 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
                                                         trace_out, perm, t.tile);
 }
 
+
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
+__global__ __launch_bounds__(kBlock) void k_eval(TSL_EVAL_PARAMS)
+{
+    eval_items<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>(TSL_EVAL_PARAM_NAMES);
+}
+
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_eval8(TSL_EVAL_PARAMS)
+{
+    eval_items<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>(TSL_EVAL_PARAM_NAMES);
+}
+
+// The evaluation kernel an instance launches.
+template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide, bool kF32>
+constexpr auto eval_kernel()
+{
+    if constexpr (kEightWaves<K, kF32>)
+        return &k_eval8<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>;
+    else
+        return &k_eval<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>;
+}
+
 // Batched evaluation: a persistent grid claims items one at a time, the
 // workgroups of XCD x (dealt round-robin by index) from the items x, x + 8,
 // ... through their XCD's counter, so all workgroups sweep the members
@@ -405,10 +450,13 @@ __global__ __launch_bounds__(kBlock) void k_eval(DevMap map, const float* __rest
 // start at zero), so a long evaluation can be cut into launches that other
 // streams' kernels interleave with; every item writes only its own partial
 // weights, so the slicing changes no bit.
-template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
-__global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member* __restrict__ members,
-                                                       BatchPlan* __restrict__ plan, float a_hit, uint32_t item0,
-                                                       uint32_t count)
+#define TSL_BATCH_PARAMS                                                                                   \
+    DevMap map, const Member* __restrict__ members, BatchPlan* __restrict__ plan, float a_hit, uint32_t item0, \
+        uint32_t count
+#define TSL_BATCH_PARAM_NAMES map, members, plan, a_hit, item0, count
+
+template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide, bool kF32>
+__device__ __forceinline__ void batch_items(TSL_BATCH_PARAMS)
 {
     const LdsTables t = stage_tables<kRank>(map, true, a_hit);
     float4* tile = t.tile;
@@ -449,6 +497,29 @@ __global__ __launch_bounds__(kBlock) void k_eval_batch(DevMap map, const Member*
     }
 }
 
+
+template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
+__global__ __launch_bounds__(kBlock) void k_eval_batch(TSL_BATCH_PARAMS)
+{
+    batch_items<K, kP, kU24, kBrick, kRank, kWide, kF32>(TSL_BATCH_PARAM_NAMES);
+}
+
+template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_eval_batch8(TSL_BATCH_PARAMS)
+{
+    batch_items<K, kP, kU24, kBrick, kRank, kWide, kF32>(TSL_BATCH_PARAM_NAMES);
+}
+
+// The batched evaluation kernel an instance launches.
+template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide, bool kF32>
+constexpr auto batch_kernel()
+{
+    if constexpr (kEightWaves<K, kF32>)
+        return &k_eval_batch8<K, kP, kU24, kBrick, kRank, kWide, kF32>;
+    else
+        return &k_eval_batch<K, kP, kU24, kBrick, kRank, kWide, kF32>;
+}
+
 // The scoring kernels read the palette at LDS address 0 (palette_at): they
 // must declare no static LDS, which would sit before the dynamic block.
 // Checked once per kernel.
@@ -468,10 +539,11 @@ template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWi
 int launch_eval_t(dim3 grid, size_t lds, hipStream_t stream, const tsl_filter* f, const Tile& tile, size_t N,
                   float* partial, void* trace_out)
 {
-    TSL_CHECK_PALETTE_AT_ZERO((&k_eval<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>));
+    constexpr auto kernel = eval_kernel<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>();
+    TSL_CHECK_PALETTE_AT_ZERO(kernel);
     const tsl_map* map = f->ev->map;
     FilterState* st = f->state.as<FilterState>();
-    hipLaunchKernelGGL((k_eval<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>), grid, dim3(kBlock), lds, stream,
+    hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, stream,
                        map->dev,
                        f->L.as<float>(), &st->count, f->points.as<float4>(), (uint32_t)N, tile.chunk_len,
                        tile.chunks, f->ev->a_hit, partial, (uint32_t)f->capacity, trace_out,
@@ -521,12 +593,13 @@ __global__ void k_batch_claims_reset(BatchPlan* __restrict__ plan)
 template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide = false, bool kF32 = false>
 int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid_out)
 {
-    TSL_CHECK_PALETTE_AT_ZERO((&k_eval_batch<K, kP, kU24, kBrick, kRank, kWide, kF32>));
+    constexpr auto kernel = batch_kernel<K, kP, kU24, kBrick, kRank, kWide, kF32>();
+    TSL_CHECK_PALETTE_AT_ZERO(kernel);
     const size_t lds = lds_bytes(b.map->dev, kRank);
     if (grid_out) {
         int per_cu = 0;
         TSL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&k_eval_batch<K, kP, kU24, kBrick, kRank, kWide, kF32>), kBlock,
+            &per_cu, reinterpret_cast<const void*>(kernel), kBlock,
             lds));
         if (per_cu < 1) per_cu = 1;
         int cus = 0;
@@ -546,7 +619,7 @@ int launch_eval_batch_t(const BatchLaunch& b, hipStream_t stream, uint32_t* grid
             hipLaunchKernelGGL(k_batch_claims_reset, dim3(1), dim3(64), 0, stream, b.plan);
             TSL_HIP_CHECK(hipGetLastError());
         }
-        hipLaunchKernelGGL((k_eval_batch<K, kP, kU24, kBrick, kRank, kWide, kF32>), dim3(b.grid), dim3(kBlock), lds,
+        hipLaunchKernelGGL(kernel, dim3(b.grid), dim3(kBlock), lds,
                            stream, b.map->dev, b.members, b.plan, b.a_hit, (uint32_t)i0,
                            (uint32_t)std::min<uint64_t>(slice, 0xffffffffu));
         TSL_HIP_CHECK(hipGetLastError());

@@ -180,6 +180,9 @@ struct DevMap {
     const OccWord* occ_words;
     uint32_t occ_nwords;              // 0: use grid_occ; else a power of two (zero words past the blocks)
     uint32_t occ_log2;                // log2(occ_nwords): block up reads word bits [5, 5 + occ_log2) of up
+    uint32_t occ_used;                // words staged in LDS: every block index inside the grid lies in
+                                      // them; the zero words past them are left out (a lane that forms a
+                                      // word index there fails the grid check and is discarded)
     uint64_t sentinel;                // element of codes holding the code of init_value
     uint32_t sentinel_code;           // that byte offset
     uint32_t wide;                    // 1: slots instead of grid_occ, 64-bit voxel and byte offsets
