@@ -677,7 +677,9 @@ def bench_campus(args):
     # afresh before each one so that every request sees the uniform particle
     # set the plain steps restore (a resampled set collapses onto the best
     # poses, whose lookups share cache lines)
-    srv = Server(ev, max_robots=2, max_particles=P, max_batch=4, chunk_len=args.served_chunk)
+    # (a robot of 65,536+ particle slots is in the server's bulk class, whose
+    # items are whole scans whatever the server's chunk length)
+    srv = Server(ev, max_robots=2, max_particles=P, max_batch=4)
     srv.set_slice(args.mixed_slice, args.mixed_reserve)
     if args.served_tile:
         srv.set_tile(1, args.served_tile[0], args.served_tile[1])
@@ -715,7 +717,7 @@ def bench_campus(args):
                      "evals_per_launch": pairs // max(launches, 1)},
         "served": {"ms_per_request": served_ms, "evals_per_s": evals / served_ms * 1e3,
                    "slice_pairs": args.mixed_slice, "reserved_slots_per_cu": args.mixed_reserve,
-                   "chunk_points": args.served_chunk or "whole scan",
+                   "chunk_points": "whole scan (bulk class)",
                    "tile": (None if not args.served_tile else
                             {"particles_per_wavefront": args.served_tile[0],
                              "points_per_lane": args.served_tile[1]})},
@@ -743,8 +745,6 @@ def main():
     ap.add_argument("--served-tile", type=int, nargs=2, default=None, metavar=("K", "P"),
                     help="campus, serving: tile of the served evaluation (campus: the bulk class; serving: the "
                          "tracking class), particles per wavefront and points per lane (default: the server's)")
-    ap.add_argument("--served-chunk", type=int, default=2048,
-                    help="campus: scan points per item of the served evaluation (0 = the whole scan)")
     ap.add_argument("--mixed-small-maps", action="store_true",
                     help="mixed: one 51.2 m building per map instead of a 12 x 11-tile campus")
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
