@@ -123,6 +123,19 @@ inline size_t lds_bytes(const DevMap& d, bool rank)
     return total > reach ? total : reach;
 }
 
+// Instances held to eight wavefronts per SIMD: the compiler then fits them
+// in 80 scalar registers (parking a few wavefront-uniform values in
+// vector-register lanes), and with only the rank table's used words in LDS
+// eight workgroups fit on a CU; left to itself it takes 106 scalar
+// registers, which allow seven.  On the global-localisation workload:
+// k_eval 4.12 -> 3.87 ms (1 particle x 2 points per wavefront iteration),
+// 4.08 -> 3.86 ms (2 particles x 1 point, the second particle's matrix in
+// vector registers).  Wider tiles, the 64-bit form with two particles and
+// the float form spill to memory under that limit and keep the compiler's
+// choice.
+template <int K, int kP, bool kWide, bool kF32>
+constexpr bool kEightWaves = !kF32 && (K == 1 ? kP <= 2 : (K == 2 && kP == 1 && !kWide));
+
 // One wavefront: K particles (pose matrices wavefront-uniform) x the points
 // [j0, j1) of one chunk.  Each iteration covers 64*kP consecutive points,
 // lane l taking points base + l + 64u (u < kP), so kP independent gather
@@ -155,10 +168,19 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const uint32_t p = active ? min(p0 + k, P - 1) : 0u;
-        // wavefront-uniform: held in scalar registers
+        // wavefront-uniform: held in scalar registers, but in the
+        // eight-wavefront instances the second particle's in vector
+        // registers (the empty asm hides the uniformity), which keeps the
+        // scalar registers within that budget
 #pragma unroll
-        for (int e = 0; e < 12; ++e)
-            m[k][e] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L[(size_t)p * 12 + e])));
+        for (int e = 0; e < 12; ++e) {
+            float v = L[(size_t)p * 12 + e];
+            if (k == 1 && kEightWaves<K, kP, kWide, kF32>)
+                asm volatile("" : "+v"(v));
+            else
+                v = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+            m[k][e] = v;
+        }
     }
     float acc[K];
 #pragma unroll
@@ -375,18 +397,6 @@ __device__ __forceinline__ void eval_wave(const DevMap& map, const LdsTables& t,
     }
 }
 
-// Instances with one particle per wavefront and integer voxel codes are
-// held to eight wavefronts per SIMD: the compiler then fits them in 80
-// scalar registers (parking a few wavefront-uniform values in vector-register
-// lanes), and with only the rank table's used words in LDS eight
-// workgroups fit on a CU; left to itself it takes 106 scalar registers,
-// which allow seven.  On the global-localisation workload: k_eval 4.12 ->
-// 3.84 ms.  Two particles per wavefront (24 matrix entries in scalar
-// registers) and the float form spill to memory under that limit and run
-// slower, so they keep the compiler's choice.
-template <int K, bool kF32>
-constexpr bool kEightWaves = K == 1 && !kF32;
-
 #define TSL_EVAL_PARAMS                                                                                    \
     DevMap map, const float* __restrict__ L, const uint32_t* __restrict__ count,                           \
         const float4* __restrict__ pts, uint32_t N, uint32_t chunk_len, uint32_t chunks, float a_hit,      \
@@ -435,7 +445,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
 template <int K, int kP, int kMode, bool kU24, bool kBrick, bool kRank, bool kWide, bool kF32>
 constexpr auto eval_kernel()
 {
-    if constexpr (kEightWaves<K, kF32>)
+    if constexpr (kEightWaves<K, kP, kWide, kF32>)
         return &k_eval8<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>;
     else
         return &k_eval<K, kP, kMode, kU24, kBrick, kRank, kWide, kF32>;
@@ -514,7 +524,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
 template <int K, int kP, bool kU24, bool kBrick, bool kRank, bool kWide, bool kF32>
 constexpr auto batch_kernel()
 {
-    if constexpr (kEightWaves<K, kF32>)
+    if constexpr (kEightWaves<K, kP, kWide, kF32>)
         return &k_eval_batch8<K, kP, kU24, kBrick, kRank, kWide, kF32>;
     else
         return &k_eval_batch<K, kP, kU24, kBrick, kRank, kWide, kF32>;
