@@ -103,9 +103,13 @@ VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 L2_FILE = os.path.join(ROOT, "profiles", "k_eval_global_localisation_l2.md")
 
-# the evaluation tile the default run uses (the autotuner's pick on this
-# workload); the counter profiles under profiles/ are taken with it
-PINNED_TILE = {"order": 1, "k": 1, "p": 2, "chunk": 0, "lookup": -1}
+# the evaluation tile the default run uses (the fastest measured on this
+# workload: 2 particles x 1 point per lane, at eight wavefronts per SIMD);
+# the counter profiles under profiles/ are taken with it
+PINNED_TILE = {"order": 1, "k": 2, "p": 1, "chunk": 0, "lookup": -1}
+# the campus workload's (64-bit map form, whose instances with two particles
+# per wavefront keep seven wavefronts per SIMD)
+CAMPUS_TILE = {"order": 1, "k": 1, "p": 2, "chunk": 0, "lookup": -1}
 
 
 def tile_key(t):
@@ -781,13 +785,14 @@ def main():
     tile_flags = (args.tile_k, args.tile_p, args.lookup, args.chunk, args.order)
     if args.workload == "room" and not args.no_autotune and all(v is None for v in tile_flags):
         args.autotune = True
+    default_tile = CAMPUS_TILE if args.workload == "campus" else PINNED_TILE
     for name, key in (("tile_k", "k"), ("tile_p", "p"), ("lookup", "lookup"), ("chunk", "chunk")):
         if getattr(args, name) is None:
-            setattr(args, name, PINNED_TILE[key])
+            setattr(args, name, default_tile[key])
     if args.order is None:
         # the room's 1,000 particles gain nothing from the spatial order's
         # sort kernels; the automatic choice takes the caller's order there
-        args.order = -1 if args.workload == "room" else PINNED_TILE["order"]
+        args.order = -1 if args.workload == "room" else default_tile["order"]
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "

@@ -32,7 +32,7 @@ def test_profile_values_only_for_their_tile(bench):
     pinned = dict(bench.PINNED_TILE)
     traffic = bench._profile_value(bench.TRAFFIC_FILE, "traffic_bytes_per_launch", pinned)
     assert traffic is not None and traffic > 1e9
-    other = dict(pinned, k=2)
+    other = dict(pinned, k=3 - pinned["k"])
     assert bench._profile_value(bench.TRAFFIC_FILE, "traffic_bytes_per_launch", other) is None
     other = dict(pinned, order=0)
     assert bench._profile_value(bench.TRAFFIC_FILE, "traffic_bytes_per_launch", other) is None
@@ -43,7 +43,7 @@ def test_every_tile_line_is_a_tile_key(bench):
     """Every committed summary that names a tile names it in the form the
     bench compares against, and the keyed figures parse as numbers."""
     prof = os.path.join(ROOT, "profiles")
-    keys = {bench.tile_key(dict(bench.PINNED_TILE))}
+    keys = {bench.tile_key(dict(bench.PINNED_TILE)), bench.tile_key(dict(bench.CAMPUS_TILE))}
     seen = 0
     for name in sorted(os.listdir(prof)):
         with open(os.path.join(prof, name)) as fh:
