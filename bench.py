@@ -75,13 +75,20 @@ def host_cores():
 
 
 def _cpu_model():
-    """Model name as lscpu reports it (or /proc/cpuinfo)."""
+    """Model name as lscpu reports it (or /proc/cpuinfo), with the CPU
+    family, model and stepping numbers, which identify the part where a
+    virtual machine shortens the name."""
     try:
         import subprocess
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        fields = {}
         for line in out.splitlines():
-            if line.startswith("Model name:"):
-                return line.split(":", 1)[1].strip()
+            key, sep, val = line.partition(":")
+            if sep and key.strip() not in fields:
+                fields[key.strip()] = val.strip()
+        if "Model name" in fields:
+            ids = [f"{k.lower()} {fields[k]}" for k in ("CPU family", "Model", "Stepping") if fields.get(k)]
+            return fields["Model name"] + (f" ({', '.join(ids)})" if ids else "")
     except (OSError, subprocess.SubprocessError):
         pass
     try:
