@@ -213,6 +213,15 @@ def test_empty_scan_returns_default_pose(tsl, room):
     assert np.array_equal(parts, room.particles)
 
 
+def test_no_particles_raises(tsl, room):
+    # a non-empty scan over an empty particle set leaves the weight sum at
+    # zero, which the reference rejects (src/cuda/cuda_evaluator.cu:366-368,
+    # src/evaluation/tsdf_evaluator.cpp:159-161)
+    ev = tsl.TSDFEvaluator(room.map)
+    with pytest.raises(RuntimeError, match="No particle is valid!"):
+        ev.evaluate(np.zeros((0, room.particles.shape[1]), np.float32), room.points, room.tf)
+
+
 def test_zero_weight_sum_raises(tsl, room):
     ev = tsl.TSDFEvaluator(room.map, a_hit=0.0, a_range=0.0, a_max=0.0)
     parts = room.particles.copy()
